@@ -1,0 +1,208 @@
+/*
+ * rewl_c.h -- C ABI of the MI355X Replica-Exchange Wang-Landau engine.
+ *
+ * The reference (meese-wj/REWL_Simulator) has no runtime plug-in boundary: its
+ * model is chosen at compile time (src/Hamiltonians/model_hamiltonians.hpp:6-40)
+ * and one MPI rank drives one walker (src/Simulation/rewl_simulation.hpp:69-108).
+ * This header is the boundary that replaces that arrangement: one context owns
+ * every walker of one or more REWL systems on one GPU, and the host talks to
+ * the device only through the functions below.  Plain pointers and sizes; no
+ * allocation crosses the boundary (callers own every output buffer); every
+ * call returns an int status (0 = success, <0 = error class) and a
+ * per-thread message is available from rewl_last_error().
+ *
+ * Entry points and the reference interface each one replaces:
+ *   rewl_create            REWL_simulation ctor + glazier::construct_windows +
+ *                          REWL_Walker ctor + import_DoFs/import_disorder
+ *                          (rewl_simulation.hpp:69-108, glazier.hpp:306-351,
+ *                           rewl_walker.hpp:118-139, main.cpp:201-208)
+ *   rewl_adjust_to_range   REWL_Walker::adjust_state_to_range (rewl_walker.hpp:71-115)
+ *   rewl_sweeps            REWL_Walker::wang_landau_walk -> Wang_Landau::
+ *                          wang_landau_sweep/update (rewl_walker.hpp:141-158,
+ *                          wang_landau.hpp:49-120)
+ *   rewl_exchange          REWL_simulation::replica_exchange_update
+ *                          (rewl_simulation.hpp:228-346)
+ *   rewl_check             check_if_window_is_flat + average_and_redistribute_window
+ *                          + reset_histogram + ln f schedule + done vote
+ *                          (rewl_simulation.hpp:458-470, 361-454, 539-671)
+ *   rewl_run               REWL_simulation::simulate main loop (rewl_simulation.hpp:473-691)
+ *   rewl_finalize          final average_and_redistribute_window (rewl_simulation.hpp:687)
+ *   rewl_get_X, rewl_set_X export_logdos / export_energy_bins / DoFs / current_energy
+ *                          (rewl_histograms.hpp:101-106, rewl_walker.hpp:59-65, 187-205)
+ *   rewl_stitch            concatenate_tables_multiple_overlap + normalize_logDoS
+ *                          (concatenate_rewl_tables.hpp:167-305, array_shift.hpp:15-43)
+ *   rewl_thermodynamics    Thermodynamics::calculate_thermodynamics (thermodynamics.hpp:156-236)
+ */
+#ifndef REWL_C_H
+#define REWL_C_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---------------------------------------------------- */
+#define REWL_OK                 0
+#define REWL_ERR_INVALID       -1   /* bad configuration or argument            */
+#define REWL_ERR_DEVICE        -2   /* HIP runtime error                         */
+#define REWL_ERR_NONFINITE     -3   /* ln g became NaN/Inf                       */
+#define REWL_ERR_ITER_CAP      -4   /* run stopped at max_checks, not converged  */
+#define REWL_ERR_NO_DEVICE     -5   /* no GPU visible                            */
+#define REWL_ERR_REPLAY        -6   /* replay stream exhausted                   */
+#define REWL_ERR_RANGE         -7   /* a walker could not be brought into range  */
+
+/* ---- enumerations ----------------------------------------------------- */
+enum rewl_model {
+    REWL_ISING2D = 0,            /* Ising2d (ising2d_hamiltonian.hpp); per-site
+                                    field = RFIM                                  */
+    REWL_ASHKIN_TELLER2D = 1     /* Ashkin_Teller2d (ashkin_teller2d_hamiltonian.hpp);
+                                    per-site field on sigma*tau = RFAT_BAXTER     */
+};
+
+enum rewl_flatness {
+    REWL_FLAT_ALL_VISITED   = 0, /* ONE_OVER_T_ALGORITHM count_flatness:
+                                    every bin visited (rewl_histograms.hpp:115-130)  */
+    REWL_FLAT_RATIO         = 1, /* classic: (max-min)/min <= tol, float arithmetic
+                                    (rewl_histograms.hpp:133-152)                    */
+    REWL_FLAT_MIN_OVER_MEAN = 2  /* min >= tol * mean (the usual WL criterion)        */
+};
+
+enum rewl_schedule {
+    REWL_SCHED_HALVE      = 0,   /* ln f <- ln f / 2 on every flat event          */
+    REWL_SCHED_ONE_OVER_T = 1    /* reference 1/t switch (rewl_simulation.hpp:557-567) */
+};
+
+enum rewl_rng {
+    REWL_RNG_PHILOX  = 0,        /* counter-based Philox4x32-10, positional draws */
+    REWL_RNG_REPLAY  = 1         /* per-walker std::mt19937_64 stream generated on
+                                    the host and consumed on demand, exactly as the
+                                    reference's random_number_generator
+                                    (random_number_generators.hpp:14-59)          */
+};
+
+/* ---- configuration ------------------------------------------------------ */
+typedef struct rewl_system_desc {
+    int32_t model;               /* enum rewl_model                                  */
+    int32_t L;                   /* periodic L x L square lattice, 2 <= L <= 64      */
+    double  J;                   /* nearest-neighbour exchange                        */
+    double  K;                   /* four-spin coupling (Ashkin-Teller only)           */
+    double  h;                   /* uniform field (Ising: on sigma, AT: on sigma*tau) */
+    const double* field;         /* L*L per-site fields, or NULL for the uniform h   */
+    double  e_min;               /* global histogram range [e_min, e_max)            */
+    double  e_max;
+    double  bin_size;
+    int32_t num_windows;
+    int32_t replicas_per_window;
+    double  overlap;             /* window overlap fraction (glazier EQUAL_WINDOWS)  */
+    uint64_t seed;               /* Philox key / base seed of the replay streams     */
+    const uint8_t* exclude_bins; /* per global bin: 1 = excluded from the flatness
+                                    test (unreachable level), or NULL                */
+    const uint64_t* walker_seeds;/* replay mode: one mt19937_64 seed per walker, or
+                                    NULL for the reference's DIFFERENT_SEEDS=OFF
+                                    choice (every walker seeded with 1)              */
+} rewl_system_desc;
+
+typedef struct rewl_params {
+    int32_t flatness_mode;       /* enum rewl_flatness                               */
+    double  flatness;            /* tolerance of the chosen flatness mode            */
+    int32_t schedule;            /* enum rewl_schedule                               */
+    double  lnf_initial;         /* reference: incrementer = 1 (rewl_walker.hpp:21)  */
+    double  lnf_final;           /* done when ln f < lnf_final (rewl_simulation.hpp:594) */
+    int32_t sweeps_per_check;    /* reference: 1000 (rewl_parameters.hpp:27)          */
+    int32_t sweeps_per_exchange; /* reference: 1 (rewl_parameters.hpp:28)             */
+    int32_t reduce_logdos;       /* REDUCE_LOGDOS: ln g -= ln g[0] after a flat event */
+    int32_t rng_mode;            /* enum rewl_rng                                     */
+    int32_t walkers_per_group;   /* lanes used per wavefront (1..64); 0 = autotune    */
+    int32_t exchange;            /* 0 disables replica exchange (independent walkers) */
+    int32_t reserved[7];
+} rewl_params;
+
+/* Per-window bookkeeping, one record per (system, window). */
+typedef struct rewl_window_info {
+    int32_t system;
+    int32_t window;
+    int32_t first_walker;        /* global index of replica 0                         */
+    int32_t num_bins;
+    double  e_min;               /* [e_min, e_max) of the window                      */
+    double  e_max;
+    double  bin_size;
+    double  lnf;                 /* current ln f of the window                        */
+    int64_t iteration;           /* flat events so far + 1                            */
+    int64_t sweeps_in_iteration;
+    int32_t done;                /* ln f < lnf_final                                  */
+    int32_t one_over_t;          /* 1/t schedule engaged                              */
+} rewl_window_info;
+
+typedef struct rewl_stats {
+    uint64_t sweeps;             /* sweeps per walker so far                          */
+    uint64_t attempts;           /* spin-flip attempts, all walkers                   */
+    uint64_t exchange_calls;
+    uint64_t exchanges_accepted;
+    uint64_t checks;
+    uint64_t flat_events;
+    int32_t  all_done;
+    int32_t  num_walkers;
+    double   kernel_ms_sweep;    /* accumulated sweep-kernel time (HIP events)        */
+    double   kernel_ms_other;    /* exchange + check kernels                          */
+} rewl_stats;
+
+typedef struct rewl_ctx rewl_ctx;
+
+/* ---- library / device --------------------------------------------------- */
+const char* rewl_last_error(void);
+int rewl_device_count(int* count);
+const char* rewl_version(void);
+
+/* ---- lifetime ------------------------------------------------------------- */
+int rewl_create(const rewl_system_desc* systems, int32_t num_systems,
+                const rewl_params* params, rewl_ctx** out);
+int rewl_destroy(rewl_ctx* ctx);
+
+/* ---- simulation steps ----------------------------------------------------- */
+int rewl_adjust_to_range(rewl_ctx* ctx);
+int rewl_sweeps(rewl_ctx* ctx, int32_t num_sweeps);
+int rewl_exchange(rewl_ctx* ctx);
+int rewl_check(rewl_ctx* ctx);
+/* One "check cycle" = sweeps_per_check sweeps with an exchange after every
+ * sweeps_per_exchange sweeps, then rewl_check.  Runs until every window is
+ * done or max_checks cycles have run (REWL_ERR_ITER_CAP). */
+int rewl_run(rewl_ctx* ctx, int64_t max_checks);
+int rewl_finalize(rewl_ctx* ctx);
+int rewl_synchronize(rewl_ctx* ctx);
+
+/* ---- queries ------------------------------------------------------------- */
+int rewl_num_walkers(const rewl_ctx* ctx, int32_t* num_walkers);
+int rewl_num_windows(const rewl_ctx* ctx, int32_t* num_windows);
+int rewl_get_window_info(rewl_ctx* ctx, int32_t window_index, rewl_window_info* out);
+int rewl_get_stats(rewl_ctx* ctx, rewl_stats* out);
+int rewl_get_logdos(rewl_ctx* ctx, int32_t walker, double* out, int32_t n);
+int rewl_get_counts(rewl_ctx* ctx, int32_t walker, uint32_t* out, int32_t n);
+int rewl_get_visits(rewl_ctx* ctx, int32_t walker, uint64_t* out, int32_t n);
+int rewl_get_spins(rewl_ctx* ctx, int32_t walker, int8_t* out, int32_t n);
+int rewl_get_energy(rewl_ctx* ctx, int32_t walker, double* energy);
+int rewl_get_flavor(rewl_ctx* ctx, int32_t walker, int32_t* flavor);
+int rewl_get_rng_cursor(rewl_ctx* ctx, int32_t walker, uint64_t* cursor);
+
+/* ---- state import (warm start, tests) ------------------------------------- */
+int rewl_set_logdos(rewl_ctx* ctx, int32_t walker, const double* lng, int32_t n);
+int rewl_set_window_lnf(rewl_ctx* ctx, int32_t window_index, double lnf);
+int rewl_set_spins(rewl_ctx* ctx, int32_t walker, const int8_t* spins, int32_t n);
+
+/* ---- results path ---------------------------------------------------------- */
+/* Stitch the replica-0 ln g of every window of one system and normalise it to
+ * states_per_site^N (full spectrum) or half of it when e_max == 0.  Writes at
+ * most capacity entries; *n_out receives the curve length. */
+int rewl_stitch(rewl_ctx* ctx, int32_t system, double states_per_site,
+                double* energies, double* logdos, int32_t capacity, int32_t* n_out);
+/* Canonical thermodynamics from (E, ln g) on the GPU, FP64, trapezoid weights:
+ * out[t*5 + {0..4}] = free energy, U, U^2, S, C per site at T = Tmin + dT*t. */
+int rewl_thermodynamics(const double* energies, const double* logdos, int32_t num_bins,
+                        int32_t system_size, double t_min, double t_max, int32_t num_t,
+                        double* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* REWL_C_H */

@@ -1,0 +1,260 @@
+"""ctypes wrapper of liboracle.so, the CPU restatement of the reference.
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by the smoke test and by the
+CPU-baseline leg of bench.py, always as the checker or the CPU baseline,
+never as a path of the product.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_SO = os.path.join(HERE, "liboracle.so")
+
+_lib = None
+
+
+def load(path=ORACLE_SO, build_if_missing=True):
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path) and build_if_missing:
+        subprocess.run(["make", "-C", HERE, "liboracle.so"], check=True, stdout=subprocess.DEVNULL)
+    lib = C.CDLL(path)
+    P = C.c_void_p
+    sig = {
+        "ora_last_error": ([], C.c_char_p),
+        "ora_destroy": ([P], C.c_int),
+        "ora_adjust_to_range": ([P], C.c_int),
+        "ora_sweeps": ([P, C.c_int32], C.c_int),
+        "ora_exchange": ([P], C.c_int),
+        "ora_check": ([P, C.POINTER(C.c_int32)], C.c_int),
+        "ora_run": ([P, C.c_int64, C.POINTER(C.c_int32)], C.c_int),
+        "ora_finalize": ([P], C.c_int),
+        "ora_num_walkers": ([P], C.c_int32),
+        "ora_walker_layout": ([P, C.c_int32] + [C.POINTER(C.c_int32)] * 4, C.c_int),
+        "ora_window_geometry": ([P, C.c_int32, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                 C.POINTER(C.c_double), C.POINTER(C.c_int32)], C.c_int),
+        "ora_window_state": ([P, C.c_int32, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_int64),
+                              C.POINTER(C.c_int64), C.POINTER(C.c_int32), C.POINTER(C.c_int32)], C.c_int),
+        "ora_get_logdos": ([P, C.c_int32, C.POINTER(C.c_double)], C.c_int),
+        "ora_get_counts": ([P, C.c_int32, C.POINTER(C.c_uint64)], C.c_int),
+        "ora_get_visits": ([P, C.c_int32, C.POINTER(C.c_uint64)], C.c_int),
+        "ora_get_spins": ([P, C.c_int32, C.POINTER(C.c_int8)], C.c_int),
+        "ora_get_state": ([P, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_int32),
+                           C.POINTER(C.c_uint64)], C.c_int),
+        "ora_recalculated_energy": ([P, C.c_int32, C.POINTER(C.c_double)], C.c_int),
+        "ora_set_logdos": ([P, C.c_int32, C.POINTER(C.c_double)], C.c_int),
+        "ora_set_window_lnf": ([P, C.c_int32, C.c_int32, C.c_double], C.c_int),
+        "ora_set_spins": ([P, C.c_int32, C.POINTER(C.c_int8)], C.c_int),
+        "ora_stats": ([P, C.POINTER(C.c_uint64)], None),
+        "ora_neighbors": ([C.c_int32, C.c_int32, C.POINTER(C.c_int32)], C.c_int),
+        "ora_philox4x32_10": ([C.POINTER(C.c_uint32)] * 3, None),
+        "ora_shuffle_mt": ([C.c_uint64, C.c_int32, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_uint64)], C.c_int),
+        "ora_mt19937_64": ([C.c_uint64, C.c_int64, C.POINTER(C.c_uint64)], None),
+        "ora_stitch": ([P, C.c_int32, C.c_double, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                        C.c_int32, C.POINTER(C.c_int32)], C.c_int),
+        "ora_thermodynamics": ([C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int32, C.c_int32, C.c_double,
+                                C.c_double, C.c_int32, C.POINTER(C.c_double)], C.c_int),
+        "ora_bench_sweeps": ([P, C.c_int32, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_uint64)], C.c_int),
+    }
+    for name, (a, r) in sig.items():
+        f = getattr(lib, name)
+        f.argtypes = a
+        f.restype = r
+    lib.ora_create.restype = C.c_int
+    lib.ora_energy_of.restype = C.c_int
+    _lib = lib
+    return lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class Oracle:
+    """The CPU restatement driven through the same system/params dicts as the engine."""
+
+    def __init__(self, systems, params):
+        from rewl_simulator_amd import abi
+        from rewl_simulator_amd.engine import _desc_array, _params
+        self.lib = load()
+        self._keep = []
+        descs = _desc_array([dict(s) for s in systems], self._keep)
+        pp = _params(dict(params))
+        ctx = C.c_void_p()
+        rc = self.lib.ora_create(descs, len(systems), C.byref(pp), C.byref(ctx))
+        if rc:
+            raise RuntimeError(self.lib.ora_last_error().decode())
+        self.ctx = ctx
+        self.systems = systems
+        self.num_walkers = self.lib.ora_num_walkers(self.ctx)
+        self.abi = abi
+
+    def close(self):
+        if self.ctx:
+            self.lib.ora_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc):
+        if rc not in (0, self.abi.REWL_ERR_ITER_CAP):
+            raise RuntimeError(self.lib.ora_last_error().decode())
+        return rc
+
+    def adjust_to_range(self):
+        self._chk(self.lib.ora_adjust_to_range(self.ctx))
+
+    def sweeps(self, n):
+        self._chk(self.lib.ora_sweeps(self.ctx, int(n)))
+
+    def exchange(self):
+        self._chk(self.lib.ora_exchange(self.ctx))
+
+    def check(self):
+        d = C.c_int32()
+        self._chk(self.lib.ora_check(self.ctx, C.byref(d)))
+        return bool(d.value)
+
+    def run(self, max_checks):
+        d = C.c_int32()
+        self._chk(self.lib.ora_run(self.ctx, int(max_checks), C.byref(d)))
+        return bool(d.value)
+
+    def finalize(self):
+        self._chk(self.lib.ora_finalize(self.ctx))
+
+    def layout(self, g):
+        s, w, r, n = C.c_int32(), C.c_int32(), C.c_int32(), C.c_int32()
+        self._chk(self.lib.ora_walker_layout(self.ctx, int(g), C.byref(s), C.byref(w), C.byref(r), C.byref(n)))
+        return s.value, w.value, r.value, n.value
+
+    def window_geometry(self, sys_, k):
+        a, b, c, n = C.c_double(), C.c_double(), C.c_double(), C.c_int32()
+        self.lib.ora_window_geometry(self.ctx, sys_, k, C.byref(a), C.byref(b), C.byref(c), C.byref(n))
+        return a.value, b.value, c.value, n.value
+
+    def window_state(self, sys_, k):
+        lnf, it, sw, dn, oot = C.c_double(), C.c_int64(), C.c_int64(), C.c_int32(), C.c_int32()
+        self.lib.ora_window_state(self.ctx, sys_, k, C.byref(lnf), C.byref(it), C.byref(sw), C.byref(dn), C.byref(oot))
+        return dict(lnf=lnf.value, iteration=it.value, sweeps=sw.value, done=dn.value, one_over_t=oot.value)
+
+    def logdos(self, g):
+        n = self.layout(g)[3]
+        out = np.empty(n)
+        self.lib.ora_get_logdos(self.ctx, int(g), _dp(out))
+        return out
+
+    def counts(self, g):
+        n = self.layout(g)[3]
+        out = np.empty(n, dtype=np.uint64)
+        self.lib.ora_get_counts(self.ctx, int(g), out.ctypes.data_as(C.POINTER(C.c_uint64)))
+        return out
+
+    def visits(self, g):
+        n = self.layout(g)[3]
+        out = np.empty(n, dtype=np.uint64)
+        self.lib.ora_get_visits(self.ctx, int(g), out.ctypes.data_as(C.POINTER(C.c_uint64)))
+        return out
+
+    def spins(self, g):
+        s = self.systems[self.layout(g)[0]]
+        n = s["L"] ** 2 * (1 if s["model"] == self.abi.ISING2D else 2)
+        out = np.empty(n, dtype=np.int8)
+        self.lib.ora_get_spins(self.ctx, int(g), out.ctypes.data_as(C.POINTER(C.c_int8)))
+        return out
+
+    def state(self, g):
+        E, M, wh, dr = C.c_double(), C.c_double(), C.c_int32(), C.c_uint64()
+        self.lib.ora_get_state(self.ctx, int(g), C.byref(E), C.byref(M), C.byref(wh), C.byref(dr))
+        return dict(E=E.value, M=M.value, flavor=wh.value, draws=dr.value)
+
+    def recalculated_energy(self, g):
+        E = C.c_double()
+        self.lib.ora_recalculated_energy(self.ctx, int(g), C.byref(E))
+        return E.value
+
+    def set_logdos(self, g, lng):
+        buf = np.ascontiguousarray(lng, dtype=np.float64)
+        self.lib.ora_set_logdos(self.ctx, int(g), _dp(buf))
+
+    def set_window_lnf(self, sys_, k, lnf):
+        self.lib.ora_set_window_lnf(self.ctx, sys_, k, float(lnf))
+
+    def set_spins(self, g, spins):
+        buf = np.ascontiguousarray(spins, dtype=np.int8)
+        self.lib.ora_set_spins(self.ctx, int(g), buf.ctypes.data_as(C.POINTER(C.c_int8)))
+
+    def stats(self):
+        out = (C.c_uint64 * 5)()
+        self.lib.ora_stats(self.ctx, out)
+        return dict(sweeps=out[0], exchange_calls=out[1], exchanges_accepted=out[2], checks=out[3],
+                    flat_events=out[4])
+
+    def stitched(self, system=0, states_per_site=None):
+        cap = 1 << 16
+        E = np.empty(cap)
+        lg = np.empty(cap)
+        n = C.c_int32()
+        sps = 1.0 if states_per_site is None else float(states_per_site)
+        self._chk(self.lib.ora_stitch(self.ctx, int(system), sps, 0 if states_per_site is None else 1, _dp(E),
+                                      _dp(lg), cap, C.byref(n)))
+        return E[:n.value].copy(), lg[:n.value].copy()
+
+    def bench_sweeps(self, sweeps, threads):
+        sec, att = C.c_double(), C.c_uint64()
+        self._chk(self.lib.ora_bench_sweeps(self.ctx, int(sweeps), int(threads), C.byref(sec), C.byref(att)))
+        return sec.value, att.value
+
+
+def neighbors(L, site):
+    out = (C.c_int32 * 4)()
+    load().ora_neighbors(L, site, out)
+    return list(out)
+
+
+def energy_of(system, spins):
+    from rewl_simulator_amd.engine import _desc_array
+    keep = []
+    d = _desc_array([dict(system)], keep)
+    buf = np.ascontiguousarray(spins, dtype=np.int8)
+    E = C.c_double()
+    rc = load().ora_energy_of(d, buf.ctypes.data_as(C.POINTER(C.c_int8)), C.byref(E))
+    assert rc == 0
+    return E.value
+
+
+def philox4x32_10(ctr, key):
+    c = (C.c_uint32 * 4)(*ctr)
+    k = (C.c_uint32 * 2)(*key)
+    o = (C.c_uint32 * 4)()
+    load().ora_philox4x32_10(c, k, o)
+    return list(o)
+
+
+def shuffle_mt(seed, n, base):
+    out = (C.c_int32 * n)()
+    draws = C.c_uint64()
+    load().ora_shuffle_mt(seed, n, base, out, C.byref(draws))
+    return list(out), draws.value
+
+
+def mt19937_64(seed, n):
+    out = np.empty(n, dtype=np.uint64)
+    load().ora_mt19937_64(seed, n, out.ctypes.data_as(C.POINTER(C.c_uint64)))
+    return out
+
+
+def thermodynamics(E, lng, N, tmin=0.01, tmax=50.01, nT=50000):
+    E = np.ascontiguousarray(E, dtype=np.float64)
+    lng = np.ascontiguousarray(lng, dtype=np.float64)
+    out = np.empty((nT, 5))
+    load().ora_thermodynamics(_dp(E), _dp(lng), E.size, int(N), tmin, tmax, nT, _dp(out))
+    return out

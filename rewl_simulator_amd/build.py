@@ -1,0 +1,59 @@
+"""Build the HIP engine (gfx950) and the CPU oracle in-tree.
+
+    python -m rewl_simulator_amd.build            # both
+    python -m rewl_simulator_amd.build engine     # HIP engine only
+"""
+import os
+import subprocess
+import sys
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG)
+CSRC = os.path.join(PKG, "csrc")
+ENGINE_SO = os.path.join(PKG, "librewl_hip.so")
+ORACLE_DIR = os.path.join(ROOT, "oracle")
+ORACLE_SO = os.path.join(ORACLE_DIR, "liboracle.so")
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+HIP_FLAGS = [
+    "--offload-arch=gfx950",
+    "-O3",
+    "-std=c++17",
+    "-fPIC",
+    "-shared",
+    "-ffp-contract=off",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall",
+    "-Wno-unused-result",
+]
+
+
+def build_engine(verbose=False):
+    src = os.path.join(CSRC, "engine.hip")
+    tmp = ENGINE_SO + ".tmp"
+    cmd = [HIPCC, *HIP_FLAGS, "-I", os.path.join(ROOT, "include"), src, "-o", tmp]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(tmp, ENGINE_SO)
+    return ENGINE_SO
+
+
+def build_oracle(verbose=False):
+    cmd = ["make", "-C", ORACLE_DIR, "-B", "liboracle.so"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True, stdout=None if verbose else subprocess.DEVNULL)
+    return ORACLE_SO
+
+
+def main(argv):
+    what = argv[1:] or ["engine", "oracle"]
+    if "engine" in what:
+        print(build_engine(verbose=True))
+    if "oracle" in what:
+        print(build_oracle(verbose=True))
+
+
+if __name__ == "__main__":
+    main(sys.argv)

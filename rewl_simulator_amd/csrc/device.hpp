@@ -1,0 +1,133 @@
+// Device-side data layout and model energetics of the REWL engine.
+//
+// Layout in HBM (all lane-interleaved so that the 64 lanes of a wavefront touch
+// 64 consecutive words):
+//   * walkers are packed into groups of at most 64, one group per wavefront,
+//     all walkers of a group belonging to one system (same L and model);
+//   * lattice word (plane, row y) of the walker in lane j of group k lives at
+//     lat[group.lat_base + (plane * L + y) * 64 + j]; bit x of the word is the
+//     spin at site y*L + x, bit = 1 meaning spin -1 (so all-up is all zeros);
+//   * histogram bin b of that walker lives at [group.hist_base + b * 64 + j] in
+//     lng (FP64), cnt (u32) and visits (u64).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "philox.hpp"
+
+namespace rewl {
+
+constexpr int kLanes = 64;
+
+struct DevSystem {
+    int32_t model, L, N, ndof;
+    int32_t intpath;        // integer energies (clean couplings, no field)
+    int32_t twoJ, twoK;     // 2J, 2K on the integer path
+    int32_t has_field;      // FP64 path: field term present
+    double J, K, h;
+    const double* field;    // per-site field (device), or nullptr for uniform h
+    uint32_t k0, k1;        // Philox key
+    int32_t first_walker, first_window, W, R;
+};
+
+struct DevWindow {
+    double emin, emax, bs;
+    int32_t nbins, sys, first_walker, R;
+    int32_t imin;           // integer path: window minimum
+    int32_t phi_lo, phi_hi; // phi(emin), phi(emax), phi(e) = (int)(e + 0.5)
+    int32_t shift;          // integer path: log2(bin size)
+    int32_t pad;
+    int64_t excl_off;       // offset of this window's bins in the exclusion mask
+};
+
+struct DevGroup {
+    int32_t sys, first_walker, nwalkers, nbins;
+    int64_t hist_base, lat_base;
+};
+
+struct KArgs {
+    const DevSystem* sys;
+    const DevWindow* win;
+    const DevGroup* grp;
+    const int32_t* w_window;
+    const int32_t* w_local;
+    const int64_t* w_hist;
+    const int64_t* w_lat;
+    double* w_E;
+    int32_t* w_flavor;
+    uint64_t* w_cursor;
+    double* win_lnf;
+    int64_t* win_iter;
+    int64_t* win_sweeps;
+    int32_t* win_oot;
+    int32_t* win_done;
+    const uint8_t* exclude;
+    uint64_t* lat;
+    double* lng;
+    uint32_t* cnt;
+    uint64_t* visits;
+    const uint64_t* ring;      // replay streams, ring_cap words per walker
+    const uint64_t* ring_filled;
+    uint64_t ring_cap;
+    int32_t* err;
+    unsigned long long* exch_accepted;
+    int32_t num_walkers, num_windows, num_groups;
+};
+
+// The reference's in-range test adds 0.5 and truncates toward zero
+// (histogram_index.hpp:337-345); for integer energies this maps e -> e + (e<0).
+__device__ __forceinline__ int phi_int(int e) { return e - (e >> 31); }
+
+__device__ __forceinline__ int trunc_half(double e) { return (int)(e + 0.5); }
+
+// Site (x, y) neighbours in the reference order: (x,y-1),(x-1,y),(x+1,y),(x,y+1)
+// (square_2d_grid_helpers.hpp:37-63).
+__device__ __forceinline__ void neighbors(int site, int L, int nb[4]) {
+    const int x = site % L, y = site / L;
+    nb[0] = (y == 0 ? L - 1 : y - 1) * L + x;
+    nb[1] = y * L + (x == 0 ? L - 1 : x - 1);
+    nb[2] = y * L + (x == L - 1 ? 0 : x + 1);
+    nb[3] = (y == L - 1 ? 0 : y + 1) * L + x;
+}
+
+// Spin of plane p at site as +-1 double, read from the packed lattice.
+__device__ __forceinline__ double spin_at(const uint64_t* lat, int64_t base, int L, int plane, int site) {
+    const int x = site % L, y = site / L;
+    const uint64_t w = lat[base + (int64_t)(plane * L + y) * kLanes];
+    return ((w >> x) & 1ull) ? -1.0 : 1.0;
+}
+
+__device__ __forceinline__ double field_at(const DevSystem& S, int site) {
+    return S.field ? S.field[site] : S.h;
+}
+
+// Ising2d::local_energy (ising2d_hamiltonian.hpp:162-188), FP64, reference order.
+__device__ __forceinline__ double ising_local_energy_mem(const DevSystem& S, const uint64_t* lat, int64_t base,
+                                                         int site, double sv) {
+    int nb[4];
+    neighbors(site, S.L, nb);
+    double f = field_at(S, site);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) f += S.J * spin_at(lat, base, S.L, 0, nb[k]);
+    return -1. * sv * f;
+}
+
+// Ashkin_Teller2d::local_energy (ashkin_teller2d_hamiltonian.hpp:169-191).
+__device__ __forceinline__ double at_local_energy_mem(const DevSystem& S, const uint64_t* lat, int64_t base,
+                                                      int site, double sg, double ta) {
+    int nb[4];
+    neighbors(site, S.L, nb);
+    double en = 0.;
+    if (S.has_field) en += field_at(S, site) * (sg * ta);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const double sn = spin_at(lat, base, S.L, 0, nb[k]);
+        const double tn = spin_at(lat, base, S.L, 1, nb[k]);
+        en += S.J * sg * sn;
+        en += S.J * ta * tn;
+        en += S.K * sg * ta * sn * tn;
+    }
+    return -en;
+}
+
+}  // namespace rewl

@@ -1,0 +1,900 @@
+// Host side of the REWL engine: the C ABI declared in include/rewl_c.h.
+//
+// One rewl_ctx owns every walker of one or more REWL systems on one GPU.  The
+// host builds the windows (glazier EQUAL_WINDOWS, glazier.hpp:306-351), packs
+// walkers into wavefront groups, uploads the tables, and then drives the
+// device with four kinds of launch that replace the reference's per-rank loop
+// and its MPI traffic: sweep (one lane per walker), exchange (one wavefront
+// per adjacent window pair), check (one wavefront per window) and finalize.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "../../include/rewl_c.h"
+#include "aux_kernels.hpp"
+#include "sweep.hpp"
+
+using namespace rewl;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                      \
+    do {                                                                                   \
+        hipError_t e_ = (expr);                                                            \
+        if (e_ != hipSuccess)                                                              \
+            return fail(REWL_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+template <typename T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    hipError_t alloc(size_t count) {
+        n = count;
+        if (count == 0) return hipSuccess;
+        return hipMalloc((void**)&p, count * sizeof(T));
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+};
+
+struct HostWindow {
+    double emin, emax, bs;
+    int nbins;
+    int sys, first_walker, R;
+};
+
+struct HostSystem {
+    rewl_system_desc d;
+    int N, ndof;
+    bool intpath;
+    int twoJ, twoK;
+    bool has_field;
+    std::vector<double> field;
+    std::vector<uint64_t> walker_seeds;
+    int first_walker, first_window;
+    std::vector<uint8_t> exclude;
+};
+
+// glazier::construct_windows, EQUAL_WINDOWS variant (glazier.hpp:306-351):
+// equal widths, each window start snapped onto the previous window's bin grid,
+// the end capped at the global maximum; windows are half-open [min, max).
+std::vector<HostWindow> make_windows(const rewl_system_desc& d) {
+    std::vector<HostWindow> out((size_t)d.num_windows);
+    const double denom = 1 + (double)(d.num_windows - 1) * (1 - d.overlap);
+    const double size = (d.e_max - d.e_min) / denom;
+    for (int k = 0; k < d.num_windows; ++k) {
+        double lo = d.e_min + (double)k * (1 - d.overlap) * size;
+        double hi = lo + size;
+        if (k != 0) {
+            const double prev = out[k - 1].emin;
+            lo = prev + d.bin_size * (double)((size_t)((lo - prev) / d.bin_size));
+        }
+        const size_t nb = (size_t)std::ceil((hi - lo) / d.bin_size);
+        hi = lo + d.bin_size * (double)nb;
+        while (hi > d.e_max) hi -= d.bin_size;
+        out[k].emin = lo;
+        out[k].emax = hi;
+        out[k].bs = d.bin_size;
+        out[k].nbins = (int)(size_t)((hi - lo) / d.bin_size);
+    }
+    return out;
+}
+
+bool integral(double v) { return std::isfinite(v) && std::floor(v) == v && std::fabs(v) < 1e9; }
+
+}  // namespace
+
+struct rewl_ctx {
+    rewl_params p{};
+    std::vector<HostSystem> sys;
+    std::vector<HostWindow> win;
+    std::vector<DevGroup> groups;
+    std::vector<int> w_window, w_local, w_group, w_lane;
+    std::vector<int64_t> w_hist, w_lat;
+    int num_walkers = 0;
+    size_t hist_total = 0, lat_total = 0;
+    // group lists per launch class: index = model * 2 + intpath
+    std::vector<int32_t> class_groups[4];
+    int class_maxbins[4] = {0, 0, 0, 0};
+    bool class_packed[4] = {false, false, false, false};
+    DevBuf<int32_t> d_class_groups[4];
+    std::vector<int32_t> pairs_even, pairs_odd;
+    DevBuf<int32_t> d_pairs_even, d_pairs_odd, d_flat;
+
+    DevBuf<DevSystem> d_sys;
+    DevBuf<DevWindow> d_win;
+    DevBuf<DevGroup> d_grp;
+    DevBuf<int32_t> d_w_window, d_w_local, d_w_flavor, d_err;
+    DevBuf<int64_t> d_w_hist, d_w_lat, d_win_iter, d_win_sweeps;
+    DevBuf<double> d_w_E, d_win_lnf, d_fields;
+    DevBuf<uint64_t> d_w_cursor, d_lat, d_visits, d_ring, d_ring_filled;
+    DevBuf<int32_t> d_win_oot, d_win_done;
+    DevBuf<uint8_t> d_exclude;
+    DevBuf<double> d_lng;
+    DevBuf<uint32_t> d_cnt;
+    DevBuf<unsigned long long> d_exch;
+
+    // replay streams
+    std::vector<std::mt19937_64> engines;
+    std::vector<uint64_t> filled;
+    uint64_t ring_cap = 0;
+
+    hipStream_t stream = nullptr;
+    int lds_max = 65536;
+    int flush_every = 1;
+    uint64_t sweep_index = 0;
+    int64_t pending_sweeps = 0;
+    bool direction_even = true;
+    rewl_stats stats{};
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_sweep;
+    std::vector<int32_t> done_host;
+    KArgs args{};
+};
+
+namespace {
+
+void destroy_ctx(rewl_ctx* c) {
+    if (!c) return;
+    for (auto& e : c->ev_sweep) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
+    c->ev_sweep.clear();
+    for (auto* b : {&c->d_w_window, &c->d_w_local, &c->d_w_flavor, &c->d_err, &c->d_win_oot, &c->d_win_done,
+                    &c->d_pairs_even, &c->d_pairs_odd, &c->d_flat})
+        b->release();
+    for (auto& b : c->d_class_groups) b.release();
+    c->d_w_hist.release(); c->d_w_lat.release(); c->d_win_iter.release(); c->d_win_sweeps.release();
+    c->d_w_E.release(); c->d_win_lnf.release(); c->d_fields.release();
+    c->d_w_cursor.release(); c->d_lat.release(); c->d_visits.release(); c->d_ring.release(); c->d_ring_filled.release();
+    c->d_exclude.release(); c->d_lng.release(); c->d_cnt.release(); c->d_exch.release();
+    c->d_sys.release(); c->d_win.release(); c->d_grp.release();
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+template <typename T>
+int upload(DevBuf<T>& b, const std::vector<T>& v, hipStream_t s) {
+    HIP_TRY(b.alloc(v.size()));
+    if (!v.empty()) HIP_TRY(hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));
+    return REWL_OK;
+}
+
+int resolve_sweep_events(rewl_ctx* c) {
+    for (auto& e : c->ev_sweep) {
+        HIP_TRY(hipEventSynchronize(e.second));
+        float ms = 0.f;
+        HIP_TRY(hipEventElapsedTime(&ms, e.first, e.second));
+        c->stats.kernel_ms_sweep += ms;
+        (void)hipEventDestroy(e.first);
+        (void)hipEventDestroy(e.second);
+    }
+    c->ev_sweep.clear();
+    return REWL_OK;
+}
+
+int check_err_flag(rewl_ctx* c) {
+    int32_t e = 0;
+    HIP_TRY(hipMemcpyAsync(&e, c->d_err.p, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (e & 1) return fail(REWL_ERR_REPLAY, "replay stream exhausted");
+    if (e & 2) return fail(REWL_ERR_RANGE, "a walker could not be brought into its window");
+    return REWL_OK;
+}
+
+// Make sure every replay stream holds at least ring_cap unconsumed words.
+int replay_topup(rewl_ctx* c) {
+    if (c->p.rng_mode != REWL_RNG_REPLAY) return REWL_OK;
+    const int W = c->num_walkers;
+    std::vector<uint64_t> cur((size_t)W);
+    HIP_TRY(hipMemcpyAsync(cur.data(), c->d_w_cursor.p, (size_t)W * sizeof(uint64_t), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    const uint64_t C = c->ring_cap;
+    std::vector<uint64_t> tmp;
+    for (int g = 0; g < W; ++g) {
+        const uint64_t target = cur[g] + C;
+        uint64_t& f = c->filled[g];
+        while (f < target) {
+            const uint64_t pos = f & (C - 1);
+            const uint64_t len = std::min<uint64_t>(target - f, C - pos);
+            tmp.resize(len);
+            for (uint64_t k = 0; k < len; ++k) tmp[k] = c->engines[g]();
+            HIP_TRY(hipMemcpyAsync(c->d_ring.p + (size_t)g * C + pos, tmp.data(), len * sizeof(uint64_t),
+                                   hipMemcpyHostToDevice, c->stream));
+            HIP_TRY(hipStreamSynchronize(c->stream));
+            f += len;
+        }
+    }
+    HIP_TRY(hipMemcpyAsync(c->d_ring_filled.p, c->filled.data(), (size_t)W * sizeof(uint64_t), hipMemcpyHostToDevice,
+                           c->stream));
+    return REWL_OK;
+}
+
+template <int MODEL, bool INTP>
+int launch_sweep_class(rewl_ctx* c, int cls, int nsweeps) {
+    const int ng = (int)c->class_groups[cls].size();
+    if (ng == 0) return REWL_OK;
+    const bool packed = c->class_packed[cls];
+    const int nb = c->class_maxbins[cls];
+    const size_t lds = (size_t)nb * kLanes * sizeof(double) +
+                       (packed ? (size_t)((nb + 1) / 2) : (size_t)nb) * kLanes * sizeof(uint32_t);
+    const bool replay = c->p.rng_mode == REWL_RNG_REPLAY;
+    const dim3 grid((unsigned)ng), block(kLanes);
+    const int32_t* gl = c->d_class_groups[cls].p;
+    const int fe = c->flush_every;
+    void (*kern)(KArgs, const int32_t*, int, int, uint64_t, int);
+    if (replay) kern = packed ? k_sweep<MODEL, INTP, true, true> : k_sweep<MODEL, INTP, true, false>;
+    else kern = packed ? k_sweep<MODEL, INTP, false, true> : k_sweep<MODEL, INTP, false, false>;
+    HIP_TRY(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL(kern, grid, block, lds, c->stream, c->args, gl, ng, nsweeps, c->sweep_index, fe);
+    HIP_TRY(hipGetLastError());
+    return REWL_OK;
+}
+
+int launch_sweeps(rewl_ctx* c, int nsweeps) {
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    HIP_TRY(hipEventRecord(e0, c->stream));
+    int rc;
+    if ((rc = launch_sweep_class<0, false>(c, 0, nsweeps))) return rc;
+    if ((rc = launch_sweep_class<0, true>(c, 1, nsweeps))) return rc;
+    if ((rc = launch_sweep_class<1, false>(c, 2, nsweeps))) return rc;
+    if ((rc = launch_sweep_class<1, true>(c, 3, nsweeps))) return rc;
+    HIP_TRY(hipEventRecord(e1, c->stream));
+    c->ev_sweep.emplace_back(e0, e1);
+    if (c->ev_sweep.size() > 256) return resolve_sweep_events(c);
+    return REWL_OK;
+}
+
+int max_ndof(const rewl_ctx* c) {
+    int m = 1;
+    for (auto& s : c->sys) m = std::max(m, s.ndof);
+    return m;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* rewl_last_error(void) { return g_err.c_str(); }
+const char* rewl_version(void) { return "rewl-mi355x 0.1"; }
+
+int rewl_device_count(int* count) {
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess) n = 0;
+    if (count) *count = n;
+    return REWL_OK;
+}
+
+int rewl_create(const rewl_system_desc* systems, int32_t num_systems, const rewl_params* params, rewl_ctx** out) {
+    if (!systems || num_systems <= 0 || !params || !out) return fail(REWL_ERR_INVALID, "null argument");
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+        return fail(REWL_ERR_NO_DEVICE, "no HIP device visible: the REWL engine runs only on the GPU");
+    const rewl_params& P = *params;
+    if (P.sweeps_per_check <= 0 || P.sweeps_per_exchange <= 0 || P.lnf_initial <= 0 ||
+        (P.flatness_mode < 0 || P.flatness_mode > 2) || (P.schedule < 0 || P.schedule > 1) ||
+        (P.rng_mode < 0 || P.rng_mode > 1) || P.walkers_per_group < 0 || P.walkers_per_group > kLanes)
+        return fail(REWL_ERR_INVALID, "invalid rewl_params");
+
+    rewl_ctx* c = new rewl_ctx();
+    c->p = P;
+    auto bail = [&](int code) { destroy_ctx(c); return code; };
+
+    // ---- systems and windows ------------------------------------------------
+    int walker = 0;
+    for (int si = 0; si < num_systems; ++si) {
+        const rewl_system_desc& d = systems[si];
+        if (d.model != REWL_ISING2D && d.model != REWL_ASHKIN_TELLER2D) return bail(fail(REWL_ERR_INVALID, "unknown model"));
+        if (d.L < 2 || d.L > 64) return bail(fail(REWL_ERR_INVALID, "L must lie in [2, 64]"));
+        if (d.num_windows < 1 || d.replicas_per_window < 1 || d.replicas_per_window > kMaxReplicas)
+            return bail(fail(REWL_ERR_INVALID, "num_windows >= 1 and 1 <= replicas_per_window <= 256 required"));
+        if (!(d.bin_size > 0) || !(d.e_max > d.e_min) || d.overlap < 0 || d.overlap >= 1)
+            return bail(fail(REWL_ERR_INVALID, "bad energy range, bin size or overlap"));
+        HostSystem S;
+        S.d = d;
+        S.N = d.L * d.L;
+        S.ndof = d.model == REWL_ISING2D ? S.N : 2 * S.N;
+        if (d.field) S.field.assign(d.field, d.field + S.N);
+        S.d.field = nullptr;
+        S.has_field = d.field != nullptr || d.h != 0.0;
+        const double tJ = 2 * d.J, tK = 2 * d.K;
+        S.twoJ = (int)tJ;
+        S.twoK = (int)tK;
+        auto wins = make_windows(d);
+        bool ip = !S.has_field && integral(tJ) && (d.model == REWL_ISING2D || integral(tK));
+        for (auto& w : wins) {
+            if (w.nbins < 1) return bail(fail(REWL_ERR_INVALID, "a window has no bins"));
+            const double lb = std::log2(w.bs);
+            ip = ip && integral(w.emin) && integral(w.emax) && integral(w.bs) && integral(lb);
+        }
+        S.intpath = ip;
+        S.first_walker = walker;
+        S.first_window = (int)c->win.size();
+        const size_t gbins = (size_t)std::llround((d.e_max - d.e_min) / d.bin_size) + 1;
+        S.exclude.assign(gbins, 0);
+        if (d.exclude_bins)
+            for (size_t b = 0; b + 1 < gbins; ++b) S.exclude[b] = d.exclude_bins[b];
+        if (d.walker_seeds) S.walker_seeds.assign(d.walker_seeds, d.walker_seeds + (size_t)d.num_windows * d.replicas_per_window);
+        for (int k = 0; k < d.num_windows; ++k) {
+            HostWindow w = wins[k];
+            w.sys = si;
+            w.R = d.replicas_per_window;
+            w.first_walker = walker + k * d.replicas_per_window;
+            c->win.push_back(w);
+        }
+        for (int k = 0; k < d.num_windows; ++k)
+            for (int r = 0; r < d.replicas_per_window; ++r) {
+                c->w_window.push_back(S.first_window + k);
+                c->w_local.push_back(k * d.replicas_per_window + r);
+            }
+        walker += d.num_windows * d.replicas_per_window;
+        c->sys.push_back(std::move(S));
+    }
+    c->num_walkers = walker;
+    const int W = walker;
+
+    // ---- device limits ------------------------------------------------------
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    int lds = 0;
+    if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess && lds > 0)
+        c->lds_max = lds;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
+        return bail(fail(REWL_ERR_DEVICE, "hipStreamCreate failed"));
+
+    // ---- wavefront groups -----------------------------------------------------
+    const int wpg = P.walkers_per_group > 0 ? P.walkers_per_group : kLanes;
+    c->w_group.assign(W, 0);
+    c->w_lane.assign(W, 0);
+    c->w_hist.assign(W, 0);
+    c->w_lat.assign(W, 0);
+    for (int si = 0; si < (int)c->sys.size(); ++si) {
+        const HostSystem& S = c->sys[si];
+        const int nw = S.d.num_windows * S.d.replicas_per_window;
+        const int planes = S.d.model == REWL_ISING2D ? 1 : 2;
+        for (int off = 0; off < nw; off += wpg) {
+            DevGroup G{};
+            G.sys = si;
+            G.first_walker = S.first_walker + off;
+            G.nwalkers = std::min(wpg, nw - off);
+            int nb = 0;
+            for (int j = 0; j < G.nwalkers; ++j) nb = std::max(nb, c->win[c->w_window[G.first_walker + j]].nbins);
+            G.nbins = nb;
+            G.hist_base = (int64_t)c->hist_total;
+            G.lat_base = (int64_t)c->lat_total;
+            c->hist_total += (size_t)nb * kLanes;
+            c->lat_total += (size_t)planes * S.d.L * kLanes;
+            const int gid = (int)c->groups.size();
+            for (int j = 0; j < G.nwalkers; ++j) {
+                const int g = G.first_walker + j;
+                c->w_group[g] = gid;
+                c->w_lane[g] = j;
+                c->w_hist[g] = G.hist_base + j;
+                c->w_lat[g] = G.lat_base + j;
+            }
+            const int cls = S.d.model * 2 + (S.intpath ? 1 : 0);
+            c->class_groups[cls].push_back(gid);
+            c->class_maxbins[cls] = std::max(c->class_maxbins[cls], nb);
+            c->groups.push_back(G);
+        }
+    }
+    for (int cls = 0; cls < 4; ++cls) {
+        const int nb = c->class_maxbins[cls];
+        const size_t full = (size_t)nb * kLanes * 12;
+        const size_t packed = (size_t)nb * kLanes * 8 + (size_t)((nb + 1) / 2) * kLanes * 4;
+        if (c->class_groups[cls].empty()) continue;
+        if (full <= (size_t)c->lds_max) c->class_packed[cls] = false;
+        else if (packed <= (size_t)c->lds_max) c->class_packed[cls] = true;
+        else return bail(fail(REWL_ERR_INVALID, "window too wide for LDS: " + std::to_string(nb) + " bins"));
+    }
+    c->flush_every = std::max(1, 65535 / (max_ndof(c) + 1));
+
+    // ---- exchange pair lists ---------------------------------------------------
+    for (auto& S : c->sys)
+        for (int k = 0; k + 1 < S.d.num_windows; ++k)
+            (k % 2 == 0 ? c->pairs_even : c->pairs_odd).push_back(S.first_window + k);
+
+    // ---- device tables ------------------------------------------------------
+    int rc;
+    std::vector<double> fields;
+    std::vector<DevSystem> ds;
+    std::vector<size_t> field_off;
+    for (auto& S : c->sys) {
+        field_off.push_back(fields.size());
+        fields.insert(fields.end(), S.field.begin(), S.field.end());
+    }
+    if ((rc = upload(c->d_fields, fields, c->stream))) return bail(rc);
+    std::vector<uint8_t> excl;
+    std::vector<DevWindow> dw;
+    for (int si = 0; si < (int)c->sys.size(); ++si) {
+        const HostSystem& S = c->sys[si];
+        DevSystem D{};
+        D.model = S.d.model; D.L = S.d.L; D.N = S.N; D.ndof = S.ndof;
+        D.intpath = S.intpath; D.twoJ = S.twoJ; D.twoK = S.twoK; D.has_field = S.has_field;
+        D.J = S.d.J; D.K = S.d.K; D.h = S.d.h;
+        D.field = S.field.empty() ? nullptr : c->d_fields.p + field_off[si];
+        D.k0 = (uint32_t)S.d.seed; D.k1 = (uint32_t)(S.d.seed >> 32);
+        D.first_walker = S.first_walker; D.first_window = S.first_window;
+        D.W = S.d.num_windows; D.R = S.d.replicas_per_window;
+        ds.push_back(D);
+        for (int k = 0; k < S.d.num_windows; ++k) {
+            const HostWindow& w = c->win[S.first_window + k];
+            DevWindow x{};
+            x.emin = w.emin; x.emax = w.emax; x.bs = w.bs; x.nbins = w.nbins;
+            x.sys = si; x.first_walker = w.first_walker; x.R = w.R;
+            x.phi_lo = (int)(w.emin + 0.5);
+            x.phi_hi = (int)(w.emax + 0.5);
+            if (S.intpath) {
+                x.imin = (int)w.emin;
+                x.shift = (int)std::llround(std::log2(w.bs));
+            }
+            x.excl_off = (int64_t)excl.size();
+            const size_t off = (size_t)std::llround((w.emin - S.d.e_min) / S.d.bin_size);
+            for (int b = 0; b < w.nbins; ++b) excl.push_back(off + b < S.exclude.size() ? S.exclude[off + b] : 0);
+            dw.push_back(x);
+        }
+    }
+    if ((rc = upload(c->d_sys, ds, c->stream))) return bail(rc);
+    if ((rc = upload(c->d_win, dw, c->stream))) return bail(rc);
+    if ((rc = upload(c->d_grp, c->groups, c->stream))) return bail(rc);
+    if ((rc = upload(c->d_exclude, excl, c->stream))) return bail(rc);
+    if ((rc = upload(c->d_w_window, c->w_window, c->stream))) return bail(rc);
+    if ((rc = upload(c->d_w_local, c->w_local, c->stream))) return bail(rc);
+    if ((rc = upload(c->d_w_hist, c->w_hist, c->stream))) return bail(rc);
+    if ((rc = upload(c->d_w_lat, c->w_lat, c->stream))) return bail(rc);
+    for (int cls = 0; cls < 4; ++cls)
+        if ((rc = upload(c->d_class_groups[cls], c->class_groups[cls], c->stream))) return bail(rc);
+    if ((rc = upload(c->d_pairs_even, c->pairs_even, c->stream))) return bail(rc);
+    if ((rc = upload(c->d_pairs_odd, c->pairs_odd, c->stream))) return bail(rc);
+
+    const size_t nwin = c->win.size();
+    std::vector<double> lnf(nwin, P.lnf_initial);
+    std::vector<int64_t> iter(nwin, 1), zeros64(nwin, 0);
+    std::vector<int32_t> zeros32(nwin, 0), flav(W, 0);
+    std::vector<double> E0(W, 0.0);
+    std::vector<uint64_t> cur0(W, 0);
+    if ((rc = upload(c->d_win_lnf, lnf, c->stream))) return bail(rc);
+    if ((rc = upload(c->d_win_iter, iter, c->stream))) return bail(rc);
+    if ((rc = upload(c->d_win_sweeps, zeros64, c->stream))) return bail(rc);
+    if ((rc = upload(c->d_win_oot, zeros32, c->stream))) return bail(rc);
+    if ((rc = upload(c->d_win_done, zeros32, c->stream))) return bail(rc);
+    if ((rc = upload(c->d_flat, zeros32, c->stream))) return bail(rc);
+    if ((rc = upload(c->d_w_flavor, flav, c->stream))) return bail(rc);
+    if ((rc = upload(c->d_w_E, E0, c->stream))) return bail(rc);
+    if ((rc = upload(c->d_w_cursor, cur0, c->stream))) return bail(rc);
+    if ((rc = upload(c->d_err, std::vector<int32_t>(1, 0), c->stream))) return bail(rc);
+    if ((rc = upload(c->d_exch, std::vector<unsigned long long>(1, 0ull), c->stream))) return bail(rc);
+    c->done_host.assign(nwin, 0);
+
+    if (c->d_lat.alloc(c->lat_total) != hipSuccess || c->d_lng.alloc(c->hist_total) != hipSuccess ||
+        c->d_cnt.alloc(c->hist_total) != hipSuccess || c->d_visits.alloc(c->hist_total) != hipSuccess)
+        return bail(fail(REWL_ERR_DEVICE, "hipMalloc failed for lattice/histograms"));
+    if (hipMemsetAsync(c->d_lat.p, 0, c->lat_total * 8, c->stream) != hipSuccess ||
+        hipMemsetAsync(c->d_lng.p, 0, c->hist_total * 8, c->stream) != hipSuccess ||
+        hipMemsetAsync(c->d_cnt.p, 0, c->hist_total * 4, c->stream) != hipSuccess ||
+        hipMemsetAsync(c->d_visits.p, 0, c->hist_total * 8, c->stream) != hipSuccess)
+        return bail(fail(REWL_ERR_DEVICE, "hipMemset failed"));
+
+    // ---- replay streams ---------------------------------------------------------
+    if (P.rng_mode == REWL_RNG_REPLAY) {
+        uint64_t need = (uint64_t)max_ndof(c) * 16 + 8 * kMaxReplicas + 1024;
+        uint64_t C = 1;
+        while (C < 2 * need) C <<= 1;
+        c->ring_cap = C;
+        if (c->d_ring.alloc((size_t)W * C) != hipSuccess) return bail(fail(REWL_ERR_DEVICE, "hipMalloc replay ring"));
+        c->engines.resize(W);
+        c->filled.assign(W, 0);
+        for (auto& S : c->sys)
+            for (int j = 0; j < S.d.num_windows * S.d.replicas_per_window; ++j)
+                c->engines[S.first_walker + j].seed(S.walker_seeds.empty() ? (uint64_t)1 : S.walker_seeds[j]);
+        if ((rc = upload(c->d_ring_filled, c->filled, c->stream))) return bail(rc);
+    } else {
+        c->ring_cap = 1;
+        if ((rc = upload(c->d_ring_filled, std::vector<uint64_t>(1, 0), c->stream))) return bail(rc);
+        if (c->d_ring.alloc(1) != hipSuccess) return bail(fail(REWL_ERR_DEVICE, "hipMalloc"));
+    }
+
+    KArgs& a = c->args;
+    a.sys = c->d_sys.p; a.win = c->d_win.p; a.grp = c->d_grp.p;
+    a.w_window = c->d_w_window.p; a.w_local = c->d_w_local.p; a.w_hist = c->d_w_hist.p; a.w_lat = c->d_w_lat.p;
+    a.w_E = c->d_w_E.p; a.w_flavor = c->d_w_flavor.p; a.w_cursor = c->d_w_cursor.p;
+    a.win_lnf = c->d_win_lnf.p; a.win_iter = c->d_win_iter.p; a.win_sweeps = c->d_win_sweeps.p;
+    a.win_oot = c->d_win_oot.p; a.win_done = c->d_win_done.p; a.exclude = c->d_exclude.p;
+    a.lat = c->d_lat.p; a.lng = c->d_lng.p; a.cnt = c->d_cnt.p; a.visits = c->d_visits.p;
+    a.ring = c->d_ring.p; a.ring_filled = c->d_ring_filled.p; a.ring_cap = c->ring_cap;
+    a.err = c->d_err.p; a.exch_accepted = c->d_exch.p;
+    a.num_walkers = W; a.num_windows = (int)nwin; a.num_groups = (int)c->groups.size();
+
+    // initial energies of the all-up lattices (recalculate_state)
+    hipLaunchKernelGGL(k_recalc, dim3((W + 63) / 64), dim3(64), 0, c->stream, c->args);
+    if (hipGetLastError() != hipSuccess) return bail(fail(REWL_ERR_DEVICE, "k_recalc launch failed"));
+    if (hipStreamSynchronize(c->stream) != hipSuccess) return bail(fail(REWL_ERR_DEVICE, "stream sync failed"));
+    c->stats.num_walkers = W;
+    *out = c;
+    return REWL_OK;
+}
+
+int rewl_destroy(rewl_ctx* c) {
+    if (c) {
+        (void)hipStreamSynchronize(c->stream);
+        destroy_ctx(c);
+    }
+    return REWL_OK;
+}
+
+int rewl_adjust_to_range(rewl_ctx* c) {
+    if (!c) return fail(REWL_ERR_INVALID, "null context");
+    int rc;
+    if ((rc = replay_topup(c))) return rc;
+    uint64_t steps = (uint64_t)max_ndof(c) * 4096ull;
+    if (c->p.rng_mode == REWL_RNG_REPLAY)
+        hipLaunchKernelGGL(k_adjust<true>, dim3((c->num_walkers + 63) / 64), dim3(64), 0, c->stream, c->args, steps);
+    else
+        hipLaunchKernelGGL(k_adjust<false>, dim3((c->num_walkers + 63) / 64), dim3(64), 0, c->stream, c->args, steps);
+    HIP_TRY(hipGetLastError());
+    return check_err_flag(c);
+}
+
+int rewl_sweeps(rewl_ctx* c, int32_t n) {
+    if (!c || n < 0) return fail(REWL_ERR_INVALID, "bad arguments");
+    int rc;
+    int done = 0;
+    const int chunk = c->p.rng_mode == REWL_RNG_REPLAY
+                          ? (int)std::max<uint64_t>(1, (c->ring_cap - 8 * kMaxReplicas - 1024) / (uint64_t)max_ndof(c))
+                          : n;
+    while (done < n) {
+        const int k = std::min(chunk, n - done);
+        if ((rc = replay_topup(c))) return rc;
+        if ((rc = launch_sweeps(c, k))) return rc;
+        c->sweep_index += (uint64_t)k;
+        done += k;
+    }
+    c->pending_sweeps += n;
+    c->stats.sweeps += (uint64_t)n;
+    uint64_t att = 0;
+    for (auto& S : c->sys) att += (uint64_t)S.ndof * (uint64_t)S.d.num_windows * (uint64_t)S.d.replicas_per_window;
+    c->stats.attempts += att * (uint64_t)n;
+    if (c->p.rng_mode == REWL_RNG_REPLAY) return check_err_flag(c);
+    return REWL_OK;
+}
+
+int rewl_exchange(rewl_ctx* c) {
+    if (!c) return fail(REWL_ERR_INVALID, "null context");
+    int rc;
+    if ((rc = replay_topup(c))) return rc;
+    const bool even = c->direction_even;
+    const std::vector<int32_t>& pl = even ? c->pairs_even : c->pairs_odd;
+    if (!pl.empty()) {
+        const int32_t* dp = even ? c->d_pairs_even.p : c->d_pairs_odd.p;
+        hipLaunchKernelGGL(k_exchange, dim3((unsigned)pl.size()), dim3(kLanes), 0, c->stream, c->args, dp,
+                           (int)pl.size(), c->p.rng_mode == REWL_RNG_REPLAY ? 1 : 0);
+        HIP_TRY(hipGetLastError());
+    }
+    c->direction_even = !c->direction_even;
+    c->stats.exchange_calls += 1;
+    if (c->p.rng_mode == REWL_RNG_REPLAY) return check_err_flag(c);
+    return REWL_OK;
+}
+
+int rewl_check(rewl_ctx* c) {
+    if (!c) return fail(REWL_ERR_INVALID, "null context");
+    CheckParams cp{};
+    cp.flatness_mode = c->p.flatness_mode;
+    cp.schedule = c->p.schedule;
+    cp.reduce_logdos = c->p.reduce_logdos;
+    cp.flatness = c->p.flatness;
+    cp.lnf_final = c->p.lnf_final;
+    cp.pending_sweeps = c->pending_sweeps;
+    const int nwin = (int)c->win.size();
+    hipLaunchKernelGGL(k_check, dim3((unsigned)nwin), dim3(kLanes), 0, c->stream, c->args, cp, c->d_flat.p);
+    HIP_TRY(hipGetLastError());
+    c->pending_sweeps = 0;
+    std::vector<int32_t> flat((size_t)nwin);
+    HIP_TRY(hipMemcpyAsync(c->done_host.data(), c->d_win_done.p, (size_t)nwin * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipMemcpyAsync(flat.data(), c->d_flat.p, (size_t)nwin * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    int all = 1;
+    for (int k = 0; k < nwin; ++k) {
+        all &= c->done_host[k] ? 1 : 0;
+        c->stats.flat_events += flat[k] ? 1 : 0;
+    }
+    c->stats.all_done = all;
+    c->stats.checks += 1;
+    return REWL_OK;
+}
+
+int rewl_run(rewl_ctx* c, int64_t max_checks) {
+    if (!c) return fail(REWL_ERR_INVALID, "null context");
+    const int per_check = c->p.sweeps_per_check / c->p.sweeps_per_exchange;
+    int rc;
+    for (int64_t k = 0; k < max_checks; ++k) {
+        for (int u = 0; u < per_check; ++u) {
+            if ((rc = rewl_sweeps(c, c->p.sweeps_per_exchange))) return rc;
+            if (c->p.exchange && (rc = rewl_exchange(c))) return rc;
+        }
+        if ((rc = rewl_check(c))) return rc;
+        if (c->stats.all_done) return REWL_OK;
+    }
+    return fail(REWL_ERR_ITER_CAP, "not converged after max_checks check cycles");
+}
+
+int rewl_finalize(rewl_ctx* c) {
+    if (!c) return fail(REWL_ERR_INVALID, "null context");
+    hipLaunchKernelGGL(k_finalize, dim3((unsigned)c->win.size()), dim3(kLanes), 0, c->stream, c->args);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return REWL_OK;
+}
+
+int rewl_synchronize(rewl_ctx* c) {
+    if (!c) return fail(REWL_ERR_INVALID, "null context");
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return REWL_OK;
+}
+
+int rewl_num_walkers(const rewl_ctx* c, int32_t* n) {
+    if (!c || !n) return fail(REWL_ERR_INVALID, "null argument");
+    *n = c->num_walkers;
+    return REWL_OK;
+}
+
+int rewl_num_windows(const rewl_ctx* c, int32_t* n) {
+    if (!c || !n) return fail(REWL_ERR_INVALID, "null argument");
+    *n = (int32_t)c->win.size();
+    return REWL_OK;
+}
+
+int rewl_get_window_info(rewl_ctx* c, int32_t wi, rewl_window_info* o) {
+    if (!c || !o || wi < 0 || wi >= (int32_t)c->win.size()) return fail(REWL_ERR_INVALID, "bad window index");
+    const HostWindow& w = c->win[wi];
+    o->system = w.sys;
+    o->window = wi - c->sys[w.sys].first_window;
+    o->first_walker = w.first_walker;
+    o->num_bins = w.nbins;
+    o->e_min = w.emin; o->e_max = w.emax; o->bin_size = w.bs;
+    int32_t done = 0, oot = 0;
+    int64_t sweeps = 0;
+    HIP_TRY(hipMemcpyAsync(&o->lnf, c->d_win_lnf.p + wi, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipMemcpyAsync(&o->iteration, c->d_win_iter.p + wi, sizeof(int64_t), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipMemcpyAsync(&sweeps, c->d_win_sweeps.p + wi, sizeof(int64_t), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipMemcpyAsync(&done, c->d_win_done.p + wi, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipMemcpyAsync(&oot, c->d_win_oot.p + wi, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    o->sweeps_in_iteration = sweeps + c->pending_sweeps;
+    o->done = done;
+    o->one_over_t = oot;
+    return REWL_OK;
+}
+
+int rewl_get_stats(rewl_ctx* c, rewl_stats* o) {
+    if (!c || !o) return fail(REWL_ERR_INVALID, "null argument");
+    int rc;
+    if ((rc = resolve_sweep_events(c))) return rc;
+    unsigned long long acc = 0;
+    HIP_TRY(hipMemcpyAsync(&acc, c->d_exch.p, sizeof(acc), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    c->stats.exchanges_accepted = acc;
+    *o = c->stats;
+    return REWL_OK;
+}
+
+static int walker_ok(const rewl_ctx* c, int32_t g) { return c && g >= 0 && g < c->num_walkers; }
+
+int rewl_get_logdos(rewl_ctx* c, int32_t g, double* out, int32_t n) {
+    if (!walker_ok(c, g) || !out) return fail(REWL_ERR_INVALID, "bad walker");
+    const int nb = c->win[c->w_window[g]].nbins;
+    if (n < nb) return fail(REWL_ERR_INVALID, "output buffer too small");
+    HIP_TRY(hipMemcpy2DAsync(out, sizeof(double), c->d_lng.p + c->w_hist[g], kLanes * sizeof(double), sizeof(double),
+                             (size_t)nb, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return REWL_OK;
+}
+
+int rewl_get_counts(rewl_ctx* c, int32_t g, uint32_t* out, int32_t n) {
+    if (!walker_ok(c, g) || !out) return fail(REWL_ERR_INVALID, "bad walker");
+    const int nb = c->win[c->w_window[g]].nbins;
+    if (n < nb) return fail(REWL_ERR_INVALID, "output buffer too small");
+    HIP_TRY(hipMemcpy2DAsync(out, sizeof(uint32_t), c->d_cnt.p + c->w_hist[g], kLanes * sizeof(uint32_t),
+                             sizeof(uint32_t), (size_t)nb, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return REWL_OK;
+}
+
+int rewl_get_visits(rewl_ctx* c, int32_t g, uint64_t* out, int32_t n) {
+    if (!walker_ok(c, g) || !out) return fail(REWL_ERR_INVALID, "bad walker");
+    const int nb = c->win[c->w_window[g]].nbins;
+    if (n < nb) return fail(REWL_ERR_INVALID, "output buffer too small");
+    HIP_TRY(hipMemcpy2DAsync(out, sizeof(uint64_t), c->d_visits.p + c->w_hist[g], kLanes * sizeof(uint64_t),
+                             sizeof(uint64_t), (size_t)nb, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return REWL_OK;
+}
+
+int rewl_get_spins(rewl_ctx* c, int32_t g, int8_t* out, int32_t n) {
+    if (!walker_ok(c, g) || !out) return fail(REWL_ERR_INVALID, "bad walker");
+    const HostSystem& S = c->sys[c->win[c->w_window[g]].sys];
+    if (n < S.ndof) return fail(REWL_ERR_INVALID, "output buffer too small");
+    const int planes = S.d.model == REWL_ISING2D ? 1 : 2, L = S.d.L;
+    std::vector<uint64_t> words((size_t)planes * L);
+    HIP_TRY(hipMemcpy2DAsync(words.data(), sizeof(uint64_t), c->d_lat.p + c->w_lat[g], kLanes * sizeof(uint64_t),
+                             sizeof(uint64_t), words.size(), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    for (int i = 0; i < S.N; ++i) {
+        const int x = i % L, y = i / L;
+        for (int q = 0; q < planes; ++q) {
+            const int8_t v = ((words[(size_t)q * L + y] >> x) & 1ull) ? -1 : 1;
+            out[planes == 1 ? i : 2 * i + q] = v;
+        }
+    }
+    return REWL_OK;
+}
+
+int rewl_get_energy(rewl_ctx* c, int32_t g, double* e) {
+    if (!walker_ok(c, g) || !e) return fail(REWL_ERR_INVALID, "bad walker");
+    HIP_TRY(hipMemcpyAsync(e, c->d_w_E.p + g, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return REWL_OK;
+}
+
+int rewl_get_flavor(rewl_ctx* c, int32_t g, int32_t* f) {
+    if (!walker_ok(c, g) || !f) return fail(REWL_ERR_INVALID, "bad walker");
+    HIP_TRY(hipMemcpyAsync(f, c->d_w_flavor.p + g, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return REWL_OK;
+}
+
+int rewl_get_rng_cursor(rewl_ctx* c, int32_t g, uint64_t* cur) {
+    if (!walker_ok(c, g) || !cur) return fail(REWL_ERR_INVALID, "bad walker");
+    HIP_TRY(hipMemcpyAsync(cur, c->d_w_cursor.p + g, sizeof(uint64_t), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return REWL_OK;
+}
+
+int rewl_set_logdos(rewl_ctx* c, int32_t g, const double* lng, int32_t n) {
+    if (!walker_ok(c, g) || !lng) return fail(REWL_ERR_INVALID, "bad walker");
+    const int nb = c->win[c->w_window[g]].nbins;
+    if (n != nb) return fail(REWL_ERR_INVALID, "ln g length must equal the window's bin count");
+    HIP_TRY(hipMemcpy2DAsync(c->d_lng.p + c->w_hist[g], kLanes * sizeof(double), lng, sizeof(double), sizeof(double),
+                             (size_t)nb, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return REWL_OK;
+}
+
+int rewl_set_window_lnf(rewl_ctx* c, int32_t wi, double lnf) {
+    if (!c || wi < 0 || wi >= (int32_t)c->win.size() || !(lnf > 0)) return fail(REWL_ERR_INVALID, "bad window or ln f");
+    HIP_TRY(hipMemcpyAsync(c->d_win_lnf.p + wi, &lnf, sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return REWL_OK;
+}
+
+int rewl_set_spins(rewl_ctx* c, int32_t g, const int8_t* spins, int32_t n) {
+    if (!walker_ok(c, g) || !spins) return fail(REWL_ERR_INVALID, "bad walker");
+    const HostSystem& S = c->sys[c->win[c->w_window[g]].sys];
+    if (n != S.ndof) return fail(REWL_ERR_INVALID, "spin array length must equal the number of degrees of freedom");
+    const int planes = S.d.model == REWL_ISING2D ? 1 : 2, L = S.d.L;
+    std::vector<uint64_t> words((size_t)planes * L, 0ull);
+    for (int i = 0; i < S.N; ++i) {
+        const int x = i % L, y = i / L;
+        for (int q = 0; q < planes; ++q) {
+            const int8_t v = spins[planes == 1 ? i : 2 * i + q];
+            if (v < 0) words[(size_t)q * L + y] |= (1ull << x);
+        }
+    }
+    HIP_TRY(hipMemcpy2DAsync(c->d_lat.p + c->w_lat[g], kLanes * sizeof(uint64_t), words.data(), sizeof(uint64_t),
+                             sizeof(uint64_t), words.size(), hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(k_recalc, dim3((c->num_walkers + 63) / 64), dim3(64), 0, c->stream, c->args);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return REWL_OK;
+}
+
+// ---- results path -------------------------------------------------------------
+// Derivative of a tabulated function as numerical_derivatives.hpp:93-116 takes
+// it: one-sided at the ends, second-order next to them, fourth-order inside.
+static double tab_deriv(size_t i, const std::vector<double>& f, const std::vector<double>& e) {
+    const size_t n = f.size();
+    if (i == 0) return (f[1] - f[0]) / (e[1] - e[0]);
+    if (i == n - 1) return (f[i] - f[i - 1]) / (e[i] - e[i - 1]);
+    if (i == 1 || i == n - 2) return (f[i + 1] - f[i - 1]) / (2. * (e[i] - e[i - 1]));
+    return (-f[i + 2] + 8. * f[i + 1] - 8. * f[i - 1] + f[i - 2]) / (12. * (e[i] - e[i - 1]));
+}
+
+// |x| as the reference evaluates it: its unqualified abs() on a double binds
+// to int abs(int), truncating toward zero (vector_matching.hpp:193, 200).
+static double int_abs(double v) { return (double)std::abs((int)v); }
+
+int rewl_stitch(rewl_ctx* c, int32_t si, double states_per_site, double* energies, double* logdos, int32_t capacity,
+                int32_t* n_out) {
+    if (!c || si < 0 || si >= (int32_t)c->sys.size() || !n_out) return fail(REWL_ERR_INVALID, "bad system");
+    const HostSystem& S = c->sys[si];
+    const int nw = S.d.num_windows;
+    std::vector<std::vector<double>> et(nw), lt(nw);
+    for (int k = 0; k < nw; ++k) {
+        const HostWindow& w = c->win[S.first_window + k];
+        lt[k].resize(w.nbins);
+        int rc = rewl_get_logdos(c, w.first_walker, lt[k].data(), w.nbins);
+        if (rc) return rc;
+        for (int b = 0; b < w.nbins; ++b) et[k].push_back(w.emin + w.bs * (double)b);
+    }
+    // concatenate_tables_multiple_overlap (concatenate_rewl_tables.hpp:167-305)
+    std::vector<double> fe = et[0], fl = lt[0];
+    double shifter = 0.;
+    for (int k = 1; k < nw; ++k) {
+        long start = -1;
+        for (long i = (long)fe.size() - 1; i >= 0; --i)
+            if (fe[i] == et[k].front()) { start = i; break; }
+        if (start < 0) return fail(REWL_ERR_INVALID, "adjacent windows do not overlap");
+        const size_t nl = fl.size();
+        double bd = int_abs(tab_deriv((size_t)start, fl, fe) - tab_deriv(0, lt[k], et[k]));
+        size_t join = (size_t)start;
+        for (size_t j = 1; j != nl - (size_t)start; ++j) {
+            const double t = int_abs(tab_deriv((size_t)start + j, fl, fe) - tab_deriv(j, lt[k], et[k]));
+            if (t + 0.5 < bd + 0.5) { bd = t; join = (size_t)start + j; }
+        }
+        size_t r = 0;
+        for (size_t i = (size_t)start; i < fe.size(); ++i) {
+            r = i - (size_t)start;
+            if (i == join) shifter = fl[i] - lt[k][r];
+            else if (i > join) fl[i] = lt[k][r] + shifter;
+        }
+        for (size_t j = r + 1; j < et[k].size(); ++j) { fe.push_back(et[k][j]); fl.push_back(lt[k][j] + shifter); }
+    }
+    // normalize_logDoS (array_shift.hpp:15-43)
+    if (states_per_site > 0) {
+        double mx = 0.;
+        for (double v : fl) mx = mx * (double)(mx > v) + v * (double)(mx <= v);
+        double sum = 0.;
+        for (double v : fl) sum += exp(-(mx - v));
+        sum = log(sum) + mx;
+        const double sh = (double)S.N * log(states_per_site) - sum - log(2.) * (double)(S.d.e_max == 0.);
+        for (double& v : fl) v += sh;
+    }
+    *n_out = (int32_t)fl.size();
+    for (size_t i = 0; i < fl.size() && (int32_t)i < capacity; ++i) {
+        if (energies) energies[i] = fe[i];
+        if (logdos) logdos[i] = fl[i];
+    }
+    return REWL_OK;
+}
+
+int rewl_thermodynamics(const double* E, const double* lng, int32_t nb, int32_t N, double tmin, double tmax, int32_t nT,
+                        double* out) {
+    if (!E || !lng || !out || nb < 2 || nT < 1 || N < 1) return fail(REWL_ERR_INVALID, "bad arguments");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(REWL_ERR_NO_DEVICE, "no HIP device");
+    double *dE = nullptr, *dl = nullptr, *dout = nullptr;
+    HIP_TRY(hipMalloc((void**)&dE, (size_t)nb * sizeof(double)));
+    HIP_TRY(hipMalloc((void**)&dl, (size_t)nb * sizeof(double)));
+    HIP_TRY(hipMalloc((void**)&dout, (size_t)nT * 5 * sizeof(double)));
+    HIP_TRY(hipMemcpy(dE, E, (size_t)nb * sizeof(double), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(dl, lng, (size_t)nb * sizeof(double), hipMemcpyHostToDevice));
+    const double dT = (tmax - tmin) / (double)nT;
+    hipLaunchKernelGGL(k_thermo, dim3((nT + 127) / 128), dim3(128), 0, nullptr, dE, dl, nb, N, tmin, dT, nT, dout);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpy(out, dout, (size_t)nT * 5 * sizeof(double), hipMemcpyDeviceToHost));
+    (void)hipFree(dE);
+    (void)hipFree(dl);
+    (void)hipFree(dout);
+    return REWL_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,284 @@
+// Wang-Landau sweep kernel: one lane per walker.
+//
+// Restates Wang_Landau::wang_landau_update / wang_landau_sweep and
+// REWL_Walker::wang_landau_walk (wang_landau.hpp:49-120, rewl_walker.hpp:141-158):
+// sites are visited in order 0..N-1 per flavour pass; a proposal outside the
+// window is rejected without a draw; inside it, d = (float)(lng[new]-lng[cur])
+// is accepted when d <= 0, otherwise a uniform is drawn and compared with
+// exp(-d) in FP64; the count and ln g of the (possibly new) current bin are
+// then incremented.
+//
+// Each wavefront owns one group of <= 64 walkers.  The group's ln g column
+// ([bin][lane], FP64) and its visit counts live in LDS for the whole launch;
+// the lattice streams row by row from HBM/L2 (one 64-bit word per row and
+// plane, one row prefetched ahead), so every spin-flip attempt touches only
+// registers and LDS.
+#pragma once
+#include "device.hpp"
+
+namespace rewl {
+
+template <bool PACKED>
+__device__ __forceinline__ void count_inc(uint32_t* s_cnt, int bin, int lane) {
+    if (PACKED) {
+        atomicAdd(&s_cnt[(bin >> 1) * kLanes + lane], 1u << ((bin & 1) << 4));
+    } else {
+        atomicAdd(&s_cnt[bin * kLanes + lane], 1u);
+    }
+}
+
+template <bool PACKED>
+__device__ __forceinline__ void flush_counts(uint32_t* s_cnt, uint32_t* g_cnt, int64_t hb, int nbins, int lane) {
+    if (PACKED) {
+        for (int b = 0; b < nbins; ++b) {
+            const uint32_t v = (s_cnt[(b >> 1) * kLanes + lane] >> ((b & 1) << 4)) & 0xFFFFu;
+            g_cnt[hb + (int64_t)b * kLanes] += v;
+        }
+        for (int b = 0; b < (nbins + 1) / 2; ++b) s_cnt[b * kLanes + lane] = 0u;
+    } else {
+        for (int b = 0; b < nbins; ++b) g_cnt[hb + (int64_t)b * kLanes] = s_cnt[b * kLanes + lane];
+    }
+}
+
+// Uniform draw for one attempt.  Replay: next word of the walker's host-made
+// mt19937_64 stream, mapped as uniform_real_distribution<double> does.
+// Philox: word (p & 3) of the block drawn for positions p..p+3.
+template <bool REPLAY>
+__device__ __forceinline__ double draw_uniform(const uint64_t* ring, uint64_t cap, uint64_t filled, uint64_t& cursor,
+                                               int32_t* err, const u32x4& rnd, int p) {
+    if (REPLAY) {
+        double u = 2.0;  // rejects if the stream ran dry (reported through err)
+        if (cursor < filled) u = canonical_from_u64(ring[cursor & (cap - 1)]);
+        else atomicOr(err, 1);
+        ++cursor;
+        return u;
+    } else {
+        const int k = p & 3;
+        const uint32_t w = k == 0 ? rnd.x : (k == 1 ? rnd.y : (k == 2 ? rnd.z : rnd.w));
+        return (double)w * 0x1p-32;
+    }
+}
+
+// Acceptance core shared by every model on the integer path.  Returns true if
+// the proposal was accepted; updates cb/lc/E and the LDS histogram column.
+template <bool REPLAY, bool PACKED>
+__device__ __forceinline__ bool wl_accept_int(int En, int& E, int& cb, double& lc, double lnf, int wmin, int plo,
+                                              int pspan, int sh, double* s_lng, uint32_t* s_cnt, int lane,
+                                              const uint64_t* ring, uint64_t cap, uint64_t filled, uint64_t& cursor,
+                                              int32_t* err, const u32x4& rnd, int p) {
+    const bool in = (unsigned)(phi_int(En) - plo) < (unsigned)pspan;
+    // (size_t)((E - min) / bs) truncates toward zero: E = min - 1, which the
+    // +0.5 range test lets through when min == 0, lands in bin 0
+    const int nb = in ? max((En - wmin) >> sh, 0) : cb;
+    const double ln = s_lng[nb * kLanes + lane];
+    const float d = (float)(ln - lc);
+    bool acc = in && (d <= 0.f);
+    if (in && !(d <= 0.f)) {
+        const double u = draw_uniform<REPLAY>(ring, cap, filled, cursor, err, rnd, p);
+        acc = u < exp(-(double)d);
+    }
+    if (acc) { E = En; cb = nb; lc = ln; }
+    lc += lnf;
+    s_lng[cb * kLanes + lane] = lc;
+    count_inc<PACKED>(s_cnt, cb, lane);
+    return acc;
+}
+
+// FP64 variant (field models): bins and range follow histogram_index.hpp:8-31
+// literally.
+template <bool REPLAY, bool PACKED>
+__device__ __forceinline__ bool wl_accept_fp(double En, double& E, int& cb, double& lc, double lnf, double emin,
+                                             double bs, int plo, int phi, int nbmax, double* s_lng, uint32_t* s_cnt, int lane,
+                                             const uint64_t* ring, uint64_t cap, uint64_t filled, uint64_t& cursor,
+                                             int32_t* err, const u32x4& rnd, int p) {
+    const int pe = trunc_half(En);
+    const bool in = pe >= plo && pe < phi;
+    const int nb = in ? min((int)((En - emin) / bs), nbmax) : cb;
+    const double ln = s_lng[nb * kLanes + lane];
+    const float d = (float)(ln - lc);
+    bool acc = in && (d <= 0.f);
+    if (in && !(d <= 0.f)) {
+        const double u = draw_uniform<REPLAY>(ring, cap, filled, cursor, err, rnd, p);
+        acc = u < exp(-(double)d);
+    }
+    if (acc) { E = En; cb = nb; lc = ln; }
+    lc += lnf;
+    s_lng[cb * kLanes + lane] = lc;
+    count_inc<PACKED>(s_cnt, cb, lane);
+    return acc;
+}
+
+__device__ __forceinline__ uint32_t bit(uint64_t w, int x) { return (uint32_t)(w >> x) & 1u; }
+__device__ __forceinline__ double pm(uint32_t b) { return b ? -1.0 : 1.0; }
+
+// MODEL: 0 Ising2d, 1 Ashkin-Teller2d.  INTP: integer energies.
+template <int MODEL, bool INTP, bool REPLAY, bool PACKED>
+__device__ void sweep_walker(const KArgs& a, const DevGroup& G, const DevSystem& S, int lane, int nsweeps,
+                             uint64_t sweep_base, int flush_every, double* s_lng, uint32_t* s_cnt) {
+    const int g = G.first_walker + lane;
+    const int wi = a.w_window[g];
+    const DevWindow Wn = a.win[wi];
+    const double lnf = a.win_lnf[wi];
+    const int64_t hb = G.hist_base + lane;
+    const int nbins = G.nbins;
+
+    for (int b = 0; b < nbins; ++b) s_lng[b * kLanes + lane] = a.lng[hb + (int64_t)b * kLanes];
+    if (PACKED) {
+        for (int b = 0; b < (nbins + 1) / 2; ++b) s_cnt[b * kLanes + lane] = 0u;
+    } else {
+        for (int b = 0; b < nbins; ++b) s_cnt[b * kLanes + lane] = a.cnt[hb + (int64_t)b * kLanes];
+    }
+
+    const int L = S.L;
+    uint64_t* lat = a.lat + G.lat_base + lane;
+    const uint32_t walker = (uint32_t)a.w_local[g];
+    uint64_t cursor = REPLAY ? a.w_cursor[g] : 0ull;
+    const uint64_t filled = REPLAY ? a.ring_filled[g] : 0ull;
+    const uint64_t* ring = REPLAY ? a.ring + (int64_t)g * (int64_t)a.ring_cap : nullptr;
+    const uint64_t cap = a.ring_cap;
+    const uint64_t B = (uint64_t)((S.ndof + 3) / 4);
+    int fl = a.w_flavor[g];
+
+    int Ei = 0;
+    double Ed = a.w_E[g];
+    int cb;
+    if (INTP) {
+        Ei = (int)Ed;
+        cb = max((Ei - Wn.imin) >> Wn.shift, 0);
+    } else {
+        cb = min(max((int)((Ed - Wn.emin) / Wn.bs), 0), Wn.nbins - 1);
+    }
+    double lc = s_lng[cb * kLanes + lane];
+    const int pspan = Wn.phi_hi - Wn.phi_lo;
+    constexpr int NP = MODEL == 0 ? 1 : 2;   // bit planes
+    constexpr int NPASS = MODEL == 0 ? 1 : 2;
+
+    for (int s = 0; s < nsweeps; ++s) {
+        const uint64_t blk0 = (sweep_base + (uint64_t)s) * B;
+        int p = 0;
+        u32x4 rnd{0u, 0u, 0u, 0u};
+        for (int pass = 0; pass < NPASS; ++pass) {
+            uint64_t up[NP], cu[NP], dn[NP], r0[NP], nx[NP];
+#pragma unroll
+            for (int q = 0; q < NP; ++q) {
+                up[q] = lat[(int64_t)(q * L + L - 1) * kLanes];
+                cu[q] = lat[(int64_t)(q * L) * kLanes];
+                dn[q] = lat[(int64_t)(q * L + (L > 1 ? 1 : 0)) * kLanes];
+                r0[q] = 0ull;
+            }
+            for (int y = 0; y < L; ++y) {
+#pragma unroll
+                for (int q = 0; q < NP; ++q) nx[q] = (y + 2 < L) ? lat[(int64_t)(q * L + y + 2) * kLanes] : 0ull;
+                for (int x = 0; x < L; ++x, ++p) {
+                    if (!REPLAY && (p & 3) == 0) {
+                        const uint64_t bi = blk0 + (uint64_t)(p >> 2);
+                        rnd = philox4x32_10(u32x4{(uint32_t)bi, (uint32_t)(bi >> 32), DOMAIN_SWEEP, walker}, S.k0, S.k1);
+                    }
+                    const int xl = (x == 0) ? L - 1 : x - 1;
+                    const int xr = (x == L - 1) ? 0 : x + 1;
+                    if (MODEL == 0) {
+                        const uint32_t si = bit(cu[0], x);
+                        if (INTP) {
+                            const uint32_t na = (si ^ bit(up[0], x)) + (si ^ bit(cu[0], xl)) + (si ^ bit(cu[0], xr)) +
+                                                (si ^ bit(dn[0], x));
+                            const int En = Ei + S.twoJ * (4 - 2 * (int)na);
+                            if (wl_accept_int<REPLAY, PACKED>(En, Ei, cb, lc, lnf, Wn.imin, Wn.phi_lo, pspan, Wn.shift,
+                                                             s_lng, s_cnt, lane, ring, cap, filled, cursor, a.err, rnd, p))
+                                cu[0] ^= (1ull << x);
+                        } else {
+                            const int site = y * L + x;
+                            const double sv = pm(si);
+                            double f = field_at(S, site);
+                            f += S.J * pm(bit(up[0], x));
+                            f += S.J * pm(bit(cu[0], xl));
+                            f += S.J * pm(bit(cu[0], xr));
+                            f += S.J * pm(bit(dn[0], x));
+                            const double t = -sv;
+                            const double En = Ed + -1. * t * f - -1. * sv * f;
+                            if (wl_accept_fp<REPLAY, PACKED>(En, Ed, cb, lc, lnf, Wn.emin, Wn.bs, Wn.phi_lo, Wn.phi_hi,
+                                                            Wn.nbins - 1, s_lng, s_cnt, lane, ring, cap, filled, cursor, a.err, rnd, p))
+                                cu[0] ^= (1ull << x);
+                        }
+                    } else {
+                        const uint32_t sb = bit(cu[0], x), tb = bit(cu[1], x);
+                        if (INTP) {
+                            const uint32_t ms = (sb ^ bit(up[0], x)) | ((sb ^ bit(cu[0], xl)) << 1) |
+                                                ((sb ^ bit(cu[0], xr)) << 2) | ((sb ^ bit(dn[0], x)) << 3);
+                            const uint32_t mt = (tb ^ bit(up[1], x)) | ((tb ^ bit(cu[1], xl)) << 1) |
+                                                ((tb ^ bit(cu[1], xr)) << 2) | ((tb ^ bit(dn[1], x)) << 3);
+                            const int pf = __popc(fl ? mt : ms);
+                            const int qq = __popc(ms ^ mt);
+                            const int En = Ei + S.twoJ * (4 - 2 * pf) + S.twoK * (4 - 2 * qq);
+                            if (wl_accept_int<REPLAY, PACKED>(En, Ei, cb, lc, lnf, Wn.imin, Wn.phi_lo, pspan, Wn.shift,
+                                                             s_lng, s_cnt, lane, ring, cap, filled, cursor, a.err, rnd, p)) {
+                                if (fl) cu[1] ^= (1ull << x); else cu[0] ^= (1ull << x);
+                            }
+                        } else {
+                            const int site = y * L + x;
+                            const double sg = pm(sb), ta = pm(tb);
+                            double ts = sg, tt = ta;
+                            ts += -2. * ts * (double)(fl == 0);
+                            tt += -2. * tt * (double)(fl == 1);
+                            const double hfield = S.has_field ? field_at(S, site) : 0.0;
+                            const double ns[4] = {pm(bit(up[0], x)), pm(bit(cu[0], xl)), pm(bit(cu[0], xr)), pm(bit(dn[0], x))};
+                            const double nt[4] = {pm(bit(up[1], x)), pm(bit(cu[1], xl)), pm(bit(cu[1], xr)), pm(bit(dn[1], x))};
+                            double en_new = 0., en_old = 0.;
+                            if (S.has_field) { en_new += hfield * (ts * tt); en_old += hfield * (sg * ta); }
+#pragma unroll
+                            for (int k = 0; k < 4; ++k) {
+                                en_new += S.J * ts * ns[k];
+                                en_new += S.J * tt * nt[k];
+                                en_new += S.K * ts * tt * ns[k] * nt[k];
+                                en_old += S.J * sg * ns[k];
+                                en_old += S.J * ta * nt[k];
+                                en_old += S.K * sg * ta * ns[k] * nt[k];
+                            }
+                            const double En = Ed + -en_new - -en_old;
+                            if (wl_accept_fp<REPLAY, PACKED>(En, Ed, cb, lc, lnf, Wn.emin, Wn.bs, Wn.phi_lo, Wn.phi_hi,
+                                                            Wn.nbins - 1, s_lng, s_cnt, lane, ring, cap, filled, cursor, a.err, rnd, p)) {
+                                if (fl) cu[1] ^= (1ull << x); else cu[0] ^= (1ull << x);
+                            }
+                        }
+                    }
+                }
+#pragma unroll
+                for (int q = 0; q < NP; ++q) {
+                    lat[(int64_t)(q * L + y) * kLanes] = cu[q];
+                    if (y == 0) r0[q] = cu[q];
+                    up[q] = cu[q];
+                    cu[q] = dn[q];
+                    dn[q] = (y + 2 < L) ? nx[q] : r0[q];
+                }
+            }
+            if (MODEL == 1) fl ^= 1;   // flavour switches after site N-1
+        }
+        if (PACKED && ((s + 1) % flush_every) == 0) flush_counts<true>(s_cnt, a.cnt, hb, nbins, lane);
+    }
+
+    if (PACKED) {
+        if (nsweeps % flush_every) flush_counts<true>(s_cnt, a.cnt, hb, nbins, lane);
+    } else {
+        flush_counts<false>(s_cnt, a.cnt, hb, nbins, lane);
+    }
+    for (int b = 0; b < nbins; ++b) a.lng[hb + (int64_t)b * kLanes] = s_lng[b * kLanes + lane];
+    a.w_E[g] = INTP ? (double)Ei : Ed;
+    a.w_flavor[g] = fl;
+    if (REPLAY) a.w_cursor[g] = cursor;
+}
+
+template <int MODEL, bool INTP, bool REPLAY, bool PACKED>
+__global__ void __launch_bounds__(kLanes) k_sweep(KArgs a, const int32_t* groups, int ngroups, int nsweeps,
+                                                  uint64_t sweep_base, int flush_every) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int lane = threadIdx.x;
+    for (int gi = blockIdx.x; gi < ngroups; gi += gridDim.x) {
+        const DevGroup G = a.grp[groups[gi]];
+        const DevSystem S = a.sys[G.sys];
+        double* s_lng = reinterpret_cast<double*>(smem);
+        uint32_t* s_cnt = reinterpret_cast<uint32_t*>(s_lng + (size_t)G.nbins * kLanes);
+        if (lane < G.nwalkers)
+            sweep_walker<MODEL, INTP, REPLAY, PACKED>(a, G, S, lane, nsweeps, sweep_base, flush_every, s_lng, s_cnt);
+    }
+}
+
+}  // namespace rewl

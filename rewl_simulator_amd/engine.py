@@ -1,0 +1,215 @@
+"""Python driver of the HIP REWL engine over the C ABI (include/rewl_c.h).
+
+Mirrors the reference's REWL_simulation surface (rewl_simulation.hpp:21-66):
+construct from model and REWL parameters, adjust walkers into their windows,
+``simulate`` (run to the final ln f), then export the stitched and normalised
+ln g.  Every call goes to the GPU; there is no CPU fallback.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import abi
+
+_SYSTEM_KEYS = ("model", "L", "J", "K", "h", "e_min", "e_max", "bin_size", "num_windows",
+                "replicas_per_window", "overlap", "seed")
+
+
+def _desc_array(systems, keep):
+    arr = (abi.SystemDesc * len(systems))()
+    for d, s in zip(arr, systems):
+        for k in _SYSTEM_KEYS:
+            setattr(d, k, s[k])
+        field = s.get("field")
+        if field is not None:
+            buf = np.ascontiguousarray(field, dtype=np.float64)
+            keep.append(buf)
+            d.field = buf.ctypes.data_as(C.POINTER(C.c_double))
+        ex = s.get("exclude_bins")
+        if ex is not None:
+            buf = np.ascontiguousarray(ex, dtype=np.uint8)
+            keep.append(buf)
+            d.exclude_bins = buf.ctypes.data_as(C.POINTER(C.c_uint8))
+        ws = s.get("walker_seeds")
+        if ws is not None:
+            buf = np.ascontiguousarray(ws, dtype=np.uint64)
+            keep.append(buf)
+            d.walker_seeds = buf.ctypes.data_as(C.POINTER(C.c_uint64))
+    return arr
+
+
+def _params(p):
+    out = abi.Params()
+    for k, v in p.items():
+        setattr(out, k, v)
+    return out
+
+
+class Engine:
+    """One rewl_ctx: every walker of the given systems on the current GPU."""
+
+    def __init__(self, systems, params):
+        self.lib = abi.load_engine()
+        self._keep = []
+        self.systems = [dict(s) for s in systems]
+        self.params = dict(params)
+        descs = _desc_array(self.systems, self._keep)
+        pp = _params(self.params)
+        ctx = C.c_void_p()
+        abi.check(self.lib.rewl_create(descs, len(systems), C.byref(pp), C.byref(ctx)))
+        self.ctx = ctx
+        n = C.c_int32()
+        abi.check(self.lib.rewl_num_walkers(self.ctx, C.byref(n)))
+        self.num_walkers = n.value
+        abi.check(self.lib.rewl_num_windows(self.ctx, C.byref(n)))
+        self.num_windows = n.value
+        self.windows = [self.window_info(k) for k in range(self.num_windows)]
+        self.walker_window = np.empty(self.num_walkers, dtype=np.int64)
+        for k, w in enumerate(self.windows):
+            R = self.systems[w["system"]]["replicas_per_window"]
+            self.walker_window[w["first_walker"]:w["first_walker"] + R] = k
+
+    def close(self):
+        if self.ctx:
+            self.lib.rewl_destroy(self.ctx)
+            self.ctx = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- steps -------------------------------------------------------------
+    def adjust_to_range(self):
+        abi.check(self.lib.rewl_adjust_to_range(self.ctx))
+
+    def sweeps(self, n):
+        abi.check(self.lib.rewl_sweeps(self.ctx, int(n)))
+
+    def exchange(self):
+        abi.check(self.lib.rewl_exchange(self.ctx))
+
+    def check(self):
+        abi.check(self.lib.rewl_check(self.ctx))
+
+    def run(self, max_checks):
+        """Returns True when every window reached the final ln f."""
+        rc = self.lib.rewl_run(self.ctx, int(max_checks))
+        if rc == abi.REWL_ERR_ITER_CAP:
+            return False
+        abi.check(rc)
+        return True
+
+    def finalize(self):
+        abi.check(self.lib.rewl_finalize(self.ctx))
+
+    def synchronize(self):
+        abi.check(self.lib.rewl_synchronize(self.ctx))
+
+    # ---- queries -------------------------------------------------------------
+    def window_info(self, k):
+        wi = abi.WindowInfo()
+        abi.check(self.lib.rewl_get_window_info(self.ctx, int(k), C.byref(wi)))
+        return {f: getattr(wi, f) for f, _ in abi.WindowInfo._fields_}
+
+    def stats(self):
+        st = abi.Stats()
+        abi.check(self.lib.rewl_get_stats(self.ctx, C.byref(st)))
+        return {f: getattr(st, f) for f, _ in abi.Stats._fields_}
+
+    def _nbins(self, g):
+        return self.windows[self.walker_window[g]]["num_bins"]
+
+    def logdos(self, g):
+        n = self._nbins(g)
+        out = np.empty(n, dtype=np.float64)
+        abi.check(self.lib.rewl_get_logdos(self.ctx, int(g), out.ctypes.data_as(C.POINTER(C.c_double)), n))
+        return out
+
+    def counts(self, g):
+        n = self._nbins(g)
+        out = np.empty(n, dtype=np.uint32)
+        abi.check(self.lib.rewl_get_counts(self.ctx, int(g), out.ctypes.data_as(C.POINTER(C.c_uint32)), n))
+        return out
+
+    def visits(self, g):
+        n = self._nbins(g)
+        out = np.empty(n, dtype=np.uint64)
+        abi.check(self.lib.rewl_get_visits(self.ctx, int(g), out.ctypes.data_as(C.POINTER(C.c_uint64)), n))
+        return out
+
+    def ndof(self, g):
+        s = self.systems[self.windows[self.walker_window[g]]["system"]]
+        return s["L"] ** 2 * (1 if s["model"] == abi.ISING2D else 2)
+
+    def spins(self, g):
+        n = self.ndof(g)
+        out = np.empty(n, dtype=np.int8)
+        abi.check(self.lib.rewl_get_spins(self.ctx, int(g), out.ctypes.data_as(C.POINTER(C.c_int8)), n))
+        return out
+
+    def energy(self, g):
+        e = C.c_double()
+        abi.check(self.lib.rewl_get_energy(self.ctx, int(g), C.byref(e)))
+        return e.value
+
+    def flavor(self, g):
+        f = C.c_int32()
+        abi.check(self.lib.rewl_get_flavor(self.ctx, int(g), C.byref(f)))
+        return f.value
+
+    def rng_cursor(self, g):
+        c = C.c_uint64()
+        abi.check(self.lib.rewl_get_rng_cursor(self.ctx, int(g), C.byref(c)))
+        return c.value
+
+    # ---- state import ----------------------------------------------------------
+    def set_logdos(self, g, lng):
+        buf = np.ascontiguousarray(lng, dtype=np.float64)
+        abi.check(self.lib.rewl_set_logdos(self.ctx, int(g), buf.ctypes.data_as(C.POINTER(C.c_double)), buf.size))
+
+    def set_window_lnf(self, k, lnf):
+        abi.check(self.lib.rewl_set_window_lnf(self.ctx, int(k), float(lnf)))
+
+    def set_spins(self, g, spins):
+        buf = np.ascontiguousarray(spins, dtype=np.int8)
+        abi.check(self.lib.rewl_set_spins(self.ctx, int(g), buf.ctypes.data_as(C.POINTER(C.c_int8)), buf.size))
+
+    # ---- results ------------------------------------------------------------------
+    def stitched(self, system=0, states_per_site=None):
+        """Stitched ln g of one system; normalised to states_per_site^N when given."""
+        cap = sum(w["num_bins"] for w in self.windows if w["system"] == system)
+        E = np.empty(cap, dtype=np.float64)
+        lg = np.empty(cap, dtype=np.float64)
+        n = C.c_int32()
+        sps = -1.0 if states_per_site is None else float(states_per_site)
+        abi.check(self.lib.rewl_stitch(self.ctx, int(system), sps, E.ctypes.data_as(C.POINTER(C.c_double)),
+                                       lg.ctypes.data_as(C.POINTER(C.c_double)), cap, C.byref(n)))
+        return E[:n.value].copy(), lg[:n.value].copy()
+
+
+def thermodynamics(energies, logdos, system_size, t_min=0.01, t_max=50.01, num_t=50000):
+    """FP64 canonical averages on the GPU: columns F/N, U/N, U^2/N^2, S/N, C/N."""
+    lib = abi.load_engine()
+    E = np.ascontiguousarray(energies, dtype=np.float64)
+    lg = np.ascontiguousarray(logdos, dtype=np.float64)
+    out = np.empty((num_t, 5), dtype=np.float64)
+    abi.check(lib.rewl_thermodynamics(E.ctypes.data_as(C.POINTER(C.c_double)),
+                                      lg.ctypes.data_as(C.POINTER(C.c_double)), E.size, int(system_size),
+                                      float(t_min), float(t_max), int(num_t),
+                                      out.ctypes.data_as(C.POINTER(C.c_double))))
+    return out
+
+
+def device_count():
+    lib = abi.load_engine()
+    n = C.c_int()
+    lib.rewl_device_count(C.byref(n))
+    return n.value
