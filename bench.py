@@ -1,0 +1,169 @@
+"""Benchmark: spin-flip attempts/s of the REWL walker sweep on one MI355X.
+
+Workload: 2D Ising, L = 64 periodic (N = 4096), 2049 bins of width 8 over
+[-8192, 8192], 32 windows at 75% overlap x 64 walkers = 2048 walkers, one lane
+per walker, flatness min >= 0.8 mean, ln f halved toward 1e-8.
+A step is one check cycle: sweeps_per_check sweeps of every walker, the replica
+exchange and the flatness/averaging check.  Inputs (lattices, histograms) are
+resident in HBM before timing starts.
+
+Prints one JSON line with the throughput, the sweep kernel's roofline figures
+(HIP-event timing of the kernel on the engine's own stream) and a bounded
+CPU-baseline sample of the oracle on the host cores.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+BYTES_PER_ATTEMPT = 32.25   # 8 B read ln g[new] + 16 B RMW ln g[cur] + 8 B RMW count + 0.25 B lattice
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--sweeps-per-step", type=int, default=1000)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target length of the CPU-baseline sample")
+    ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, host cores)")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--walkers-per-group", type=int, default=0)
+    ap.add_argument("--producers", type=int, default=0, help="producer wavefronts (0 auto, -1 none)")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
+                    help="only run a plain device-to-device copy of BYTES (counter calibration) and exit")
+    return ap.parse_args()
+
+
+def calibrate_copy(nbytes):
+    """A plain copy kernel of known size: rocprof's FETCH_SIZE / WRITE_SIZE of
+    this launch give the scale that converts those counters into bytes."""
+    import torch
+    n = nbytes // 4
+    src = torch.arange(n, dtype=torch.int32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    assert torch.equal(dst[:16], src[:16])
+    print(json.dumps({"calibration_copy_bytes": n * 4, "launches": 3}), flush=True)
+
+
+def cpu_baseline(systems, params, target_s, threads):
+    """The oracle (a single-threaded-per-walker restatement of the reference
+    update, FP64 spins) timed on the host cores over a bounded sample of the
+    same workload: every walker swept the same number of times."""
+    from oracle.oracle import Oracle
+    ora = Oracle(systems, params)
+    ora.adjust_to_range()
+    sec, att = ora.bench_sweeps(1, threads)
+    sweeps = max(1, int(target_s / max(sec, 1e-3)))
+    sec, att = ora.bench_sweeps(sweeps, threads)
+    ora.close()
+    return dict(value=att / sec, unit="spin-flip attempts/s", cores=threads, kind="port",
+                sample=f"{sweeps} sweep(s) of all {len(range(ora.num_walkers))} walkers from the adjusted "
+                       f"initial state, ln f = 1, {att} attempts in {sec:.2f} s")
+
+
+def main():
+    a = parse()
+    if a.calibrate_copy:
+        calibrate_copy(a.calibrate_copy)
+        return
+    if a.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {a.gpus} is not supported by this build "
+              f"(every box here has one GPU).", file=sys.stderr)
+        sys.exit(2)
+    from rewl_simulator_amd import configs
+    from rewl_simulator_amd.engine import Engine
+
+    systems, params = configs.config2()
+    params = dict(params, sweeps_per_check=a.sweeps_per_step, sweeps_per_exchange=a.sweeps_per_step,
+                  walkers_per_group=a.walkers_per_group, producers=a.producers)
+    eng = Engine(systems, params)
+    eng.adjust_to_range()
+    W = eng.num_walkers
+    N = systems[0]["L"] ** 2
+    per_check = params["sweeps_per_check"] // params["sweeps_per_exchange"]
+
+    def step():
+        for _ in range(per_check):
+            eng.sweeps(params["sweeps_per_exchange"])
+            eng.exchange()
+        eng.check()
+
+    for _ in range(a.warmup):
+        step()
+    eng.synchronize()
+    st0 = eng.stats()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        step()
+    eng.synchronize()
+    t1 = time.perf_counter()
+    st1 = eng.stats()
+    elapsed = t1 - t0
+    attempts = st1["attempts"] - st0["attempts"]
+    rate = attempts / elapsed
+    launches = st1["sweep_launches"] - st0["sweep_launches"]
+    kms = st1["kernel_ms_sweep"] - st0["kernel_ms_sweep"]
+    avg_launch_s = kms / 1e3 / max(launches, 1)
+    att_per_launch = attempts / max(launches, 1)
+    achieved = att_per_launch * BYTES_PER_ATTEMPT / avg_launch_s / 1e9
+    lnf = min(w["lnf"] for w in (eng.window_info(k) for k in range(eng.num_windows)))
+
+    line = {
+        "metric": "spin-flip attempts/s; time to ln f=1e-6 (learned warm start vs plain)",
+        "value": rate,
+        "unit": "spin-flip attempts/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": elapsed * 1e3 / a.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32 energies, f64 ln g",
+        "data": "synthetic (all-up start, Philox4x32-10 draws)",
+        "config": {
+            "workload": "2D Ising REWL, L=64 periodic, 2049 bins of width 8, 32 windows (75% overlap) x 64 "
+                        "walkers, flatness 0.8, ln f halving to 1e-8",
+            "walkers": W,
+            "sweeps_per_step": params["sweeps_per_check"],
+            "attempts_per_step": attempts // a.steps,
+            "parallelism": "one lane per walker, 1 GPU",
+            "producers": a.producers,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "kernel": "k_sweep (Ising, integer path)",
+            "avg_launch_ms": avg_launch_s * 1e3,
+            "attempts_per_launch": att_per_launch,
+            "bytes_per_attempt": BYTES_PER_ATTEMPT,
+            "kernel_attempts_per_s": att_per_launch / avg_launch_s,
+        },
+        "state": {"min_lnf_after": lnf, "flat_events": st1["flat_events"],
+                  "exchanges_accepted": st1["exchanges_accepted"]},
+    }
+    eng.close()
+    if not a.no_cpu:
+        threads = a.cpu_threads or min(16, os.cpu_count() or 1)
+        line["cpu_baseline"] = cpu_baseline(systems, params, a.cpu_seconds, threads)
+        line["cpu_baseline"]["gpu_over_cpu"] = rate / line["cpu_baseline"]["value"]
+    else:
+        line["cpu_baseline"] = None
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -116,7 +116,10 @@ typedef struct rewl_params {
     int32_t rng_mode;            /* enum rewl_rng                                     */
     int32_t walkers_per_group;   /* lanes used per wavefront (1..64); 0 = autotune    */
     int32_t exchange;            /* 0 disables replica exchange (independent walkers) */
-    int32_t reserved[7];
+    int32_t producers;           /* Philox fast path: wavefronts per workgroup that
+                                    precompute the acceptance thresholds; 0 = auto,
+                                    <0 = none (the walkers draw their own)          */
+    int32_t reserved[6];
 } rewl_params;
 
 /* Per-window bookkeeping, one record per (system, window). */
@@ -146,6 +149,7 @@ typedef struct rewl_stats {
     int32_t  num_walkers;
     double   kernel_ms_sweep;    /* accumulated sweep-kernel time (HIP events)        */
     double   kernel_ms_other;    /* exchange + check kernels                          */
+    uint64_t sweep_launches;     /* sweep-kernel launches timed in kernel_ms_sweep    */
 } rewl_stats;
 
 typedef struct rewl_ctx rewl_ctx;
@@ -201,6 +205,12 @@ int rewl_stitch(rewl_ctx* ctx, int32_t system, double states_per_site,
 int rewl_thermodynamics(const double* energies, const double* logdos, int32_t num_bins,
                         int32_t system_size, double t_min, double t_max, int32_t num_t,
                         double* out);
+
+/* ---- device self-check ----------------------------------------------------- */
+/* No counterpart in the reference.  what = 0: largest absolute error of the
+ * FP32 -ln(u) the sweep kernel screens acceptances with, over every 32-bit
+ * Philox word (exhaustive); the kernel's acceptance band is 2^-12. */
+int rewl_selfcheck(int32_t what, double* out);
 
 #ifdef __cplusplus
 }

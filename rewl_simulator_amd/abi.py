@@ -65,7 +65,8 @@ class Params(C.Structure):
         ("rng_mode", C.c_int32),
         ("walkers_per_group", C.c_int32),
         ("exchange", C.c_int32),
-        ("reserved", C.c_int32 * 7),
+        ("producers", C.c_int32),
+        ("reserved", C.c_int32 * 6),
     ]
 
 
@@ -98,6 +99,7 @@ class Stats(C.Structure):
         ("num_walkers", C.c_int32),
         ("kernel_ms_sweep", C.c_double),
         ("kernel_ms_other", C.c_double),
+        ("sweep_launches", C.c_uint64),
     ]
 
 
@@ -134,6 +136,7 @@ SIGNATURES = {
                      C.POINTER(C.c_int32)], C.c_int),
     "rewl_thermodynamics": ([C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int32, C.c_int32, C.c_double,
                              C.c_double, C.c_int32, C.POINTER(C.c_double)], C.c_int),
+    "rewl_selfcheck": ([C.c_int32, C.POINTER(C.c_double)], C.c_int),
 }
 
 _lib = None

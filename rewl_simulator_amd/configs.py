@@ -25,6 +25,7 @@ REFERENCE_PARAMS = dict(
     rng_mode=abi.RNG_PHILOX,
     walkers_per_group=0,
     exchange=1,
+    producers=0,
 )
 
 DEFAULT_SEED = 20240501

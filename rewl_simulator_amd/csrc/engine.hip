@@ -13,6 +13,7 @@
 #include <cstdio>
 #include <cstring>
 #include <mutex>
+#include <numeric>
 #include <random>
 #include <string>
 #include <vector>
@@ -20,6 +21,7 @@
 #include "../../include/rewl_c.h"
 #include "aux_kernels.hpp"
 #include "sweep.hpp"
+#include "sweep_fast.hpp"
 
 using namespace rewl;
 
@@ -115,6 +117,10 @@ struct rewl_ctx {
     std::vector<int32_t> class_groups[4];
     int class_maxbins[4] = {0, 0, 0, 0};
     bool class_packed[4] = {false, false, false, false};
+    bool class_simple[4] = {true, true, true, true};   // plain [min, max) range test is exact
+    int class_maxlanes[4] = {0, 0, 0, 0};
+    int class_maxL[4] = {0, 0, 0, 0};
+    int nprod = 1;                                      // producer wavefronts of the fast path
     DevBuf<int32_t> d_class_groups[4];
     std::vector<int32_t> pairs_even, pairs_odd;
     DevBuf<int32_t> d_pairs_even, d_pairs_odd, d_flat;
@@ -181,6 +187,7 @@ int resolve_sweep_events(rewl_ctx* c) {
         float ms = 0.f;
         HIP_TRY(hipEventElapsedTime(&ms, e.first, e.second));
         c->stats.kernel_ms_sweep += ms;
+        c->stats.sweep_launches += 1;
         (void)hipEventDestroy(e.first);
         (void)hipEventDestroy(e.second);
     }
@@ -225,10 +232,38 @@ int replay_topup(rewl_ctx* c) {
     return REWL_OK;
 }
 
+// Philox mode on the integer path: the throughput kernel (sweep_fast.hpp).
+template <int MODEL, int NPROD>
+void* fast_kernel(bool simple) {
+    return simple ? (void*)k_sweep_fast<MODEL, true, NPROD> : (void*)k_sweep_fast<MODEL, false, NPROD>;
+}
+
+template <int MODEL>
+int launch_sweep_fast(rewl_ctx* c, int cls, int nsweeps) {
+    const int ng = (int)c->class_groups[cls].size();
+    const int nb = c->class_maxbins[cls];
+    const int np = c->nprod;
+    const size_t lanes = (size_t)c->class_maxlanes[cls];
+    const size_t lds = (size_t)nb * lanes * (sizeof(double) + sizeof(uint32_t)) +
+                       (np ? 2 * (size_t)c->class_maxL[cls] * lanes * sizeof(float) : 0);
+    const bool simple = c->class_simple[cls];
+    void* kern = np == 0 ? fast_kernel<MODEL, 0>(simple) : np == 1 ? fast_kernel<MODEL, 1>(simple)
+               : np == 2 ? fast_kernel<MODEL, 2>(simple) : fast_kernel<MODEL, 3>(simple);
+    HIP_TRY(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    KArgs args = c->args;
+    const int32_t* gl = c->d_class_groups[cls].p;
+    uint64_t base = c->sweep_index;
+    void* kargs[] = {&args, (void*)&gl, (void*)&ng, (void*)&nsweeps, (void*)&base};
+    HIP_TRY(hipLaunchKernel((const void*)kern, dim3((unsigned)ng), dim3(kLanes * (1 + np)), kargs, lds, c->stream));
+    HIP_TRY(hipGetLastError());
+    return REWL_OK;
+}
+
 template <int MODEL, bool INTP>
 int launch_sweep_class(rewl_ctx* c, int cls, int nsweeps) {
     const int ng = (int)c->class_groups[cls].size();
     if (ng == 0) return REWL_OK;
+    if (INTP && c->p.rng_mode == REWL_RNG_PHILOX) return launch_sweep_fast<MODEL>(c, cls, nsweeps);
     const bool packed = c->class_packed[cls];
     const int nb = c->class_maxbins[cls];
     const size_t lds = (size_t)nb * kLanes * sizeof(double) +
@@ -362,7 +397,6 @@ int rewl_create(const rewl_system_desc* systems, int32_t num_systems, const rewl
         return bail(fail(REWL_ERR_DEVICE, "hipStreamCreate failed"));
 
     // ---- wavefront groups -----------------------------------------------------
-    const int wpg = P.walkers_per_group > 0 ? P.walkers_per_group : kLanes;
     c->w_group.assign(W, 0);
     c->w_lane.assign(W, 0);
     c->w_hist.assign(W, 0);
@@ -371,6 +405,35 @@ int rewl_create(const rewl_system_desc* systems, int32_t num_systems, const rewl
         const HostSystem& S = c->sys[si];
         const int nw = S.d.num_windows * S.d.replicas_per_window;
         const int planes = S.d.model == REWL_ISING2D ? 1 : 2;
+        int maxbins = 1;
+        for (int k = 0; k < S.d.num_windows; ++k) maxbins = std::max(maxbins, c->win[S.first_window + k].nbins);
+        // The throughput kernel keeps FP64 ln g and u32 counts of every walker
+        // of the group in LDS with the bin stride equal to the group size, so
+        // a window of many bins gets fewer walkers per wavefront.
+        const bool fast = S.intpath && P.rng_mode == REWL_RNG_PHILOX;
+        c->nprod = P.producers < 0 ? 0 : (P.producers == 0 ? 1 : std::min<int>(P.producers, 3));
+        int wpg = kLanes;
+        if (fast)
+            wpg = std::min(kLanes, (int)((size_t)c->lds_max /
+                                         ((size_t)(maxbins + 1) * 12 + (c->nprod ? 8 * (size_t)S.d.L : 0))));
+        if (P.walkers_per_group > 0) wpg = std::min(wpg, (int)P.walkers_per_group);
+        if (wpg < 1) return bail(fail(REWL_ERR_INVALID, "window too wide for LDS: " + std::to_string(maxbins) + " bins"));
+        {
+            // E = -1 collides with E = 0 in the reference's +0.5 range test; it
+            // matters only if E = -1 is reachable and a window bound is 0.
+            const long long g = S.d.model == REWL_ISING2D ? 2LL * S.twoJ
+                                                          : 2LL * std::gcd((long long)S.twoJ, (long long)S.twoK);
+            const long long e0 = S.d.model == REWL_ISING2D ? -(long long)S.N * S.twoJ
+                                                           : -(long long)S.N * (2LL * S.twoJ + S.twoK);
+            bool zero_bound = false;
+            for (int k = 0; k < S.d.num_windows; ++k)
+                zero_bound = zero_bound || c->win[S.first_window + k].emin == 0.0 || c->win[S.first_window + k].emax == 0.0;
+            const bool m1_reachable = g == 0 ? (e0 == -1) : (((-1 - e0) % g) == 0);
+            const int cls = S.d.model * 2 + (S.intpath ? 1 : 0);
+            if (zero_bound && m1_reachable) c->class_simple[cls] = false;
+            c->class_maxlanes[cls] = std::max(c->class_maxlanes[cls], wpg);
+            c->class_maxL[cls] = std::max(c->class_maxL[cls], S.d.L);
+        }
         for (int off = 0; off < nw; off += wpg) {
             DevGroup G{};
             G.sys = si;
@@ -402,6 +465,13 @@ int rewl_create(const rewl_system_desc* systems, int32_t num_systems, const rewl
         const size_t full = (size_t)nb * kLanes * 12;
         const size_t packed = (size_t)nb * kLanes * 8 + (size_t)((nb + 1) / 2) * kLanes * 4;
         if (c->class_groups[cls].empty()) continue;
+        if ((cls & 1) && P.rng_mode == REWL_RNG_PHILOX) {
+            // throughput kernel: u32 counts, stride = group size (checked above)
+            if ((size_t)nb * c->class_maxlanes[cls] * 12 +
+                    (c->nprod ? 8 * (size_t)c->class_maxL[cls] * c->class_maxlanes[cls] : 0) > (size_t)c->lds_max)
+                return bail(fail(REWL_ERR_INVALID, "group too wide for LDS"));
+            continue;
+        }
         if (full <= (size_t)c->lds_max) c->class_packed[cls] = false;
         else if (packed <= (size_t)c->lds_max) c->class_packed[cls] = true;
         else return bail(fail(REWL_ERR_INVALID, "window too wide for LDS: " + std::to_string(nb) + " bins"));
@@ -894,6 +964,27 @@ int rewl_thermodynamics(const double* E, const double* lng, int32_t nb, int32_t 
     (void)hipFree(dE);
     (void)hipFree(dl);
     (void)hipFree(dout);
+    return REWL_OK;
+}
+
+int rewl_selfcheck(int32_t what, double* out) {
+    if (what != 0 || !out) return fail(REWL_ERR_INVALID, "unknown self-check");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(REWL_ERR_NO_DEVICE, "no HIP device");
+    unsigned long long* d = nullptr;
+    HIP_TRY(hipMalloc((void**)&d, sizeof(unsigned long long)));
+    HIP_TRY(hipMemset(d, 0, sizeof(unsigned long long)));
+    // all 2^32 words in 4 launches of 2^30
+    for (uint32_t q = 0; q < 4; ++q) {
+        hipLaunchKernelGGL(k_selfcheck_neglog, dim3(4096), dim3(256), 0, nullptr, q << 30, 1u << 30, d);
+        HIP_TRY(hipGetLastError());
+    }
+    unsigned long long bits = 0;
+    HIP_TRY(hipMemcpy(&bits, d, sizeof(bits), hipMemcpyDeviceToHost));
+    (void)hipFree(d);
+    double v;
+    std::memcpy(&v, &bits, sizeof(v));
+    *out = v;
     return REWL_OK;
 }
 

@@ -155,3 +155,59 @@ def test_halving_run_and_finalize_bit_exact(gpu, rng):
     Eo, lo = ora.stitched(0, 2.0)
     np.testing.assert_array_equal(Ee, Eo)
     np.testing.assert_allclose(le, lo, rtol=0, atol=1e-12)
+
+
+def test_neglog_band_covers_fp32_error(gpu):
+    """The throughput kernel screens u < exp(-d) with T = -ln u in FP32 and an
+    absolute band of 2^-12; every 32-bit Philox word must keep T well inside it."""
+    import ctypes as C
+    lib = abi.load_engine()
+    err = C.c_double()
+    abi.check(lib.rewl_selfcheck(0, C.byref(err)))
+    print(f"max |T - (-ln u)| = {err.value:.3e}", flush=True)
+    assert err.value < 2.0 ** -12 / 16
+
+
+@pytest.mark.parametrize("cfg", ["config2", "config3"])
+def test_production_configs_few_sweeps_bit_exact(gpu, cfg):
+    """Full-size workloads (L=64 Ising, 2048 walkers; L=32 Ashkin-Teller, 512
+    walkers): a few Philox sweeps and an exchange, bit-exact with the oracle."""
+    print(f"case: {cfg} few sweeps", flush=True)
+    systems, params = getattr(configs, cfg)()
+    eng, ora = make_pair(systems, params)
+    eng.adjust_to_range(); ora.adjust_to_range()
+    eng.sweeps(3); ora.sweeps(3)
+    eng.exchange(); ora.exchange()
+    eng.sweeps(2); ora.sweeps(2)
+    eng.check(); ora.check()
+    rng = np.random.default_rng(0)
+    for g in sorted(set(rng.integers(0, eng.num_walkers, 24).tolist()) | {0, eng.num_walkers - 1}):
+        st = ora.state(g)
+        assert eng.energy(g) == st["E"], g
+        np.testing.assert_array_equal(eng.spins(g), ora.spins(g))
+        np.testing.assert_array_equal(bits(eng.logdos(g)), bits(ora.logdos(g)))
+        np.testing.assert_array_equal(eng.counts(g).astype(np.uint64), ora.counts(g))
+
+
+@pytest.mark.parametrize("producers", [-1, 1, 2, 3])
+@pytest.mark.parametrize("model", [abi.ISING2D, abi.ASHKIN_TELLER2D])
+def test_fast_path_producers_bit_exact(gpu, producers, model):
+    """The throughput kernel with the walkers drawing their own Philox words
+    (producers = -1) or with 1-3 producer wavefronts filling the threshold
+    ring: identical chains, bit-exact with the oracle."""
+    print(f"case: fast path producers={producers} model={model}", flush=True)
+    if model == abi.ISING2D:
+        s = configs.ising_full_range(12, 8.0, 3, 7)
+    else:
+        s = configs.at_system(12, -5.0 * 144, 3.0 * 144 + 8.0, 8.0, 3, 5)
+    p = configs.params(producers=producers, lnf_initial=0.02, flatness_mode=abi.FLAT_MIN_OVER_MEAN, flatness=0.5,
+                       schedule=abi.SCHED_HALVE, sweeps_per_check=50, sweeps_per_exchange=10)
+    eng, ora = make_pair([s], p)
+    eng.adjust_to_range(); ora.adjust_to_range()
+    for _ in range(4):
+        for _ in range(5):
+            eng.sweeps(10); ora.sweeps(10)
+            eng.exchange(); ora.exchange()
+        eng.check(); ora.check()
+    assert_same(eng, ora, "after 200 sweeps", cursors=False)
+    window_states_equal(eng, ora)
