@@ -35,6 +35,10 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--walkers-per-group", type=int, default=0)
     ap.add_argument("--producers", type=int, default=0, help="producer wavefronts (0 auto, -1 none)")
+    ap.add_argument("--converge", type=float, default=0.0, metavar="LNF",
+                    help="instead of the throughput steps, time the run from the first sweep until every "
+                         "window's ln f is below LNF")
+    ap.add_argument("--max-checks", type=int, default=200000)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="only run a plain device-to-device copy of BYTES (counter calibration) and exit")
     return ap.parse_args()
@@ -70,6 +74,41 @@ def cpu_baseline(systems, params, target_s, threads):
                        f"initial state, ln f = 1, {att} attempts in {sec:.2f} s")
 
 
+def converge(eng, params, a):
+    """Time to ln f < a.converge: wall time from the first sweep until the
+    slowest window's ln f drops below the threshold (the reference's
+    f < final_increment test, rewl_simulation.hpp:594, 636-671)."""
+    per_check = params["sweeps_per_check"] // params["sweeps_per_exchange"]
+    eng.synchronize()
+    st0 = eng.stats()
+    t0 = time.perf_counter()
+    checks = 0
+    lnf = 1.0
+    while checks < a.max_checks:
+        for _ in range(per_check):
+            eng.sweeps(params["sweeps_per_exchange"])
+            eng.exchange()
+        eng.check()
+        checks += 1
+        if checks % 10 == 0 or eng.stats()["all_done"]:
+            lnf = max(eng.window_info(k)["lnf"] for k in range(eng.num_windows))
+            if checks % 200 == 0:
+                print(f"# check {checks}: max ln f {lnf:.3e}, {time.perf_counter() - t0:.1f} s", flush=True)
+            if lnf < a.converge:
+                break
+    eng.synchronize()
+    t1 = time.perf_counter()
+    st1 = eng.stats()
+    print(json.dumps({
+        "metric": "time to ln f < %g" % a.converge, "value": t1 - t0, "unit": "s", "higher_is_better": False,
+        "converged": bool(lnf < a.converge), "checks": checks, "sweeps": st1["sweeps"] - st0["sweeps"],
+        "attempts": st1["attempts"] - st0["attempts"],
+        "attempts_per_s": (st1["attempts"] - st0["attempts"]) / (t1 - t0),
+        "flat_events": st1["flat_events"], "max_lnf": lnf,
+        "config": {"workload": "2D Ising REWL, L=64 periodic, 32 windows x 64 walkers, flatness 0.8, halving"},
+    }), flush=True)
+
+
 def main():
     a = parse()
     if a.calibrate_copy:
@@ -97,6 +136,9 @@ def main():
             eng.exchange()
         eng.check()
 
+    if a.converge > 0:
+        converge(eng, params, a)
+        return
     for _ in range(a.warmup):
         step()
     eng.synchronize()

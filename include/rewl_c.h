@@ -66,7 +66,10 @@ enum rewl_flatness {
                                     every bin visited (rewl_histograms.hpp:115-130)  */
     REWL_FLAT_RATIO         = 1, /* classic: (max-min)/min <= tol, float arithmetic
                                     (rewl_histograms.hpp:133-152)                    */
-    REWL_FLAT_MIN_OVER_MEAN = 2  /* min >= tol * mean (the usual WL criterion)        */
+    REWL_FLAT_MIN_OVER_MEAN = 2, /* min >= tol * mean (the usual WL criterion)        */
+    REWL_FLAT_MIN_OVER_MEAN_VISITED = 3  /* the same over the bins some replica of the
+                                    window has visited: for continuous (random-field)
+                                    spectra whose empty bins are not known in advance */
 };
 
 enum rewl_schedule {
@@ -119,7 +122,12 @@ typedef struct rewl_params {
     int32_t producers;           /* Philox fast path: wavefronts per workgroup that
                                     precompute the acceptance thresholds; 0 = auto,
                                     <0 = none (the walkers draw their own)          */
-    int32_t reserved[6];
+    int32_t adjust_mode;         /* range adjustment: 0 = the reference's scan, whose
+                                    Ashkin-Teller flavour toggles twice per wrap so
+                                    only sigma flips (enough for [E0, 0)); 1 = the
+                                    same scan alternating sigma and tau passes, which
+                                    reaches the upper half of the AT spectrum        */
+    int32_t reserved[5];
 } rewl_params;
 
 /* Per-window bookkeeping, one record per (system, window). */

@@ -240,7 +240,7 @@ struct Proposal { double E, M, Ms, Mt, Bax, sg, ta; };
 
 /* change_state: the proposal for site i; for AT it also advances the flavour
  * after site N-1 (ashkin_teller2d_hamiltonian.hpp:223-250). */
-inline Proposal change_state(const System& S, Walker& w, int i) {
+inline Proposal change_state(const System& S, Walker& w, int i, bool toggle_at_end = true) {
     Proposal t{};
     if (S.d.model == REWL_ISING2D) {
         const double cur = w.s[i];
@@ -257,7 +257,7 @@ inline Proposal change_state(const System& S, Walker& w, int i) {
         t.Mt = w.Mt + tt - ct;
         t.Bax = w.Bax + (ts * tt) - (cs * ct);
         t.E = w.E + at_local_energy(S, w, i, ts, tt) - at_local_energy(S, w, i, cs, ct);
-        if (i == S.N - 1) w.which ^= 1;
+        if (toggle_at_end && i == S.N - 1) w.which ^= 1;
     }
     return t;
 }
@@ -332,7 +332,8 @@ int adjust_walker(ora_ctx& c, System& S, Walker& w) {
         if (++guard > cap) return fail(REWL_ERR_RANGE, "walker could not be brought into its window");
         ++site;
         if (site == (size_t)S.N) { site = 0; switch_flavor(S, w); }
-        Proposal t = change_state(S, w, (int)site);
+        /* adjust_mode 1: only the wrap switches the flavour, so passes alternate */
+        Proposal t = change_state(S, w, (int)site, c.p.adjust_mode == 0);
         if (w.hi.too_low(w.E) && t.E > w.E) set_state(S, w, (int)site, t);
         else if (w.hi.too_high(w.E) && t.E <= w.E) set_state(S, w, (int)site, t);
         in = w.hi.in_range(w.E);
@@ -341,7 +342,7 @@ int adjust_walker(ora_ctx& c, System& S, Walker& w) {
 }
 
 /* ---------------- flatness (rewl_histograms.hpp:115-152) ------------------ */
-bool walker_flat(const ora_ctx& c, const Window& win, const Walker& w) {
+bool walker_flat(const ora_ctx& c, const Window& win, const Walker& w, const std::vector<uint8_t>& unseen) {
     if (c.p.schedule == REWL_SCHED_ONE_OVER_T && win.oot) return true;
     const size_t n = win.nbins;
     if (c.p.flatness_mode == REWL_FLAT_ALL_VISITED) {
@@ -352,6 +353,7 @@ bool walker_flat(const ora_ctx& c, const Window& win, const Walker& w) {
     uint64_t mn = UINT64_MAX, mx = 0, sum = 0; size_t ne = 0;
     for (size_t b = 0; b < n; ++b) {
         if (win.exclude[b]) continue;
+        if (c.p.flatness_mode == 3 && unseen[b]) continue;   /* never visited by any replica */
         mn = std::min(mn, w.cnt[b]); mx = std::max(mx, w.cnt[b]); sum += w.cnt[b]; ++ne;
     }
     if (ne == 0) return true;
@@ -558,8 +560,14 @@ int ora_check(ora_ctx* c, int32_t* all_done) {
     for (auto& S : c->sys) {
         for (int k = 0; k < S.W; ++k) {
             Window& win = S.windows[k];
+            std::vector<uint8_t> unseen(win.nbins, 1);
+            for (int r = 0; r < S.R; ++r) {
+                const Walker& w = c->w[S.first + k * S.R + r];
+                for (size_t b = 0; b < win.nbins; ++b)
+                    if (w.visits[b] + w.cnt[b] != 0) unseen[b] = 0;
+            }
             bool flat = true;
-            for (int r = 0; r < S.R; ++r) flat = flat && walker_flat(*c, win, c->w[S.first + k * S.R + r]);
+            for (int r = 0; r < S.R; ++r) flat = flat && walker_flat(*c, win, c->w[S.first + k * S.R + r], unseen);
             if (flat) {
                 average_window(*c, S, k, false);
                 for (int r = 0; r < S.R; ++r) reset_counts(*c, c->w[S.first + k * S.R + r]);

@@ -24,6 +24,7 @@ ASHKIN_TELLER2D = 1
 FLAT_ALL_VISITED = 0
 FLAT_RATIO = 1
 FLAT_MIN_OVER_MEAN = 2
+FLAT_MIN_OVER_MEAN_VISITED = 3
 
 SCHED_HALVE = 0
 SCHED_ONE_OVER_T = 1
@@ -66,7 +67,8 @@ class Params(C.Structure):
         ("walkers_per_group", C.c_int32),
         ("exchange", C.c_int32),
         ("producers", C.c_int32),
-        ("reserved", C.c_int32 * 6),
+        ("adjust_mode", C.c_int32),
+        ("reserved", C.c_int32 * 5),
     ]
 
 

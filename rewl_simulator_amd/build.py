@@ -11,6 +11,7 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 ENGINE_SO = os.path.join(PKG, "librewl_hip.so")
+HOST_DIR = os.path.join(PKG, "host")
 ORACLE_DIR = os.path.join(ROOT, "oracle")
 ORACLE_SO = os.path.join(ORACLE_DIR, "liboracle.so")
 
@@ -39,6 +40,21 @@ def build_engine(verbose=False):
     return ENGINE_SO
 
 
+def build_host(verbose=False):
+    """The C++ host program over rewl_host.hpp, for both model plug-ins."""
+    outs = []
+    for name, defs in (("rewl_run", []), ("rewl_run_at", ["-DREWL_MODEL_AT"])):
+        out = os.path.join(HOST_DIR, name)
+        cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-Wextra", *defs, "-I", os.path.join(ROOT, "include"),
+               "-I", HOST_DIR, os.path.join(HOST_DIR, "rewl_run.cpp"), "-o", out, "-L", PKG, "-lrewl_hip",
+               "-Wl,-rpath,$ORIGIN/.."]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.run(cmd, check=True)
+        outs.append(out)
+    return outs
+
+
 def build_oracle(verbose=False):
     cmd = ["make", "-C", ORACLE_DIR, "-B", "liboracle.so"]
     if verbose:
@@ -48,9 +64,11 @@ def build_oracle(verbose=False):
 
 
 def main(argv):
-    what = argv[1:] or ["engine", "oracle"]
+    what = argv[1:] or ["engine", "host", "oracle"]
     if "engine" in what:
         print(build_engine(verbose=True))
+    if "host" in what:
+        print(build_host(verbose=True))
     if "oracle" in what:
         print(build_oracle(verbose=True))
 

@@ -26,6 +26,7 @@ REFERENCE_PARAMS = dict(
     walkers_per_group=0,
     exchange=1,
     producers=0,
+    adjust_mode=0,
 )
 
 DEFAULT_SEED = 20240501
@@ -100,7 +101,7 @@ def config3(lnf_final=1e-7):
     N = 32 * 32
     s = at_system(32, -5.0 * N, 3.0 * N + 8.0, 8.0, 16, 32)
     p = params(flatness_mode=abi.FLAT_MIN_OVER_MEAN, flatness=0.8, schedule=abi.SCHED_HALVE, lnf_final=lnf_final,
-               sweeps_per_check=1000, sweeps_per_exchange=1000)
+               sweeps_per_check=1000, sweeps_per_exchange=1000, adjust_mode=1)
     return [s], p
 
 
@@ -110,3 +111,48 @@ def gaussian_strain(L, sigma, seed):
     random_fields.hpp:279-313, so no realisation of it can be reproduced)."""
     rng = np.random.default_rng(seed)
     return rng.normal(0.0, sigma, size=L * L)
+
+
+def at_energy(L, sigma, tau, J=1.0, K=0.5, field=None):
+    """Ashkin-Teller energy of a configuration (numpy bond sum)."""
+    s = np.asarray(sigma, float).reshape(L, L)
+    t = np.asarray(tau, float).reshape(L, L)
+    E = 0.0
+    for ax in (0, 1):
+        ss = s * np.roll(s, -1, ax)
+        tt = t * np.roll(t, -1, ax)
+        E += float((-J * (ss + tt) - K * ss * tt).sum())
+    if field is not None:
+        E -= float((np.asarray(field).reshape(L, L) * s * t).sum())
+    return E
+
+
+def rfat_range(L, field, J=1.0, K=0.5):
+    """[E(all-up), E(sigma = tau = checkerboard)] of a random-strain
+    Ashkin-Teller realisation: the ground state for |h_i| < 4(J+K) and the
+    state with every bond at its maximum."""
+    N = L * L
+    up = np.ones(N)
+    chk = np.array([1.0 if (i % L + i // L) % 2 == 0 else -1.0 for i in range(N)])
+    return at_energy(L, up, up, J, K, field), at_energy(L, chk, chk, J, K, field)
+
+
+def config4(realisations=256, sizes=(16, 24, 32, 48), sigma=0.5, bins=512, windows=8, replicas=4,
+            lnf_final=1e-6, base_seed=1000):
+    """Batched random-strain Ashkin-Teller: H + -sum_i h_i s_i t_i with
+    h_i ~ N(0, sigma^2) per realisation, sizes cycled over `sizes`, 8 windows x
+    4 walkers each, `bins` uniform bins over [E(all-up), E(checkerboard)] with
+    the top level included; all realisations in one context (one launch per
+    sweep step).  Flatness is taken over the bins some replica has visited:
+    the low end of a disordered spectrum has empty bins whose position depends
+    on the realisation."""
+    systems = []
+    for r in range(realisations):
+        L = sizes[r % len(sizes)]
+        h = gaussian_strain(L, sigma, seed=base_seed + r)
+        e0, e1 = rfat_range(L, h)
+        bs = (e1 - e0) / (bins - 1)
+        systems.append(at_system(L, e0, e0 + bs * bins, bs, windows, replicas, field=h, seed=base_seed + r))
+    p = params(flatness_mode=abi.FLAT_MIN_OVER_MEAN_VISITED, flatness=0.8, schedule=abi.SCHED_HALVE,
+               lnf_final=lnf_final, sweeps_per_check=1000, sweeps_per_exchange=1000, adjust_mode=1)
+    return systems, p

@@ -41,7 +41,8 @@ __global__ void k_recalc(KArgs a) {
 // Proposal of REWL_Walker::adjust_state_to_range via change_state; FP64 as in
 // the reference.  Returns the proposed energy; the AT flavour advances after
 // site N-1 exactly as change_state does.
-__device__ double propose_fp(const DevSystem& S, const uint64_t* lat, int64_t base, int site, int& fl, double E) {
+__device__ double propose_fp(const DevSystem& S, const uint64_t* lat, int64_t base, int site, int& fl, double E,
+                             bool toggle_at_end = true) {
     double En;
     if (S.model == 0) {
         const double sv = spin_at(lat, base, S.L, 0, site);
@@ -53,7 +54,7 @@ __device__ double propose_fp(const DevSystem& S, const uint64_t* lat, int64_t ba
         ts += -2. * ts * (double)(fl == 0);
         tt += -2. * tt * (double)(fl == 1);
         En = E + at_local_energy_mem(S, lat, base, site, ts, tt) - at_local_energy_mem(S, lat, base, site, cs, ct);
-        if (site == S.N - 1) fl ^= 1;
+        if (toggle_at_end && site == S.N - 1) fl ^= 1;
     }
     return En;
 }
@@ -61,8 +62,9 @@ __device__ double propose_fp(const DevSystem& S, const uint64_t* lat, int64_t ba
 // REWL_Walker::adjust_state_to_range (rewl_walker.hpp:71-115): one draw picks
 // the starting site, then sites are scanned in order, accepting flips that move
 // the energy toward the window.
+// mode 1 keeps only the toggle at the wrap, so sigma and tau passes alternate.
 template <bool REPLAY>
-__global__ void k_adjust(KArgs a, uint64_t max_steps) {
+__global__ void k_adjust(KArgs a, uint64_t max_steps, int mode) {
     const int g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= a.num_walkers) return;
     const DevWindow W = a.win[a.w_window[g]];
@@ -88,7 +90,7 @@ __global__ void k_adjust(KArgs a, uint64_t max_steps) {
         ++site;
         if (site == (uint64_t)S.N) { site = 0; if (S.model == 1) fl ^= 1; }
         const int plane = (S.model == 1 && fl == 1) ? 1 : 0;
-        const double En = propose_fp(S, lat, base, (int)site, fl, E);
+        const double En = propose_fp(S, lat, base, (int)site, fl, E, mode == 0);
         const bool low = trunc_half(E) < W.phi_lo, high = trunc_half(E) >= W.phi_hi;
         if ((low && En > E) || (!low && high && En <= E)) {
             const int x = (int)site % S.L, y = (int)site / S.L;
@@ -259,6 +261,8 @@ __device__ __forceinline__ T wave_sum(T v) {
     return v;
 }
 
+constexpr int kMaxCheckBins = 8192;   // largest window the visited-support flatness test handles
+
 struct CheckParams {
     int32_t flatness_mode;
     int32_t schedule;
@@ -281,6 +285,19 @@ __global__ void __launch_bounds__(kLanes) k_check(KArgs a, CheckParams cp, int32
     const uint8_t* ex = a.exclude + W.excl_off;
     const bool oot_on = cp.schedule == 1 && a.win_oot[wi];
 
+    // mode 3: bins that no replica of the window has ever visited are left out
+    __shared__ uint8_t unseen[kMaxCheckBins];
+    if (cp.flatness_mode == 3) {
+        for (int b = lane; b < nb; b += kLanes) {
+            bool seen = false;
+            for (int r = 0; r < R && !seen; ++r) {
+                const int64_t i = a.w_hist[W.first_walker + r] + (int64_t)b * kLanes;
+                seen = (a.visits[i] + a.cnt[i]) != 0ull;
+            }
+            unseen[b] = seen ? 0 : 1;
+        }
+        __syncthreads();
+    }
     bool flat = true;
     if (!oot_on) {
         for (int r = 0; r < R && flat; ++r) {
@@ -290,6 +307,7 @@ __global__ void __launch_bounds__(kLanes) k_check(KArgs a, CheckParams cp, int32
             int ne = 0, zero = 0;
             for (int b = lane; b < nb; b += kLanes) {
                 if (ex[b]) continue;
+                if (cp.flatness_mode == 3 && unseen[b]) continue;
                 const uint32_t c = a.cnt[h + (int64_t)b * kLanes];
                 mn = c < mn ? c : mn;
                 mx = c > mx ? c : mx;

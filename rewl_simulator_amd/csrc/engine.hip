@@ -326,8 +326,8 @@ int rewl_create(const rewl_system_desc* systems, int32_t num_systems, const rewl
         return fail(REWL_ERR_NO_DEVICE, "no HIP device visible: the REWL engine runs only on the GPU");
     const rewl_params& P = *params;
     if (P.sweeps_per_check <= 0 || P.sweeps_per_exchange <= 0 || P.lnf_initial <= 0 ||
-        (P.flatness_mode < 0 || P.flatness_mode > 2) || (P.schedule < 0 || P.schedule > 1) ||
-        (P.rng_mode < 0 || P.rng_mode > 1) || P.walkers_per_group < 0 || P.walkers_per_group > kLanes)
+        (P.flatness_mode < 0 || P.flatness_mode > 3) || (P.schedule < 0 || P.schedule > 1) ||
+        (P.rng_mode < 0 || P.rng_mode > 1) || (P.adjust_mode < 0 || P.adjust_mode > 1) || P.walkers_per_group < 0 || P.walkers_per_group > kLanes)
         return fail(REWL_ERR_INVALID, "invalid rewl_params");
 
     rewl_ctx* c = new rewl_ctx();
@@ -358,6 +358,8 @@ int rewl_create(const rewl_system_desc* systems, int32_t num_systems, const rewl
         bool ip = !S.has_field && integral(tJ) && (d.model == REWL_ISING2D || integral(tK));
         for (auto& w : wins) {
             if (w.nbins < 1) return bail(fail(REWL_ERR_INVALID, "a window has no bins"));
+            if (P.flatness_mode == REWL_FLAT_MIN_OVER_MEAN_VISITED && w.nbins > kMaxCheckBins)
+                return bail(fail(REWL_ERR_INVALID, "visited-support flatness handles at most 8192 bins per window"));
             const double lb = std::log2(w.bs);
             ip = ip && integral(w.emin) && integral(w.emax) && integral(w.bs) && integral(lb);
         }
@@ -616,10 +618,13 @@ int rewl_adjust_to_range(rewl_ctx* c) {
     int rc;
     if ((rc = replay_topup(c))) return rc;
     uint64_t steps = (uint64_t)max_ndof(c) * 4096ull;
+    const int mode = c->p.adjust_mode;
     if (c->p.rng_mode == REWL_RNG_REPLAY)
-        hipLaunchKernelGGL(k_adjust<true>, dim3((c->num_walkers + 63) / 64), dim3(64), 0, c->stream, c->args, steps);
+        hipLaunchKernelGGL(k_adjust<true>, dim3((c->num_walkers + 63) / 64), dim3(64), 0, c->stream, c->args, steps,
+                           mode);
     else
-        hipLaunchKernelGGL(k_adjust<false>, dim3((c->num_walkers + 63) / 64), dim3(64), 0, c->stream, c->args, steps);
+        hipLaunchKernelGGL(k_adjust<false>, dim3((c->num_walkers + 63) / 64), dim3(64), 0, c->stream, c->args, steps,
+                           mode);
     HIP_TRY(hipGetLastError());
     return check_err_flag(c);
 }

@@ -29,7 +29,7 @@ constexpr float kLn2 = 0.693147180559945309f;
 // -ln(u) for u = k * 2^-32, NaN for k == 0 (forces the exact path).
 __device__ __forceinline__ float neg_log_u32(uint32_t k) {
     const float kf = (float)k;
-    const float t = (32.0f - log2f(kf)) * kLn2;   // log2f: 1 ulp (OCML)
+    const float t = (32.0f - __builtin_amdgcn_logf(kf)) * kLn2;   // v_log_f32 on kf >= 1
     return k == 0u ? __int_as_float(0x7fc00000) : t;
 }
 
@@ -239,44 +239,55 @@ __device__ void sweep_walker_fast(const KArgs& a, const DevGroup& G, const DevSy
 
 // Producer side of the ring: -ln(u) for every attempt position of one row of
 // one sweep pass, for every walker of the group, from the same Philox words
-// the walkers would draw themselves.
+// the walkers would draw themselves.  Producer lane j serves walker j of the
+// group; the NPROD producer wavefronts split the row's Philox blocks.
 template <int MODEL>
-__device__ void fill_row(const KArgs& a, const DevGroup& G, const DevSystem& S, int pt, int nthreads,
-                         uint64_t sweep_base, int row, float* ring_buf) {
+__device__ void fill_row(const DevGroup& G, const DevSystem& S, int wave, int nwaves, int lane, uint64_t sweep_base,
+                         int row, float* ring_buf) {
     constexpr int NPASS = MODEL == 0 ? 1 : 2;
     const int L = S.L, stride = G.nwalkers;
+    if (lane >= stride) return;
+    // walkers of a group are consecutive local ids of one system
+    const uint32_t walker = (uint32_t)(G.first_walker - S.first_walker + lane);
     const int per_sweep = NPASS * L;
     const int s = row / per_sweep, rem = row - s * per_sweep;
     const int pass = rem / L, y = rem - pass * L;
     const int p0 = pass * S.N + y * L;
-    const int b0 = p0 >> 2, nblk = ((p0 + L - 1) >> 2) - b0 + 1;
+    const int b0 = p0 >> 2, b1 = (p0 + L - 1) >> 2;
     const uint64_t B = (uint64_t)((S.ndof + 3) / 4);
     const uint64_t blk0 = (sweep_base + (uint64_t)s) * B;
-    const int items = stride * nblk;
-    for (int i = pt; i < items; i += nthreads) {
-        const int w = i % stride, blk = b0 + i / stride;
-        const uint32_t walker = (uint32_t)a.w_local[G.first_walker + w];
+    const bool aligned = ((p0 | L) & 3) == 0;
+    for (int blk = b0 + wave; blk <= b1; blk += nwaves) {
         const uint64_t bi = blk0 + (uint64_t)blk;
         const u32x4 r = philox4x32_10(u32x4{(uint32_t)bi, (uint32_t)(bi >> 32), DOMAIN_SWEEP, walker}, S.k0, S.k1);
+        const int x0 = blk * 4 - p0;
+        if (aligned) {
+            float* o = ring_buf + x0 * stride + lane;
+            o[0] = neg_log_u32(r.x);
+            o[stride] = neg_log_u32(r.y);
+            o[2 * stride] = neg_log_u32(r.z);
+            o[3 * stride] = neg_log_u32(r.w);
+        } else {
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const int x = blk * 4 + j - p0;
-            if (x >= 0 && x < L) ring_buf[x * stride + w] = neg_log_u32(word_of(r, j));
+            for (int j = 0; j < 4; ++j) {
+                const int x = x0 + j;
+                if (x >= 0 && x < L) ring_buf[x * stride + lane] = neg_log_u32(word_of(r, j));
+            }
         }
     }
 }
 
 template <int MODEL>
-__device__ void produce_thresholds(const KArgs& a, const DevGroup& G, const DevSystem& S, int pt, int nthreads,
-                                   int nsweeps, uint64_t sweep_base, float* ring) {
+__device__ void produce_thresholds(const DevGroup& G, const DevSystem& S, int wave, int nwaves, int lane, int nsweeps,
+                                   uint64_t sweep_base, float* ring) {
     constexpr int NPASS = MODEL == 0 ? 1 : 2;
     const int L = S.L;
     const int rows = nsweeps * NPASS * L;
     const size_t bufsz = (size_t)L * G.nwalkers;
-    fill_row<MODEL>(a, G, S, pt, nthreads, sweep_base, 0, ring);
+    fill_row<MODEL>(G, S, wave, nwaves, lane, sweep_base, 0, ring);
     __syncthreads();   // pairs with the walkers' barrier after loading the histograms
     for (int r = 0; r < rows; ++r) {
-        if (r + 1 < rows) fill_row<MODEL>(a, G, S, pt, nthreads, sweep_base, r + 1, ring + ((r + 1) & 1) * bufsz);
+        if (r + 1 < rows) fill_row<MODEL>(G, S, wave, nwaves, lane, sweep_base, r + 1, ring + ((r + 1) & 1) * bufsz);
         __syncthreads();   // pairs with the walkers' end-of-row barrier
     }
 }
@@ -297,7 +308,7 @@ __global__ void __launch_bounds__(256) k_sweep_fast(KArgs a, const int32_t* grou
         if (tid < kLanes)
             sweep_walker_fast<MODEL, SIMPLE, NPROD>(a, G, S, tid, nsweeps, sweep_base, s_lng, s_cnt, ring);
         else if (NPROD)
-            produce_thresholds<MODEL>(a, G, S, tid - kLanes, NPROD * kLanes, nsweeps, sweep_base, ring);
+            produce_thresholds<MODEL>(G, S, tid / kLanes - 1, NPROD, tid & (kLanes - 1), nsweeps, sweep_base, ring);
         if (NPROD) __syncthreads();   // the ring and LDS histograms are reused by the next group
     }
 }

@@ -1,0 +1,214 @@
+// rewl_host.hpp -- header-only C++17 host layer over the C ABI (rewl_c.h).
+//
+// It keeps the shape of the reference's compile-time plug-in surface so that a
+// program written against it reads the same:
+//   * model plug-ins Ising2d<data_t> and Ashkin_Teller2d<data_t>
+//     (ising2d_hamiltonian.hpp:64-160, ashkin_teller2d_hamiltonian.hpp:76-167)
+//     with their parameter blocks (ising2d_parameters.cxx:13-69,
+//     ashkin_teller2d_parameters.cxx:13-70) and the random-field / random-strain
+//     variants (RFIM, RFAT_BAXTER) as a per-site field;
+//   * the selector aliases Hamiltonian_t / System_Parameters of
+//     model_hamiltonians.hpp:6-40, chosen here by a template argument;
+//   * REWL_Parameters (rewl_parameters.hpp:30-49);
+//   * REWL_simulation<Hamiltonian> with simulate() and the results path of
+//     main.cpp:201-365 (stitch, normalise, thermodynamics).
+// Nothing here computes on the CPU: every step is a call into librewl_hip.so.
+#pragma once
+
+#include <cmath>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "rewl_c.h"
+
+namespace rewl_host {
+
+class Error : public std::runtime_error {
+  public:
+    Error(int code, const std::string& what) : std::runtime_error(what), code(code) {}
+    int code;
+};
+
+inline void check(int rc) {
+    if (rc != REWL_OK) throw Error(rc, std::string("rewl: ") + rewl_last_error());
+}
+
+// ---- model plug-ins ----------------------------------------------------------
+// Parameters of the 2D Ising model: L x L periodic, H = -J sum s_i s_j - sum h_i s_i.
+struct Ising2d_Parameters {
+    int L = 24;                      // SYSTEM_SIZE_L
+    double J = 1.0;                  // SYSTEM_EXCHANGE_J
+    double h = 0.0;                  // SYSTEM_FIELD_H (uniform)
+    std::vector<double> field;       // RFIM: one field per site (empty = uniform h)
+    int N() const { return L * L; }
+    int num_DoF() const { return N(); }
+    double ground_state_degeneracy() const { return 2.0; }
+    double states_per_site() const { return 2.0; }
+    // Wang-Landau range of the reference: [-2JN, 0), bin 8J (ising2d_parameters.cxx:44-54)
+    double energy_min() const { return -0.5 * J * 4.0 * N(); }
+    double energy_max() const { return 0.0; }
+    double energy_bin_size() const { return J * 4.0 * 2.0; }
+};
+
+// Ashkin-Teller: H = -sum [J(s s' + t t') + K s s' t t'] - sum h_i s_i t_i.
+struct Ashkin_Teller2d_Parameters {
+    int L = 24;
+    double J = 1.0;                  // SYSTEM_EXCHANGE_J
+    double K = 0.5;                  // SYSTEM_EXCHANGE_K
+    double h = 0.0;                  // uniform Baxter field
+    std::vector<double> field;       // RFAT_BAXTER: random strain per site
+    int N() const { return L * L; }
+    int num_DoF() const { return 2 * N(); }
+    double ground_state_degeneracy() const { return 4.0; }
+    double states_per_site() const { return 4.0; }
+    // [-(2J+K) 2N, 0), bin 8(J+K) (ashkin_teller2d_parameters.cxx:417-427)
+    double energy_min() const { return -0.5 * (2 * J + K) * 4.0 * N(); }
+    double energy_max() const { return 0.0; }
+    double energy_bin_size() const { return (J + K) * 4.0 * 2.0; }
+};
+
+template <typename data_t>
+struct Ising2d {
+    using Parameters = Ising2d_Parameters;
+    static constexpr int32_t model_id = REWL_ISING2D;
+    static void describe(const Parameters& P, rewl_system_desc& d) {
+        d.model = model_id;
+        d.L = P.L;
+        d.J = P.J;
+        d.K = 0.0;
+        d.h = P.h;
+        d.field = P.field.empty() ? nullptr : P.field.data();
+    }
+};
+
+template <typename data_t>
+struct Ashkin_Teller2d {
+    using Parameters = Ashkin_Teller2d_Parameters;
+    static constexpr int32_t model_id = REWL_ASHKIN_TELLER2D;
+    static void describe(const Parameters& P, rewl_system_desc& d) {
+        d.model = model_id;
+        d.L = P.L;
+        d.J = P.J;
+        d.K = P.K;
+        d.h = P.h;
+        d.field = P.field.empty() ? nullptr : P.field.data();
+    }
+};
+
+// ---- REWL parameters (rewl_parameters.hpp:30-49) ---------------------------------
+struct REWL_Parameters {
+    int num_windows = 8;
+    int replicas_per_window = 1;
+    double window_overlap = 0.75;
+    int sweeps_per_check = 1000;
+    int sweeps_per_exchange = 1;
+    double final_increment = 1e-7;
+    double flatness_criterion = 0.3;
+    int flatness_mode = REWL_FLAT_ALL_VISITED;   // ONE_OVER_T_ALGORITHM default
+    int schedule = REWL_SCHED_ONE_OVER_T;
+    double initial_increment = 1.0;
+    bool reduce_logdos = true;
+    bool replay = false;                         // per-walker mt19937_64 streams
+    uint64_t seed = 1;
+    int adjust_mode = 0;                         // 1: alternate AT planes (full spectrum)
+};
+
+template <class Hamiltonian_t>
+class REWL_simulation {
+  public:
+    using System_Parameters = typename Hamiltonian_t::Parameters;
+
+    // energy range [emin, emax) with bins of bsize, as the reference's
+    // REWL_simulation(all_energy_min, all_energy_max, all_energy_binsize)
+    REWL_simulation(const System_Parameters& sp, const REWL_Parameters& rp, double emin, double emax, double bsize,
+                    const std::vector<uint8_t>& exclude_bins = {})
+        : sys_(sp), rp_(rp), exclude_(exclude_bins) {
+        rewl_system_desc d{};
+        Hamiltonian_t::describe(sys_, d);
+        d.e_min = emin;
+        d.e_max = emax;
+        d.bin_size = bsize;
+        d.num_windows = rp.num_windows;
+        d.replicas_per_window = rp.replicas_per_window;
+        d.overlap = rp.window_overlap;
+        d.seed = rp.seed;
+        d.exclude_bins = exclude_.empty() ? nullptr : exclude_.data();
+        d.walker_seeds = nullptr;
+        rewl_params p{};
+        p.flatness_mode = rp.flatness_mode;
+        p.flatness = rp.flatness_criterion;
+        p.schedule = rp.schedule;
+        p.lnf_initial = rp.initial_increment;
+        p.lnf_final = rp.final_increment;
+        p.sweeps_per_check = rp.sweeps_per_check;
+        p.sweeps_per_exchange = rp.sweeps_per_exchange;
+        p.reduce_logdos = rp.reduce_logdos ? 1 : 0;
+        p.rng_mode = rp.replay ? REWL_RNG_REPLAY : REWL_RNG_PHILOX;
+        p.walkers_per_group = 0;
+        p.exchange = 1;
+        p.producers = 0;
+        p.adjust_mode = rp.adjust_mode;
+        check(rewl_create(&d, 1, &p, &ctx_));
+        check(rewl_adjust_to_range(ctx_));
+    }
+
+    // the reference's default range for the model
+    REWL_simulation(const System_Parameters& sp, const REWL_Parameters& rp)
+        : REWL_simulation(sp, rp, sp.energy_min(), sp.energy_max(), sp.energy_bin_size()) {}
+
+    REWL_simulation(const REWL_simulation&) = delete;
+    REWL_simulation& operator=(const REWL_simulation&) = delete;
+    ~REWL_simulation() { rewl_destroy(ctx_); }
+
+    // REWL_simulation::simulate: until every window is at the final increment.
+    // Returns false when max_checks check cycles ran first.
+    bool simulate(int64_t max_checks = INT64_MAX / 2) {
+        const int rc = rewl_run(ctx_, max_checks);
+        if (rc == REWL_ERR_ITER_CAP) return false;
+        check(rc);
+        check(rewl_finalize(ctx_));
+        return true;
+    }
+
+    // stitched and normalised (E, ln g), as main.cpp:309-325 produces them
+    void results(std::vector<double>& energies, std::vector<double>& logdos) const {
+        int32_t nw = 0, n = 0, cap = 0;
+        check(rewl_num_windows(ctx_, &nw));
+        for (int32_t k = 0; k < nw; ++k) {
+            rewl_window_info wi{};
+            check(rewl_get_window_info(ctx_, k, &wi));
+            cap += wi.num_bins;
+        }
+        energies.assign(cap, 0.0);
+        logdos.assign(cap, 0.0);
+        check(rewl_stitch(ctx_, 0, sys_.states_per_site(), energies.data(), logdos.data(), cap, &n));
+        energies.resize(n);
+        logdos.resize(n);
+    }
+
+    // Thermodynamics::calculate_thermodynamics: rows of (F, U, U^2, S, C) per site
+    std::vector<double> thermodynamics(double tmin = 0.01, double tmax = 50.01, int32_t nT = 50000) const {
+        std::vector<double> E, lg, out((size_t)nT * 5);
+        results(E, lg);
+        check(rewl_thermodynamics(E.data(), lg.data(), (int32_t)E.size(), sys_.N(), tmin, tmax, nT, out.data()));
+        return out;
+    }
+
+    rewl_stats stats() const {
+        rewl_stats s{};
+        check(rewl_get_stats(ctx_, &s));
+        return s;
+    }
+
+    rewl_ctx* context() const { return ctx_; }
+
+  private:
+    System_Parameters sys_;
+    REWL_Parameters rp_;
+    std::vector<uint8_t> exclude_;
+    rewl_ctx* ctx_ = nullptr;
+};
+
+}  // namespace rewl_host

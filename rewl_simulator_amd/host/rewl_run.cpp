@@ -1,0 +1,45 @@
+// rewl_run -- a small host program over rewl_host.hpp, the counterpart of the
+// reference's main() (main.cpp:18-418) without its file tree: it runs one REWL
+// simulation of a model chosen at compile time (-DREWL_MODEL_AT for
+// Ashkin-Teller, Ising otherwise, as model_hamiltonians.hpp chooses by macro)
+// and prints the normalised microcanonical ln g/N per energy bin.
+//
+//   rewl_run L windows replicas final_increment
+#include <cstdio>
+#include <cstdlib>
+
+#include "rewl_host.hpp"
+
+#ifdef REWL_MODEL_AT
+template <typename T>
+using Hamiltonian_t = rewl_host::Ashkin_Teller2d<T>;
+#else
+template <typename T>
+using Hamiltonian_t = rewl_host::Ising2d<T>;
+#endif
+using System_Parameters = Hamiltonian_t<double>::Parameters;
+
+int main(int argc, char** argv) {
+    System_Parameters sp;
+    sp.L = argc > 1 ? std::atoi(argv[1]) : 8;
+    rewl_host::REWL_Parameters rp;
+    rp.num_windows = argc > 2 ? std::atoi(argv[2]) : 2;
+    rp.replicas_per_window = argc > 3 ? std::atoi(argv[3]) : 2;
+    rp.final_increment = argc > 4 ? std::atof(argv[4]) : 1e-4;
+    try {
+        rewl_host::REWL_simulation<Hamiltonian_t<double>> sim(sp, rp);
+        const bool done = sim.simulate(1000000);
+        std::vector<double> E, lg;
+        sim.results(E, lg);
+        const rewl_stats st = sim.stats();
+        std::printf("# converged %d sweeps %llu flat_events %llu exchanges %llu\n", done ? 1 : 0,
+                    (unsigned long long)st.sweeps, (unsigned long long)st.flat_events,
+                    (unsigned long long)st.exchanges_accepted);
+        std::printf("# E/N, ln g/N\n");
+        for (size_t i = 0; i < E.size(); ++i) std::printf("%.15e,%.15e\n", E[i] / sp.N(), lg[i] / sp.N());
+        return done ? 0 : 3;
+    } catch (const rewl_host::Error& e) {
+        std::fprintf(stderr, "%s\n", e.what());
+        return 2;
+    }
+}

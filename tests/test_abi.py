@@ -42,7 +42,7 @@ int main(void) {
   printf("rewl_window_info %zu\n", sizeof(rewl_window_info));
   printf("rewl_stats %zu\n", sizeof(rewl_stats));
   F(rewl_system_desc, field) F(rewl_system_desc, seed) F(rewl_system_desc, walker_seeds)
-  F(rewl_params, lnf_initial) F(rewl_params, exchange) F(rewl_params, producers) F(rewl_params, reserved)
+  F(rewl_params, lnf_initial) F(rewl_params, exchange) F(rewl_params, producers) F(rewl_params, adjust_mode) F(rewl_params, reserved)
   F(rewl_window_info, lnf) F(rewl_window_info, one_over_t)
   F(rewl_stats, kernel_ms_sweep) F(rewl_stats, sweep_launches)
   return 0;

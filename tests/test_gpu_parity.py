@@ -211,3 +211,43 @@ def test_fast_path_producers_bit_exact(gpu, producers, model):
         eng.check(); ora.check()
     assert_same(eng, ora, "after 200 sweeps", cursors=False)
     window_states_equal(eng, ora)
+
+
+@pytest.mark.parametrize("rng", [abi.RNG_REPLAY, abi.RNG_PHILOX])
+def test_batched_mixed_sizes_bit_exact(gpu, rng):
+    """Several random-strain AT realisations of different L and one clean Ising
+    system in one context (one launch per model/path class), with the
+    visited-support flatness test: bit-exact with the oracle, which runs each
+    system with its own streams."""
+    print(f"case: batched mixed sizes rng={rng}", flush=True)
+    systems, params = configs.config4(realisations=4, sizes=(4, 6, 8, 6), bins=64, windows=3, replicas=2,
+                                      lnf_final=1e-3)
+    systems.append(configs.ising_full_range(6, 4.0, 2, 3, seed=99))
+    params = dict(params, rng_mode=rng, sweeps_per_check=40, sweeps_per_exchange=20, flatness=0.5)
+    eng, ora = make_pair(systems, params)
+    eng.adjust_to_range(); ora.adjust_to_range()
+    for cycle in range(12):
+        for _ in range(2):
+            eng.sweeps(20); ora.sweeps(20)
+            eng.exchange(); ora.exchange()
+        eng.check(); ora.check()
+    assert_same(eng, ora, "batched", cursors=(rng == abi.RNG_REPLAY))
+    window_states_equal(eng, ora)
+    assert eng.stats()["flat_events"] > 0
+
+
+@pytest.mark.parametrize("rng", [abi.RNG_REPLAY, abi.RNG_PHILOX])
+def test_ashkin_teller_full_spectrum_adjust_bit_exact(gpu, rng):
+    """Full AT spectrum [-5N, 3N]: the plane-alternating range adjustment
+    reaches the top windows; engine and oracle agree."""
+    print(f"case: AT full spectrum adjust rng={rng}", flush=True)
+    N = 100
+    s = configs.at_system(10, -5.0 * N, 3.0 * N + 8.0, 8.0, 20, 2)
+    p = configs.params(rng_mode=rng, adjust_mode=1, lnf_initial=0.05)
+    eng, ora = make_pair([s], p)
+    assert eng.windows[-1]["e_min"] > N   # above what sigma flips alone can reach
+    eng.adjust_to_range(); ora.adjust_to_range()
+    assert_same(eng, ora, "after adjust", cursors=(rng == abi.RNG_REPLAY))
+    assert eng.energy(eng.num_walkers - 1) >= eng.windows[-1]["e_min"]
+    eng.sweeps(50); ora.sweeps(50)
+    assert_same(eng, ora, "after 50 sweeps", cursors=(rng == abi.RNG_REPLAY))

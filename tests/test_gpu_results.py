@@ -1,0 +1,62 @@
+"""End-to-end results on the GPU: converged ln g against the exact density of
+states, the C++ host program, and the FP64 thermodynamics kernel."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from rewl_simulator_amd import abi, configs, exact
+from rewl_simulator_amd.engine import Engine, thermodynamics
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _anchored_error(E, lng, L, bin_size):
+    """max |ln g - exact| over reachable bins after matching the lowest bin
+    (a common anchor removes the normalisation convention)."""
+    exact_bins = exact.binned_lng(L, E[0], bin_size, len(E))
+    ok = np.isfinite(exact_bins)
+    d = (lng - exact_bins)[ok]
+    d -= d[0]
+    return np.max(np.abs(d))
+
+
+def test_config1_converges_to_exact(gpu):
+    """Config 1 (L=16 Ising, 4 windows x 2 walkers, flatness 0.8, halving to
+    1e-6): the stitched ln g against Beale's exact finite-lattice DoS."""
+    systems, params = configs.config1()
+    with Engine(systems, params) as eng:
+        eng.adjust_to_range()
+        assert eng.run(5000)
+        eng.finalize()
+        E, lng = eng.stitched(0, states_per_site=2.0)
+        err = _anchored_error(E, lng, 16, 4.0)
+        st = eng.stats()
+        print(f"config1: {st['sweeps']} sweeps, {st['flat_events']} flat events, max |d ln g| = {err:.4f}", flush=True)
+    assert err < 0.05
+
+
+def test_host_program_matches_exact(gpu):
+    """The C++ host program (rewl_host.hpp over the C ABI) with the reference's
+    default Ising range [E0, 0), bin 8, 1/t schedule."""
+    exe = os.path.join(ROOT, "rewl_simulator_amd", "host", "rewl_run")
+    out = subprocess.run([exe, "8", "2", "2", "1e-5"], capture_output=True, text=True, timeout=600)
+    assert out.returncode == 0, out.stderr
+    rows = [l for l in out.stdout.splitlines() if l and not l.startswith("#")]
+    data = np.array([[float(v) for v in r.split(",")] for r in rows])
+    E, lng = data[:, 0] * 64, data[:, 1] * 64
+    err = _anchored_error(E, lng, 8, 8.0)
+    print(f"host program L=8: max |d ln g| = {err:.4f}", flush=True)
+    assert err < 0.1
+
+
+def test_thermodynamics_kernel_matches_oracle(gpu):
+    """GPU FP64 trapezoid thermodynamics against the oracle's restatement of
+    Thermodynamics::calculate_thermodynamics on the exact L=16 ln g."""
+    E, lng = exact.ising_lng(16)
+    g = thermodynamics(E, lng, 256, 0.01, 50.01, 5000)
+    o = O.thermodynamics(E, lng, 256, 0.01, 50.01, 5000)
+    np.testing.assert_allclose(g, o, rtol=1e-11, atol=1e-12)

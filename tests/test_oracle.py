@@ -230,3 +230,17 @@ def test_thermodynamics_two_level_closed_form():
         assert row[2] == pytest.approx(x / (1 + x), rel=1e-14)
         assert row[0] == pytest.approx(-t * np.log((1 + x) / 2), rel=1e-13)
         assert row[4] == pytest.approx((x / (1 + x) - (x / (1 + x)) ** 2) / t ** 2, rel=1e-12)
+
+
+def test_config4_ranges_and_visited_flatness():
+    """Random-strain AT ranges: E(all-up) and E(checkerboard) from the bond sum;
+    the oracle runs the visited-support flatness mode to flat events."""
+    systems, params = configs.config4(realisations=2, sizes=(4, 6), bins=32, windows=2, replicas=2, lnf_final=1e-2)
+    for s in systems:
+        L = s["L"]
+        up = np.ones(2 * L * L, np.int8)
+        assert O.energy_of(s, up) == pytest.approx(s["e_min"], abs=1e-9)
+    o = O.Oracle(systems, dict(params, sweeps_per_check=50, sweeps_per_exchange=25, flatness=0.5))
+    o.adjust_to_range()
+    assert o.run(2000)
+    assert o.stats()["flat_events"] > 0
