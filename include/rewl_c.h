@@ -127,7 +127,12 @@ typedef struct rewl_params {
                                     only sigma flips (enough for [E0, 0)); 1 = the
                                     same scan alternating sigma and tau passes, which
                                     reaches the upper half of the AT spectrum        */
-    int32_t reserved[5];
+    int32_t average_mode;        /* intra-window ln g average: 0 = weighted by each
+                                    replica's counts, as rewl_simulation.hpp:393-397;
+                                    1 = equal weights 1/R, the alternative left in
+                                    rewl_simulation.hpp:401 (unbiased when replicas
+                                    also exchange with other windows)               */
+    int32_t reserved[4];
 } rewl_params;
 
 /* Per-window bookkeeping, one record per (system, window). */

@@ -366,7 +366,8 @@ bool walker_flat(const ora_ctx& c, const Window& win, const Walker& w, const std
 }
 
 /* count-weighted intra-window ln g average (rewl_simulation.hpp:361-454);
- * replicas folded in replica order: acc = t_0; acc += t_r. */
+ * replicas folded in replica order: acc = t_0; acc += t_r.  average_mode 1
+ * weighs every replica 1/R (the alternative of rewl_simulation.hpp:401). */
 void average_window(ora_ctx& c, System& S, int wi, bool final_pass) {
     const int R = S.R;
     if (R == 1) return;
@@ -374,7 +375,13 @@ void average_window(ora_ctx& c, System& S, int wi, bool final_pass) {
     Walker* rep = &c.w[S.first + wi * R];
     for (size_t b = 0; b < win.nbins; ++b) {
         double acc = 0.;
-        if (!final_pass) {
+        if (c.p.average_mode == 1) {
+            const double wgt = 1. / static_cast<double>(R);
+            for (int r = 0; r < R; ++r) {
+                double t = wgt * rep[r].lng[b];
+                acc = (r == 0) ? t : acc + t;
+            }
+        } else if (!final_pass) {
             uint64_t tot = 0;
             for (int r = 0; r < R; ++r) tot += rep[r].cnt[b];
             if (tot == 0) continue;

@@ -68,7 +68,8 @@ class Params(C.Structure):
         ("exchange", C.c_int32),
         ("producers", C.c_int32),
         ("adjust_mode", C.c_int32),
-        ("reserved", C.c_int32 * 5),
+        ("average_mode", C.c_int32),
+        ("reserved", C.c_int32 * 4),
     ]
 
 
@@ -107,6 +108,23 @@ class Stats(C.Structure):
 
 # Every function the header declares, with its argtypes.
 _P = C.c_void_p
+class MlpTrainParams(C.Structure):
+    _fields_ = [
+        ("steps", C.c_int64),
+        ("batch", C.c_int64),
+        ("lr", C.c_double),
+        ("lr_final", C.c_double),
+        ("beta1", C.c_double),
+        ("beta2", C.c_double),
+        ("eps", C.c_double),
+        ("seed", C.c_uint64),
+        ("warmup", C.c_int64),
+        ("reserved", C.c_int64 * 3),
+    ]
+
+
+MLP_NUM_PARAMS = 33793
+
 SIGNATURES = {
     "rewl_last_error": ([], C.c_char_p),
     "rewl_device_count": ([C.POINTER(C.c_int)], C.c_int),
@@ -139,6 +157,17 @@ SIGNATURES = {
     "rewl_thermodynamics": ([C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int32, C.c_int32, C.c_double,
                              C.c_double, C.c_int32, C.POINTER(C.c_double)], C.c_int),
     "rewl_selfcheck": ([C.c_int32, C.POINTER(C.c_double)], C.c_int),
+    # include/rewl_mlp.h
+    "rewl_mlp_create": ([C.c_uint64, C.POINTER(_P)], C.c_int),
+    "rewl_mlp_destroy": ([_P], C.c_int),
+    "rewl_mlp_get_params": ([_P, C.POINTER(C.c_float), C.c_int64], C.c_int),
+    "rewl_mlp_set_params": ([_P, C.POINTER(C.c_float), C.c_int64], C.c_int),
+    "rewl_mlp_forward": ([_P, C.POINTER(C.c_float), C.c_int64, C.POINTER(C.c_float)], C.c_int),
+    "rewl_mlp_gradient": ([_P, C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_int64,
+                           C.POINTER(C.c_float), C.POINTER(C.c_double)], C.c_int),
+    "rewl_mlp_train": ([_P, C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_int64,
+                        C.POINTER(MlpTrainParams), C.POINTER(C.c_float)], C.c_int),
+    "rewl_mlp_last_ms": ([_P, C.POINTER(C.c_double)], C.c_int),
 }
 
 _lib = None

@@ -27,6 +27,7 @@ REFERENCE_PARAMS = dict(
     exchange=1,
     producers=0,
     adjust_mode=0,
+    average_mode=0,
 )
 
 DEFAULT_SEED = 20240501

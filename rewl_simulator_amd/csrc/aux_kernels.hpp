@@ -267,7 +267,7 @@ struct CheckParams {
     int32_t flatness_mode;
     int32_t schedule;
     int32_t reduce_logdos;
-    int32_t pad;
+    int32_t average_mode;
     double flatness;
     double lnf_final;
     int64_t pending_sweeps;
@@ -327,7 +327,17 @@ __global__ void __launch_bounds__(kLanes) k_check(KArgs a, CheckParams cp, int32
     }
 
     if (flat) {
-        if (R > 1) {
+        if (R > 1 && cp.average_mode == 1) {
+            const double wgt = 1. / static_cast<double>(R);
+            for (int b = lane; b < nb; b += kLanes) {
+                double acc = 0.;
+                for (int r = 0; r < R; ++r) {
+                    const double t = wgt * a.lng[a.w_hist[W.first_walker + r] + (int64_t)b * kLanes];
+                    acc = (r == 0) ? t : acc + t;
+                }
+                for (int r = 0; r < R; ++r) a.lng[a.w_hist[W.first_walker + r] + (int64_t)b * kLanes] = acc;
+            }
+        } else if (R > 1) {
             for (int b = lane; b < nb; b += kLanes) {
                 unsigned long long tot = 0ull;
                 for (int r = 0; r < R; ++r) tot += a.cnt[a.w_hist[W.first_walker + r] + (int64_t)b * kLanes];
@@ -377,14 +387,24 @@ __global__ void __launch_bounds__(kLanes) k_check(KArgs a, CheckParams cp, int32
 }
 
 // Final intra-window average weighted by all-time visits
-// (rewl_simulation.hpp:687 with simulation_incomplete == false).
-__global__ void __launch_bounds__(kLanes) k_finalize(KArgs a) {
+// (rewl_simulation.hpp:687 with simulation_incomplete == false), or by 1/R.
+__global__ void __launch_bounds__(kLanes) k_finalize(KArgs a, int average_mode) {
     const int wi = blockIdx.x;
     if (wi >= a.num_windows) return;
     const DevWindow W = a.win[wi];
     const int R = W.R;
     if (R == 1) return;
     for (int b = threadIdx.x; b < W.nbins; b += kLanes) {
+        if (average_mode == 1) {
+            const double wgt = 1. / static_cast<double>(R);
+            double acc = 0.;
+            for (int r = 0; r < R; ++r) {
+                const double t = wgt * a.lng[a.w_hist[W.first_walker + r] + (int64_t)b * kLanes];
+                acc = (r == 0) ? t : acc + t;
+            }
+            for (int r = 0; r < R; ++r) a.lng[a.w_hist[W.first_walker + r] + (int64_t)b * kLanes] = acc;
+            continue;
+        }
         double tot = 0.;
         for (int r = 0; r < R; ++r) {
             const int64_t i = a.w_hist[W.first_walker + r] + (int64_t)b * kLanes;

@@ -327,7 +327,7 @@ int rewl_create(const rewl_system_desc* systems, int32_t num_systems, const rewl
     const rewl_params& P = *params;
     if (P.sweeps_per_check <= 0 || P.sweeps_per_exchange <= 0 || P.lnf_initial <= 0 ||
         (P.flatness_mode < 0 || P.flatness_mode > 3) || (P.schedule < 0 || P.schedule > 1) ||
-        (P.rng_mode < 0 || P.rng_mode > 1) || (P.adjust_mode < 0 || P.adjust_mode > 1) || P.walkers_per_group < 0 || P.walkers_per_group > kLanes)
+        (P.rng_mode < 0 || P.rng_mode > 1) || (P.adjust_mode < 0 || P.adjust_mode > 1) || (P.average_mode < 0 || P.average_mode > 1) || P.walkers_per_group < 0 || P.walkers_per_group > kLanes)
         return fail(REWL_ERR_INVALID, "invalid rewl_params");
 
     rewl_ctx* c = new rewl_ctx();
@@ -676,6 +676,7 @@ int rewl_check(rewl_ctx* c) {
     cp.flatness_mode = c->p.flatness_mode;
     cp.schedule = c->p.schedule;
     cp.reduce_logdos = c->p.reduce_logdos;
+    cp.average_mode = c->p.average_mode;
     cp.flatness = c->p.flatness;
     cp.lnf_final = c->p.lnf_final;
     cp.pending_sweeps = c->pending_sweeps;
@@ -714,7 +715,8 @@ int rewl_run(rewl_ctx* c, int64_t max_checks) {
 
 int rewl_finalize(rewl_ctx* c) {
     if (!c) return fail(REWL_ERR_INVALID, "null context");
-    hipLaunchKernelGGL(k_finalize, dim3((unsigned)c->win.size()), dim3(kLanes), 0, c->stream, c->args);
+    hipLaunchKernelGGL(k_finalize, dim3((unsigned)c->win.size()), dim3(kLanes), 0, c->stream, c->args,
+                       (int)c->p.average_mode);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(c->stream));
     return REWL_OK;
@@ -990,6 +992,336 @@ int rewl_selfcheck(int32_t what, double* out) {
     double v;
     std::memcpy(&v, &bits, sizeof(v));
     *out = v;
+    return REWL_OK;
+}
+
+}  // extern "C"
+
+// ---- warm-start MLP (include/rewl_mlp.h) ---------------------------------------------
+#include "../../include/rewl_mlp.h"
+#include "mlp.hpp"
+
+static_assert(rewl::mlp::NP == REWL_MLP_NUM_PARAMS, "parameter count");
+
+struct rewl_mlp {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    float* theta = nullptr;     // FP32 master
+    float* adam_m = nullptr;
+    float* adam_v = nullptr;
+    uint16_t* wq = nullptr;     // bf16 copies
+    float* grad = nullptr;      // NPART
+    float* part = nullptr;      // part_rows x NPART
+    int part_rows = 0;
+    float* dx = nullptr;        // data on the device
+    float* dt = nullptr;
+    float* dw = nullptr;
+    float* dy = nullptr;
+    int64_t data_cap = 0;
+    int64_t adam_t = 0;
+    double last_ms = 0.0;
+};
+
+namespace {
+
+constexpr int kMlpMaxGrid = 512;
+
+void mlp_free_data(rewl_mlp* m) {
+    for (float** p : {&m->dx, &m->dt, &m->dw, &m->dy})
+        if (*p) {
+            (void)hipFree(*p);
+            *p = nullptr;
+        }
+    m->data_cap = 0;
+}
+
+int mlp_reserve(rewl_mlp* m, int64_t n) {
+    if (n <= m->data_cap) return REWL_OK;
+    mlp_free_data(m);
+    HIP_TRY(hipMalloc((void**)&m->dx, (size_t)n * rewl::mlp::NIN * sizeof(float)));
+    HIP_TRY(hipMalloc((void**)&m->dt, (size_t)n * sizeof(float)));
+    HIP_TRY(hipMalloc((void**)&m->dw, (size_t)n * sizeof(float)));
+    HIP_TRY(hipMalloc((void**)&m->dy, (size_t)n * sizeof(float)));
+    m->data_cap = n;
+    return REWL_OK;
+}
+
+int mlp_grid(int64_t npts) {
+    const int64_t tiles = (npts + rewl::mlp::TILE - 1) / rewl::mlp::TILE;
+    return (int)std::max<int64_t>(1, std::min<int64_t>(tiles, kMlpMaxGrid));
+}
+
+// uploads x (and t, w when given); w == nullptr means unit weights
+int mlp_upload(rewl_mlp* m, const float* x, const float* t, const float* w, int64_t n, double* wsum) {
+    for (int64_t i = 0; i < (int64_t)n * rewl::mlp::NIN; ++i)
+        if (!std::isfinite(x[i])) return fail(REWL_ERR_NONFINITE, "rewl_mlp: non-finite input feature");
+    int rc = mlp_reserve(m, n);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpyAsync(m->dx, x, (size_t)n * rewl::mlp::NIN * sizeof(float), hipMemcpyHostToDevice, m->stream));
+    if (t) {
+        for (int64_t i = 0; i < n; ++i)
+            if (!std::isfinite(t[i])) return fail(REWL_ERR_NONFINITE, "rewl_mlp: non-finite target");
+        HIP_TRY(hipMemcpyAsync(m->dt, t, (size_t)n * sizeof(float), hipMemcpyHostToDevice, m->stream));
+        double s = 0.0;
+        if (w) {
+            for (int64_t i = 0; i < n; ++i) {
+                if (!(w[i] >= 0.f) || !std::isfinite(w[i]))
+                    return fail(REWL_ERR_INVALID, "rewl_mlp: weights must be finite and >= 0");
+                s += w[i];
+            }
+            HIP_TRY(hipMemcpyAsync(m->dw, w, (size_t)n * sizeof(float), hipMemcpyHostToDevice, m->stream));
+        } else {
+            s = (double)n;
+        }
+        if (!(s > 0.0)) return fail(REWL_ERR_INVALID, "rewl_mlp: weights sum to zero");
+        if (wsum) *wsum = s;
+    }
+    return REWL_OK;
+}
+
+rewl::mlp::Args mlp_args(const rewl_mlp* m, int64_t n, bool weighted) {
+    rewl::mlp::Args a{};
+    a.theta = m->theta;
+    a.wq = m->wq;
+    a.x = m->dx;
+    a.t = m->dt;
+    a.w = weighted ? m->dw : nullptr;
+    a.n = n;
+    a.npts = n;
+    return a;
+}
+
+int mlp_ensure_part(rewl_mlp* m, int rows) {
+    if (rows <= m->part_rows) return REWL_OK;
+    if (m->part) (void)hipFree(m->part);
+    m->part = nullptr;
+    m->part_rows = 0;
+    HIP_TRY(hipMalloc((void**)&m->part, (size_t)rows * rewl::mlp::NPART * sizeof(float)));
+    m->part_rows = rows;
+    return REWL_OK;
+}
+
+int mlp_check(const rewl_mlp* m) {
+    if (!m) return fail(REWL_ERR_INVALID, "rewl_mlp: null handle");
+    HIP_TRY(hipSetDevice(m->device));
+    return REWL_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int rewl_mlp_create(uint64_t seed, rewl_mlp** out) {
+    using namespace rewl::mlp;
+    if (!out) return fail(REWL_ERR_INVALID, "rewl_mlp_create: null output");
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+        return fail(REWL_ERR_NO_DEVICE, "no HIP device visible: the warm-start MLP runs only on the GPU");
+    // He-uniform weights from a splitmix64 stream, zero biases
+    std::vector<float> th(NP, 0.f);
+    uint64_t st = seed;
+    auto uni = [&st]() {
+        uint64_t z = (st += 0x9E3779B97F4A7C15ull);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        z ^= z >> 31;
+        return (double)(z >> 11) * 0x1p-53;
+    };
+    auto fill = [&](int off, int count, int fan_in) {
+        const double a = std::sqrt(6.0 / fan_in);
+        for (int i = 0; i < count; ++i) th[off + i] = (float)((2.0 * uni() - 1.0) * a);
+    };
+    fill(O_W1, H * NIN, NIN);
+    fill(O_W2, H * H, H);
+    fill(O_W3, H * H, H);
+    fill(O_W4, H, 4 * H);
+    auto* m = new rewl_mlp();
+    int rc = REWL_OK;
+    auto body = [&]() -> int {
+        HIP_TRY(hipGetDevice(&m->device));
+        HIP_TRY(hipStreamCreate(&m->stream));
+        HIP_TRY(hipEventCreate(&m->ev0));
+        HIP_TRY(hipEventCreate(&m->ev1));
+        HIP_TRY(hipMalloc((void**)&m->theta, NP * sizeof(float)));
+        HIP_TRY(hipMalloc((void**)&m->adam_m, NP * sizeof(float)));
+        HIP_TRY(hipMalloc((void**)&m->adam_v, NP * sizeof(float)));
+        HIP_TRY(hipMalloc((void**)&m->wq, NQ * sizeof(uint16_t)));
+        HIP_TRY(hipMalloc((void**)&m->grad, NPART * sizeof(float)));
+        return REWL_OK;
+    };
+    rc = body();
+    if (rc == REWL_OK) rc = rewl_mlp_set_params(m, th.data(), NP);
+    if (rc != REWL_OK) {
+        std::string keep = g_err;
+        rewl_mlp_destroy(m);
+        g_err = keep;
+        return rc;
+    }
+    *out = m;
+    return REWL_OK;
+}
+
+int rewl_mlp_destroy(rewl_mlp* m) {
+    if (!m) return REWL_OK;
+    (void)hipSetDevice(m->device);
+    if (m->stream) (void)hipStreamSynchronize(m->stream);
+    mlp_free_data(m);
+    for (float* p : {m->theta, m->adam_m, m->adam_v, m->grad, m->part})
+        if (p) (void)hipFree(p);
+    if (m->wq) (void)hipFree(m->wq);
+    if (m->ev0) (void)hipEventDestroy(m->ev0);
+    if (m->ev1) (void)hipEventDestroy(m->ev1);
+    if (m->stream) (void)hipStreamDestroy(m->stream);
+    delete m;
+    return REWL_OK;
+}
+
+int rewl_mlp_get_params(const rewl_mlp* m, float* theta, int64_t n) {
+    int rc = mlp_check(m);
+    if (rc) return rc;
+    if (!theta || n != rewl::mlp::NP) return fail(REWL_ERR_INVALID, "rewl_mlp_get_params: need REWL_MLP_NUM_PARAMS values");
+    HIP_TRY(hipMemcpyAsync(theta, m->theta, (size_t)n * sizeof(float), hipMemcpyDeviceToHost, m->stream));
+    HIP_TRY(hipStreamSynchronize(m->stream));
+    return REWL_OK;
+}
+
+int rewl_mlp_set_params(rewl_mlp* m, const float* theta, int64_t n) {
+    using namespace rewl::mlp;
+    int rc = mlp_check(m);
+    if (rc) return rc;
+    if (!theta || n != NP) return fail(REWL_ERR_INVALID, "rewl_mlp_set_params: need REWL_MLP_NUM_PARAMS values");
+    for (int64_t i = 0; i < n; ++i)
+        if (!std::isfinite(theta[i])) return fail(REWL_ERR_NONFINITE, "rewl_mlp_set_params: non-finite parameter");
+    HIP_TRY(hipMemcpyAsync(m->theta, theta, NP * sizeof(float), hipMemcpyHostToDevice, m->stream));
+    HIP_TRY(hipMemsetAsync(m->adam_m, 0, NP * sizeof(float), m->stream));
+    HIP_TRY(hipMemsetAsync(m->adam_v, 0, NP * sizeof(float), m->stream));
+    hipLaunchKernelGGL(k_mlp_quantise, dim3((NP + 255) / 256), dim3(256), 0, m->stream, m->theta, m->wq);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(m->stream));
+    m->adam_t = 0;
+    return REWL_OK;
+}
+
+int rewl_mlp_forward(rewl_mlp* m, const float* x, int64_t n, float* y) {
+    using namespace rewl::mlp;
+    int rc = mlp_check(m);
+    if (rc) return rc;
+    if (!x || !y || n <= 0) return fail(REWL_ERR_INVALID, "rewl_mlp_forward: empty input");
+    rc = mlp_upload(m, x, nullptr, nullptr, n, nullptr);
+    if (rc) return rc;
+    Args a = mlp_args(m, n, false);
+    a.y = m->dy;
+    HIP_TRY(hipEventRecord(m->ev0, m->stream));
+    hipLaunchKernelGGL(k_mlp<false>, dim3(mlp_grid(n)), dim3(256), 0, m->stream, a);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(m->ev1, m->stream));
+    HIP_TRY(hipMemcpyAsync(y, m->dy, (size_t)n * sizeof(float), hipMemcpyDeviceToHost, m->stream));
+    HIP_TRY(hipStreamSynchronize(m->stream));
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, m->ev0, m->ev1));
+    m->last_ms = ms;
+    return REWL_OK;
+}
+
+int rewl_mlp_gradient(rewl_mlp* m, const float* x, const float* t, const float* w, int64_t n, float* grad,
+                      double* loss) {
+    using namespace rewl::mlp;
+    int rc = mlp_check(m);
+    if (rc) return rc;
+    if (!x || !t || !grad || n <= 0) return fail(REWL_ERR_INVALID, "rewl_mlp_gradient: empty input");
+    double wsum = 0.0;
+    rc = mlp_upload(m, x, t, w, n, &wsum);
+    if (rc) return rc;
+    const int G = mlp_grid(n);
+    rc = mlp_ensure_part(m, G);
+    if (rc) return rc;
+    Args a = mlp_args(m, n, w != nullptr);
+    a.inv_wsum = (float)(1.0 / wsum);
+    a.part = m->part;
+    HIP_TRY(hipEventRecord(m->ev0, m->stream));
+    hipLaunchKernelGGL(k_mlp<true>, dim3(G), dim3(256), 0, m->stream, a);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(k_mlp_reduce, dim3((NPART + 255) / 256), dim3(256), 0, m->stream, m->part, G, m->grad);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(m->ev1, m->stream));
+    std::vector<float> g(NPART);
+    HIP_TRY(hipMemcpyAsync(g.data(), m->grad, NPART * sizeof(float), hipMemcpyDeviceToHost, m->stream));
+    HIP_TRY(hipStreamSynchronize(m->stream));
+    std::copy(g.begin(), g.begin() + NP, grad);
+    if (loss) *loss = (double)g[NP] / wsum;
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, m->ev0, m->ev1));
+    m->last_ms = ms;
+    return REWL_OK;
+}
+
+int rewl_mlp_train(rewl_mlp* m, const float* x, const float* t, const float* w, int64_t n,
+                   const rewl_mlp_train_params* p, float* loss_history) {
+    using namespace rewl::mlp;
+    int rc = mlp_check(m);
+    if (rc) return rc;
+    if (!x || !t || !p || n <= 0) return fail(REWL_ERR_INVALID, "rewl_mlp_train: empty input");
+    if (n >= (int64_t)1 << 32) return fail(REWL_ERR_INVALID, "rewl_mlp_train: at most 2^32 - 1 points");
+    if (p->steps < 0 || p->batch <= 0 || !(p->lr > 0.0) || !(p->lr_final >= 0.0) || !(p->beta1 >= 0.0 && p->beta1 < 1.0) ||
+        !(p->beta2 >= 0.0 && p->beta2 < 1.0) || !(p->eps > 0.0) || p->warmup < 0)
+        return fail(REWL_ERR_INVALID, "rewl_mlp_train: bad hyper-parameters");
+    double wsum = 0.0;
+    rc = mlp_upload(m, x, t, w, n, &wsum);
+    if (rc) return rc;
+    const int G = mlp_grid(p->batch);
+    rc = mlp_ensure_part(m, G);
+    if (rc) return rc;
+    Args a = mlp_args(m, n, w != nullptr);
+    a.npts = p->batch;
+    a.sample = 1;
+    a.k0 = (uint32_t)p->seed;
+    a.k1 = (uint32_t)(p->seed >> 32);
+    // minibatch loss: sum_k w_k (y_k - t_k)^2 / (batch * mean weight)
+    a.inv_wsum = (float)(1.0 / ((double)p->batch * (wsum / (double)n)));
+    a.part = m->part;
+    float* dloss = nullptr;
+    if (loss_history && p->steps > 0) HIP_TRY(hipMalloc((void**)&dloss, (size_t)p->steps * sizeof(float)));
+    HIP_TRY(hipEventRecord(m->ev0, m->stream));
+    for (int64_t s = 0; s < p->steps; ++s) {
+        a.step = (uint64_t)s;
+        hipLaunchKernelGGL(k_mlp<true>, dim3(G), dim3(256), 0, m->stream, a);
+        hipLaunchKernelGGL(k_mlp_reduce, dim3((NPART + 255) / 256), dim3(256), 0, m->stream, m->part, G, m->grad);
+        if (dloss)
+            HIP_TRY(hipMemcpyAsync(dloss + s, m->grad + NP, sizeof(float), hipMemcpyDeviceToDevice, m->stream));
+        m->adam_t += 1;
+        double lr = p->lr_final + 0.5 * (p->lr - p->lr_final) * (1.0 + std::cos(M_PI * (double)s / (double)std::max<int64_t>(1, p->steps)));
+        if (s < p->warmup) lr *= (double)(s + 1) / (double)p->warmup;
+        const double bc1 = 1.0 - std::pow(p->beta1, (double)m->adam_t);
+        const double bc2 = 1.0 - std::pow(p->beta2, (double)m->adam_t);
+        hipLaunchKernelGGL(k_mlp_adam, dim3((NP + 255) / 256), dim3(256), 0, m->stream, m->theta, m->adam_m,
+                           m->adam_v, m->grad, (float)lr, (float)p->beta1, (float)p->beta2, (float)p->eps, (float)bc1,
+                           (float)bc2, m->wq);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) {
+            if (dloss) (void)hipFree(dloss);
+            return fail(REWL_ERR_DEVICE, std::string("rewl_mlp_train launch: ") + hipGetErrorString(e));
+        }
+    }
+    HIP_TRY(hipEventRecord(m->ev1, m->stream));
+    HIP_TRY(hipStreamSynchronize(m->stream));
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, m->ev0, m->ev1));
+    m->last_ms = ms;
+    if (dloss) {
+        std::vector<float> h((size_t)p->steps);
+        hipError_t e = hipMemcpy(h.data(), dloss, h.size() * sizeof(float), hipMemcpyDeviceToHost);
+        (void)hipFree(dloss);
+        if (e != hipSuccess) return fail(REWL_ERR_DEVICE, std::string("rewl_mlp_train: ") + hipGetErrorString(e));
+        for (int64_t s = 0; s < p->steps; ++s) loss_history[s] = h[(size_t)s] * a.inv_wsum;
+    }
+    return REWL_OK;
+}
+
+int rewl_mlp_last_ms(const rewl_mlp* m, double* ms) {
+    if (!m || !ms) return fail(REWL_ERR_INVALID, "rewl_mlp_last_ms: null argument");
+    *ms = m->last_ms;
     return REWL_OK;
 }
 

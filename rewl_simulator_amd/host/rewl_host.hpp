@@ -113,6 +113,7 @@ struct REWL_Parameters {
     bool replay = false;                         // per-walker mt19937_64 streams
     uint64_t seed = 1;
     int adjust_mode = 0;                         // 1: alternate AT planes (full spectrum)
+    int average_mode = 0;                        // 1: equal-weight replica average
 };
 
 template <class Hamiltonian_t>
@@ -150,6 +151,7 @@ class REWL_simulation {
         p.exchange = 1;
         p.producers = 0;
         p.adjust_mode = rp.adjust_mode;
+        p.average_mode = rp.average_mode;
         check(rewl_create(&d, 1, &p, &ctx_));
         check(rewl_adjust_to_range(ctx_));
     }

@@ -135,13 +135,15 @@ def test_rewl_cycles_bit_exact(gpu, rng, replicas):
     assert eng.stats()["exchanges_accepted"] == ora.stats()["exchanges_accepted"]
 
 
-@pytest.mark.parametrize("rng", [abi.RNG_REPLAY, abi.RNG_PHILOX])
-def test_halving_run_and_finalize_bit_exact(gpu, rng):
-    """Whole run to completion with min >= 0.8 mean flatness and halving."""
-    print(f"case: full halving run rng={rng}", flush=True)
+@pytest.mark.parametrize("rng,average", [(abi.RNG_REPLAY, 0), (abi.RNG_PHILOX, 0), (abi.RNG_PHILOX, 1)])
+def test_halving_run_and_finalize_bit_exact(gpu, rng, average):
+    """Whole run to completion with min >= 0.8 mean flatness and halving;
+    count-weighted (reference) and equal-weight replica averaging."""
+    print(f"case: full halving run rng={rng} average={average}", flush=True)
     s = configs.ising_full_range(4, 4.0, 2, 3)
     p = configs.params(rng_mode=rng, flatness_mode=abi.FLAT_MIN_OVER_MEAN, flatness=0.8,
-                       schedule=abi.SCHED_HALVE, lnf_final=1e-3, sweeps_per_check=100, sweeps_per_exchange=10)
+                       schedule=abi.SCHED_HALVE, lnf_final=1e-3, sweeps_per_check=100, sweeps_per_exchange=10,
+                       average_mode=average)
     eng, ora = make_pair([s], p)
     eng.adjust_to_range(); ora.adjust_to_range()
     done_e = eng.run(400)

@@ -244,3 +244,42 @@ def test_config4_ranges_and_visited_flatness():
     o.adjust_to_range()
     assert o.run(2000)
     assert o.stats()["flat_events"] > 0
+
+
+@pytest.mark.parametrize("average", [0, 1])
+def test_window_average_weights(average):
+    """average_and_redistribute_window: after a flat check every replica of a
+    window holds sum_r w_r ln g_r (then REDUCE_LOGDOS), with w_r = c_r / sum c
+    (the reference, rewl_simulation.hpp:393-397) or 1/R (its alternative,
+    line 401).  Computed here from the pre-check ln g and counts in numpy."""
+    s = configs.ising_full_range(6, 4.0, 2, 3)
+    p = configs.params(flatness_mode=abi.FLAT_ALL_VISITED, schedule=abi.SCHED_HALVE, lnf_initial=0.5,
+                       average_mode=average, reduce_logdos=0, exchange=0)
+    o = O.Oracle([s], p)
+    o.adjust_to_range()
+    o.sweeps(400)
+    R = 3
+    for k in range(2):
+        lng = np.array([o.logdos(k * R + r) for r in range(R)])
+        cnt = np.array([o.counts(k * R + r) for r in range(R)], dtype=np.float64)
+        if average == 0:
+            tot = cnt.sum(0)
+            want = np.zeros(lng.shape[1])
+            for r in range(R):
+                want = want + np.where(tot > 0, cnt[r] / np.where(tot > 0, tot, 1) * lng[r], 0)
+            keep = tot == 0
+        else:
+            want = sum(lng[r] / R for r in range(R))
+            keep = np.zeros(lng.shape[1], bool)
+        lng_k = lng, want, keep
+        if k == 0:
+            first = lng_k
+        else:
+            second = lng_k
+    o.check()
+    for k, (lng, want, keep) in enumerate((first, second)):
+        assert o.window_state(0, k)["iteration"] == 2, "the window was not flat"
+        for r in range(R):
+            got = o.logdos(k * R + r)
+            np.testing.assert_allclose(got[~keep], want[~keep], rtol=1e-14, atol=1e-14)
+            np.testing.assert_array_equal(got[keep], lng[r][keep])
