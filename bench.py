@@ -35,6 +35,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--walkers-per-group", type=int, default=0)
     ap.add_argument("--producers", type=int, default=0, help="producer wavefronts (0 auto, -1 none)")
+    ap.add_argument("--sweep-kernel", type=int, default=0, help="0 auto, 1 row kernel, 2 row-unrolled kernel")
     ap.add_argument("--converge", type=float, default=0.0, metavar="LNF",
                     help="instead of the throughput steps, time the run from the first sweep until every "
                          "window's ln f is below LNF")
@@ -123,7 +124,7 @@ def main():
 
     systems, params = configs.config2()
     params = dict(params, sweeps_per_check=a.sweeps_per_step, sweeps_per_exchange=a.sweeps_per_step,
-                  walkers_per_group=a.walkers_per_group, producers=a.producers)
+                  walkers_per_group=a.walkers_per_group, producers=a.producers, sweep_kernel=a.sweep_kernel)
     eng = Engine(systems, params)
     eng.adjust_to_range()
     W = eng.num_walkers
@@ -180,6 +181,7 @@ def main():
             "attempts_per_step": attempts // a.steps,
             "parallelism": "one lane per walker, 1 GPU",
             "producers": a.producers,
+            "sweep_kernel": a.sweep_kernel,
         },
         "roofline": {
             "bound": "hbm",

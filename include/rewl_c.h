@@ -132,7 +132,11 @@ typedef struct rewl_params {
                                     1 = equal weights 1/R, the alternative left in
                                     rewl_simulation.hpp:401 (unbiased when replicas
                                     also exchange with other windows)               */
-    int32_t reserved[4];
+    int32_t sweep_kernel;        /* Philox integer path: 0 = auto (row-unrolled kernel
+                                    for L in {16,24,32,48,64}, else the row kernel);
+                                    1 = row kernel; 2 = row-unrolled (error if L is
+                                    not one of those sizes or producers < 0)       */
+    int32_t reserved[3];
 } rewl_params;
 
 /* Per-window bookkeeping, one record per (system, window). */
@@ -178,6 +182,8 @@ int rewl_create(const rewl_system_desc* systems, int32_t num_systems,
 int rewl_destroy(rewl_ctx* ctx);
 
 /* ---- simulation steps ----------------------------------------------------- */
+/* rewl_sweeps (and rewl_run) return REWL_ERR_RANGE until rewl_adjust_to_range
+ * has put every walker inside its window, and again after rewl_set_spins. */
 int rewl_adjust_to_range(rewl_ctx* ctx);
 int rewl_sweeps(rewl_ctx* ctx, int32_t num_sweeps);
 int rewl_exchange(rewl_ctx* ctx);

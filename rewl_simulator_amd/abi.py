@@ -69,7 +69,8 @@ class Params(C.Structure):
         ("producers", C.c_int32),
         ("adjust_mode", C.c_int32),
         ("average_mode", C.c_int32),
-        ("reserved", C.c_int32 * 4),
+        ("sweep_kernel", C.c_int32),
+        ("reserved", C.c_int32 * 3),
     ]
 
 

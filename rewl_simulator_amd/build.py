@@ -24,6 +24,9 @@ HIP_FLAGS = [
     "-shared",
     "-ffp-contract=off",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # the row-unrolled sweep kernel unrolls L attempts (with their rare exact
+    # path) per row; the default cap on pragma-requested unrolling is too low
+    "-mllvm", "-pragma-unroll-threshold=200000",
     "-Wall",
     "-Wno-unused-result",
 ]

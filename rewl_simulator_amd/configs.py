@@ -28,6 +28,7 @@ REFERENCE_PARAMS = dict(
     producers=0,
     adjust_mode=0,
     average_mode=0,
+    sweep_kernel=0,
 )
 
 DEFAULT_SEED = 20240501

@@ -22,6 +22,7 @@
 #include "aux_kernels.hpp"
 #include "sweep.hpp"
 #include "sweep_fast.hpp"
+#include "sweep_tuned.hpp"
 
 using namespace rewl;
 
@@ -102,6 +103,13 @@ std::vector<HostWindow> make_windows(const rewl_system_desc& d) {
 
 bool integral(double v) { return std::isfinite(v) && std::floor(v) == v && std::fabs(v) < 1e9; }
 
+constexpr int kTunedL[5] = {16, 24, 32, 48, 64};
+int tuned_index(int L) {
+    for (int i = 0; i < 5; ++i)
+        if (kTunedL[i] == L) return i;
+    return -1;
+}
+
 }  // namespace
 
 struct rewl_ctx {
@@ -122,6 +130,14 @@ struct rewl_ctx {
     int class_maxL[4] = {0, 0, 0, 0};
     int nprod = 1;                                      // producer wavefronts of the fast path
     DevBuf<int32_t> d_class_groups[4];
+    // row-unrolled kernel: group lists per (model, lattice size kTunedL[i])
+    std::vector<int32_t> tuned_groups[2][5];
+    int tuned_maxbins[2][5] = {};
+    int tuned_maxlanes[2][5] = {};
+    bool tuned_simple[2][5] = {{true, true, true, true, true}, {true, true, true, true, true}};
+    int tuned_guard[2][5] = {};                          // guard bins each side: max |Delta E| / bin size
+    int tuned_nprod = 2;
+    DevBuf<int32_t> d_tuned_groups[2][5];
     std::vector<int32_t> pairs_even, pairs_odd;
     DevBuf<int32_t> d_pairs_even, d_pairs_odd, d_flat;
 
@@ -149,6 +165,7 @@ struct rewl_ctx {
     uint64_t sweep_index = 0;
     int64_t pending_sweeps = 0;
     bool direction_even = true;
+    bool in_windows = false;   // every walker inside its window (rewl_adjust_to_range ran)
     rewl_stats stats{};
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_sweep;
     std::vector<int32_t> done_host;
@@ -165,6 +182,8 @@ void destroy_ctx(rewl_ctx* c) {
                     &c->d_pairs_even, &c->d_pairs_odd, &c->d_flat})
         b->release();
     for (auto& b : c->d_class_groups) b.release();
+    for (auto& row : c->d_tuned_groups)
+        for (auto& b : row) b.release();
     c->d_w_hist.release(); c->d_w_lat.release(); c->d_win_iter.release(); c->d_win_sweeps.release();
     c->d_w_E.release(); c->d_win_lnf.release(); c->d_fields.release();
     c->d_w_cursor.release(); c->d_lat.release(); c->d_visits.release(); c->d_ring.release(); c->d_ring_filled.release();
@@ -259,6 +278,40 @@ int launch_sweep_fast(rewl_ctx* c, int cls, int nsweeps) {
     return REWL_OK;
 }
 
+// Philox mode, integer path, L in kTunedL: the row-unrolled kernel (sweep_tuned.hpp).
+template <int MODEL, int L>
+int launch_sweep_tuned(rewl_ctx* c, int li, int nsweeps) {
+    const auto& list = c->tuned_groups[MODEL][li];
+    const int ng = (int)list.size();
+    if (ng == 0) return REWL_OK;
+    const size_t lanes = (size_t)c->tuned_maxlanes[MODEL][li];
+    const int guard = c->tuned_guard[MODEL][li];
+    const size_t lds = ((size_t)c->tuned_maxbins[MODEL][li] + 2 * guard) * lanes * kTunedSlot +
+                       (size_t)tuned_ring_stride(L) * lanes;
+    const bool simple = c->tuned_simple[MODEL][li];
+    const int np = c->tuned_nprod;
+    void* kern = simple ? (void*)k_sweep_tuned<MODEL, L, true> : (void*)k_sweep_tuned<MODEL, L, false>;
+    HIP_TRY(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    KArgs args = c->args;
+    const int32_t* gl = c->d_tuned_groups[MODEL][li].p;
+    uint64_t base = c->sweep_index;
+    int guard_arg = guard;
+    void* kargs[] = {&args, (void*)&gl, (void*)&ng, (void*)&nsweeps, (void*)&base, (void*)&guard_arg};
+    HIP_TRY(hipLaunchKernel((const void*)kern, dim3((unsigned)ng), dim3(kLanes * (1 + np)), kargs, lds, c->stream));
+    HIP_TRY(hipGetLastError());
+    return REWL_OK;
+}
+
+template <int MODEL>
+int launch_tuned_all(rewl_ctx* c, int nsweeps) {
+    int rc;
+    if ((rc = launch_sweep_tuned<MODEL, 16>(c, 0, nsweeps))) return rc;
+    if ((rc = launch_sweep_tuned<MODEL, 24>(c, 1, nsweeps))) return rc;
+    if ((rc = launch_sweep_tuned<MODEL, 32>(c, 2, nsweeps))) return rc;
+    if ((rc = launch_sweep_tuned<MODEL, 48>(c, 3, nsweeps))) return rc;
+    return launch_sweep_tuned<MODEL, 64>(c, 4, nsweeps);
+}
+
 template <int MODEL, bool INTP>
 int launch_sweep_class(rewl_ctx* c, int cls, int nsweeps) {
     const int ng = (int)c->class_groups[cls].size();
@@ -291,6 +344,8 @@ int launch_sweeps(rewl_ctx* c, int nsweeps) {
     if ((rc = launch_sweep_class<0, true>(c, 1, nsweeps))) return rc;
     if ((rc = launch_sweep_class<1, false>(c, 2, nsweeps))) return rc;
     if ((rc = launch_sweep_class<1, true>(c, 3, nsweeps))) return rc;
+    if ((rc = launch_tuned_all<0>(c, nsweeps))) return rc;
+    if ((rc = launch_tuned_all<1>(c, nsweeps))) return rc;
     HIP_TRY(hipEventRecord(e1, c->stream));
     c->ev_sweep.emplace_back(e0, e1);
     if (c->ev_sweep.size() > 256) return resolve_sweep_events(c);
@@ -327,7 +382,7 @@ int rewl_create(const rewl_system_desc* systems, int32_t num_systems, const rewl
     const rewl_params& P = *params;
     if (P.sweeps_per_check <= 0 || P.sweeps_per_exchange <= 0 || P.lnf_initial <= 0 ||
         (P.flatness_mode < 0 || P.flatness_mode > 3) || (P.schedule < 0 || P.schedule > 1) ||
-        (P.rng_mode < 0 || P.rng_mode > 1) || (P.adjust_mode < 0 || P.adjust_mode > 1) || (P.average_mode < 0 || P.average_mode > 1) || P.walkers_per_group < 0 || P.walkers_per_group > kLanes)
+        (P.rng_mode < 0 || P.rng_mode > 1) || (P.adjust_mode < 0 || P.adjust_mode > 1) || (P.average_mode < 0 || P.average_mode > 1) || (P.sweep_kernel < 0 || P.sweep_kernel > 2) || P.walkers_per_group < 0 || P.walkers_per_group > kLanes)
         return fail(REWL_ERR_INVALID, "invalid rewl_params");
 
     rewl_ctx* c = new rewl_ctx();
@@ -414,8 +469,23 @@ int rewl_create(const rewl_system_desc* systems, int32_t num_systems, const rewl
         // a window of many bins gets fewer walkers per wavefront.
         const bool fast = S.intpath && P.rng_mode == REWL_RNG_PHILOX;
         c->nprod = P.producers < 0 ? 0 : (P.producers == 0 ? 1 : std::min<int>(P.producers, 3));
+        c->tuned_nprod = P.producers == 0 ? 3 : std::min<int>(std::max<int>(P.producers, 1), 3);
+        const int li = tuned_index(S.d.L);
+        const bool tuned = fast && P.producers >= 0 && li >= 0 && P.sweep_kernel != 1;
+        if (P.sweep_kernel == 2 && !tuned)
+            return bail(fail(REWL_ERR_INVALID, "the row-unrolled kernel needs the Philox integer path, producers >= 0 "
+                                               "and L in {16, 24, 32, 48, 64}"));
         int wpg = kLanes;
-        if (fast)
+        // guard bins: |Delta E| <= 4 (2J) (+ 4 (2K)) on the integer path
+        int guard = 0;
+        if (tuned) {
+            const long long maxde = 4LL * std::abs(S.twoJ) + (S.d.model == REWL_ASHKIN_TELLER2D ? 4LL * std::abs(S.twoK) : 0LL);
+            const long long bs = (long long)c->win[S.first_window].bs;
+            guard = (int)((maxde + bs - 1) / bs) + 1;
+            wpg = std::min(kLanes, (int)((size_t)c->lds_max /
+                                         ((size_t)(maxbins + 2 * guard) * kTunedSlot + (size_t)tuned_ring_stride(S.d.L))));
+        }
+        else if (fast)
             wpg = std::min(kLanes, (int)((size_t)c->lds_max /
                                          ((size_t)(maxbins + 1) * 12 + (c->nprod ? 8 * (size_t)S.d.L : 0))));
         if (P.walkers_per_group > 0) wpg = std::min(wpg, (int)P.walkers_per_group);
@@ -431,10 +501,16 @@ int rewl_create(const rewl_system_desc* systems, int32_t num_systems, const rewl
             for (int k = 0; k < S.d.num_windows; ++k)
                 zero_bound = zero_bound || c->win[S.first_window + k].emin == 0.0 || c->win[S.first_window + k].emax == 0.0;
             const bool m1_reachable = g == 0 ? (e0 == -1) : (((-1 - e0) % g) == 0);
-            const int cls = S.d.model * 2 + (S.intpath ? 1 : 0);
-            if (zero_bound && m1_reachable) c->class_simple[cls] = false;
-            c->class_maxlanes[cls] = std::max(c->class_maxlanes[cls], wpg);
-            c->class_maxL[cls] = std::max(c->class_maxL[cls], S.d.L);
+            if (tuned) {
+                if (zero_bound && m1_reachable) c->tuned_simple[S.d.model][li] = false;
+                c->tuned_maxlanes[S.d.model][li] = std::max(c->tuned_maxlanes[S.d.model][li], wpg);
+                c->tuned_guard[S.d.model][li] = std::max(c->tuned_guard[S.d.model][li], guard);
+            } else {
+                const int cls = S.d.model * 2 + (S.intpath ? 1 : 0);
+                if (zero_bound && m1_reachable) c->class_simple[cls] = false;
+                c->class_maxlanes[cls] = std::max(c->class_maxlanes[cls], wpg);
+                c->class_maxL[cls] = std::max(c->class_maxL[cls], S.d.L);
+            }
         }
         for (int off = 0; off < nw; off += wpg) {
             DevGroup G{};
@@ -456,9 +532,14 @@ int rewl_create(const rewl_system_desc* systems, int32_t num_systems, const rewl
                 c->w_hist[g] = G.hist_base + j;
                 c->w_lat[g] = G.lat_base + j;
             }
-            const int cls = S.d.model * 2 + (S.intpath ? 1 : 0);
-            c->class_groups[cls].push_back(gid);
-            c->class_maxbins[cls] = std::max(c->class_maxbins[cls], nb);
+            if (tuned) {
+                c->tuned_groups[S.d.model][li].push_back(gid);
+                c->tuned_maxbins[S.d.model][li] = std::max(c->tuned_maxbins[S.d.model][li], nb);
+            } else {
+                const int cls = S.d.model * 2 + (S.intpath ? 1 : 0);
+                c->class_groups[cls].push_back(gid);
+                c->class_maxbins[cls] = std::max(c->class_maxbins[cls], nb);
+            }
             c->groups.push_back(G);
         }
     }
@@ -535,6 +616,14 @@ int rewl_create(const rewl_system_desc* systems, int32_t num_systems, const rewl
     if ((rc = upload(c->d_w_lat, c->w_lat, c->stream))) return bail(rc);
     for (int cls = 0; cls < 4; ++cls)
         if ((rc = upload(c->d_class_groups[cls], c->class_groups[cls], c->stream))) return bail(rc);
+    for (int m = 0; m < 2; ++m)
+        for (int i = 0; i < 5; ++i) {
+            if (c->tuned_groups[m][i].empty()) continue;
+            const size_t need = ((size_t)c->tuned_maxbins[m][i] + 2 * c->tuned_guard[m][i]) * c->tuned_maxlanes[m][i] *
+                                    kTunedSlot + (size_t)tuned_ring_stride(kTunedL[i]) * c->tuned_maxlanes[m][i];
+            if (need > (size_t)c->lds_max) return bail(fail(REWL_ERR_INVALID, "group too wide for LDS"));
+            if ((rc = upload(c->d_tuned_groups[m][i], c->tuned_groups[m][i], c->stream))) return bail(rc);
+        }
     if ((rc = upload(c->d_pairs_even, c->pairs_even, c->stream))) return bail(rc);
     if ((rc = upload(c->d_pairs_odd, c->pairs_odd, c->stream))) return bail(rc);
 
@@ -626,11 +715,16 @@ int rewl_adjust_to_range(rewl_ctx* c) {
         hipLaunchKernelGGL(k_adjust<false>, dim3((c->num_walkers + 63) / 64), dim3(64), 0, c->stream, c->args, steps,
                            mode);
     HIP_TRY(hipGetLastError());
-    return check_err_flag(c);
+    rc = check_err_flag(c);
+    c->in_windows = rc == REWL_OK;
+    return rc;
 }
 
 int rewl_sweeps(rewl_ctx* c, int32_t n) {
     if (!c || n < 0) return fail(REWL_ERR_INVALID, "bad arguments");
+    // the sweep kernels index a walker's histogram by its current energy
+    if (!c->in_windows)
+        return fail(REWL_ERR_RANGE, "walkers are not inside their windows: call rewl_adjust_to_range first");
     int rc;
     int done = 0;
     const int chunk = c->p.rng_mode == REWL_RNG_REPLAY
@@ -881,6 +975,7 @@ int rewl_set_spins(rewl_ctx* c, int32_t g, const int8_t* spins, int32_t n) {
     hipLaunchKernelGGL(k_recalc, dim3((c->num_walkers + 63) / 64), dim3(64), 0, c->stream, c->args);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(c->stream));
+    c->in_windows = false;   // the new state may lie outside the window
     return REWL_OK;
 }
 

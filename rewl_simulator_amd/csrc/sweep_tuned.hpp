@@ -1,0 +1,368 @@
+// Wang-Landau sweep, Philox mode, integer energies, lattice side L known at
+// compile time: the row-unrolled throughput kernel.
+//
+// Same chain, same draws and the same FP64 ln g arithmetic as k_sweep and
+// k_sweep_fast (the reference's wang_landau_update, wang_landau.hpp:49-93).
+// With one walker per lane and only a few thousand walkers, a lane's time per
+// attempt is set by the length of its instruction stream (a wavefront issues
+// at most one instruction per SIMD slot), so this kernel is written to
+// minimise instructions per attempt and has no branch inside a row:
+//
+//  * the site loop of a row is unrolled (L = 16, 24, 32, 48, 64), so every
+//    bit position is a constant and neighbour bits come out of 32-bit row
+//    halves with one bit-field extract each;
+//  * per row, the static disagreement count cs (up, down, right neighbours)
+//    of every site is packed as the signed 2-bit field cs - 2, and the left
+//    neighbour as the static pdl = P ^ rotl(P) corrected by the previous
+//    site's acceptance a, so that Delta E = -4J (cs - 2 + (pdl ^ a)) is two
+//    extracts, an add and one 24-bit multiply-add;
+//  * the window offset o = E - window minimum is the walker's state, so the
+//    proposal's LDS slot comes straight from o + Delta E;
+//  * guard bins around the window hold ln g = +inf, so a proposal outside the
+//    window is rejected by the acceptance test itself, with no range test;
+//  * producer wavefronts of the same workgroup turn every Philox word into
+//    the bound B of accept_bound.hpp, so acceptance is the single FP64
+//    comparison ln g[new] - ln g[cur] < B;
+//  * ln g (FP64) and the u32 count of a (bin, walker) share one 16-byte LDS
+//    slot, so the count increment is a ds_add at offset 8 of the ln g slot.
+#pragma once
+#include "accept_bound.hpp"
+#include "device.hpp"
+#include "sweep_fast.hpp"
+
+namespace rewl {
+
+constexpr int kTunedSlot = 16;   // bytes per (bin, walker): f64 ln g, u32 count, pad
+
+// bytes of one walker's bound ring: two buffers of L FP64 values + 16 pad
+constexpr __host__ __device__ int tuned_ring_stride(int L) { return 16 * L + 16; }
+
+__device__ __forceinline__ uint32_t bit_of(uint32_t w, int b) { return __builtin_amdgcn_ubfe(w, b, 1); }
+
+// bit i of a 16-bit value to bit 2 i
+__device__ __forceinline__ uint32_t spread16(uint32_t v) {
+    v = (v | (v << 8)) & 0x00ff00ffu;
+    v = (v | (v << 4)) & 0x0f0f0f0fu;
+    v = (v | (v << 2)) & 0x33333333u;
+    v = (v | (v << 1)) & 0x55555555u;
+    return v;
+}
+
+template <int L>
+__device__ __forceinline__ uint64_t rotr1(uint64_t r) {   // bit x <- bit (x + 1) mod L
+    return (r >> 1) | ((r & 1ull) << (L - 1));
+}
+template <int L>
+__device__ __forceinline__ uint64_t rotl1(uint64_t r) {   // bit x <- bit (x - 1) mod L
+    constexpr uint64_t m = L == 64 ? ~0ull : ((1ull << L) - 1ull);
+    return ((r << 1) | (r >> (L - 1))) & m;
+}
+
+// Dynamic LDS of the tuned kernel.  Slots are addressed by their 32-bit LDS
+// address (lds_base() + byte offset), so that an access is one ds_* with no
+// address arithmetic left at the access.
+extern __shared__ __align__(16) unsigned char tuned_lds[];
+
+typedef __attribute__((address_space(3))) double lds_double;
+typedef __attribute__((address_space(3))) uint32_t lds_u32;
+typedef __attribute__((address_space(3))) unsigned char lds_uchar;
+typedef double f64x2 __attribute__((ext_vector_type(2)));
+typedef __attribute__((address_space(3))) f64x2 lds_f64x2_t;
+
+__device__ __forceinline__ uint32_t lds_base() {
+    return (uint32_t)(uintptr_t)(lds_uchar*)(tuned_lds);
+}
+__device__ __forceinline__ double lds_f64(uint32_t addr) { return *(lds_double*)(uintptr_t)addr; }
+__device__ __forceinline__ void lds_store_f64(uint32_t addr, double v) { *(lds_double*)(uintptr_t)addr = v; }
+__device__ __forceinline__ f64x2 lds_f64x2(uint32_t addr) { return *(lds_f64x2_t*)(uintptr_t)addr; }
+__device__ __forceinline__ void lds_store_f64x2(uint32_t addr, f64x2 v) { *(lds_f64x2_t*)(uintptr_t)addr = v; }
+// ln g += ln f as an LDS FP64 atomic (no return): the same IEEE add as a
+// load, add and store, in one instruction
+__device__ __forceinline__ void lds_add_f64(uint32_t addr, double v) {
+    __hip_atomic_fetch_add((lds_double*)(uintptr_t)addr, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ uint32_t lds_u32_at(uint32_t addr) { return *(lds_u32*)(uintptr_t)addr; }
+__device__ __forceinline__ void lds_store_u32(uint32_t addr, uint32_t v) { *(lds_u32*)(uintptr_t)addr = v; }
+__device__ __forceinline__ void lds_add_u32(uint32_t addr, uint32_t v) {
+    __hip_atomic_fetch_add((lds_u32*)(uintptr_t)addr, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+struct TunedWindow {
+    int32_t wmin;       // window minimum (integer path)
+    int32_t span;       // SIMPLE: emax - emin (= bins x bin size); else phi_hi - phi_lo
+    int32_t phi_lo;
+    int32_t sh;         // log2(bin size)
+};
+
+// One attempt at a site whose Delta E has been reduced to the proposed window
+// offset `on`.  SIMPLE windows: bins [-guard, nbins + guard) exist in LDS, the
+// guard bins holding ln g = +inf, so a proposal outside the window needs no
+// range test (|Delta E| <= guard x bin size).  Otherwise the reference's
+// phi(e) = (int)(e + 0.5) range test picks the +inf sentinel slot.
+template <bool SIMPLE>
+__device__ __forceinline__ void tuned_attempt(int on, const TunedWindow& tw, int& o, uint32_t& cbo, double lnf,
+                                              int stride16, uint32_t zero_off, uint32_t sent, double B, int& a) {
+    uint32_t nbo;
+    if (SIMPLE) {
+        // |Delta E| <= guard bins: in range because the walker is inside its
+        // window (the host refuses to sweep before rewl_adjust_to_range)
+        nbo = (uint32_t)__mul24(on >> tw.sh, stride16) + zero_off;
+    } else {
+        const int En = on + tw.wmin;
+        const bool in = (unsigned)(phi_int(En) - tw.phi_lo) < (unsigned)tw.span;
+        nbo = in ? (uint32_t)__mul24(max(on, 0) >> tw.sh, stride16) + zero_off : sent;
+    }
+    // the current bin is re-read rather than carried: its ln g changes only
+    // through the lds_add_f64 below, which LDS orders before this read
+    const double ln = lds_f64(nbo);
+    const double lc = lds_f64(cbo);
+    const bool acc = (ln - lc) < B;
+    o = acc ? on : o;
+    cbo = acc ? nbo : cbo;
+    lds_add_f64(cbo, lnf);
+    lds_add_u32(cbo + 8, 1u);
+    a = acc ? 1 : 0;
+}
+
+template <int MODEL, int L, bool SIMPLE>
+__device__ __forceinline__ void sweep_walker_tuned(const KArgs& a, const DevGroup& G, const DevSystem& S, int lane,
+                                                   int nsweeps, int guard, uint32_t ring_lane) {
+    constexpr int NW = (L + 31) / 32;    // 32-bit words per row
+    constexpr int NW2 = (L + 15) / 16;   // words of 2-bit fields per row
+    constexpr int NP = MODEL == 0 ? 1 : 2;
+    constexpr int NPASS = MODEL == 0 ? 1 : 2;
+    constexpr int N = L * L;
+    const bool active = lane < G.nwalkers;
+    const int g = G.first_walker + (active ? lane : 0);
+    const int wi = a.w_window[g];
+    const DevWindow Wn = a.win[wi];
+    const double lnf = a.win_lnf[wi];
+    const int64_t hb = G.hist_base + lane;
+    const int gbins = G.nbins;     // bins of the widest window of the group (LDS rows)
+    const int nbins = Wn.nbins;    // bins of this walker's window
+    const int stride = G.nwalkers;
+    const int stride16 = stride * kTunedSlot;
+    // slot of bin b: (b + guard) * stride16 + lane * 16, b in [-guard, nbins + guard)
+    const uint32_t zero_off = lds_base() + (uint32_t)(guard * stride16 + lane * kTunedSlot);
+    const uint32_t sent = zero_off + (uint32_t)(nbins * stride16);   // first upper guard bin
+
+    if (active) {
+        for (int b = -guard; b < gbins + guard; ++b) {
+            const uint32_t off = zero_off + (uint32_t)(b * stride16);
+            const bool inside = b >= 0 && b < nbins;   // a narrower window's spare bins are guards too
+            lds_store_f64(off, inside ? a.lng[hb + (int64_t)b * kLanes] : __builtin_inf());
+            lds_store_u32(off + 8, inside ? a.cnt[hb + (int64_t)b * kLanes] : 0u);
+        }
+    }
+    __syncthreads();   // histograms loaded, first row of thresholds ready
+
+    TunedWindow tw;
+    tw.wmin = Wn.imin;
+    tw.sh = Wn.shift;
+    tw.phi_lo = Wn.phi_lo;
+    tw.span = SIMPLE ? (int)(Wn.emax - Wn.emin) : Wn.phi_hi - Wn.phi_lo;
+
+
+    uint64_t* lat = a.lat + G.lat_base + lane;
+    int fl = a.w_flavor[g];
+    int o = (int)a.w_E[g] - tw.wmin;
+    // the walker is inside its window (rewl_adjust_to_range); the clamp keeps
+    // the slot in bounds regardless
+    uint32_t cbo = (uint32_t)(min(max(o, 0) >> tw.sh, max(nbins - 1, 0)) * stride16) + zero_off;
+
+    const int m4J = -2 * S.twoJ, m4K = -2 * S.twoK;
+    int buf = 0;
+
+    for (int s = 0; s < nsweeps; ++s) {
+        for (int pass = 0; pass < NPASS; ++pass) {
+            uint64_t up[NP], cu[NP], dn[NP], r0[NP], nx[NP];
+#pragma unroll
+            for (int q = 0; q < NP; ++q) {
+                up[q] = active ? lat[(int64_t)(q * L + L - 1) * kLanes] : 0ull;
+                cu[q] = active ? lat[(int64_t)(q * L) * kLanes] : 0ull;
+                dn[q] = active ? lat[(int64_t)(q * L + 1) * kLanes] : 0ull;
+                r0[q] = 0ull;
+            }
+            for (int y = 0; y < L; ++y) {
+                if (active) {
+#pragma unroll
+                    for (int q = 0; q < NP; ++q) nx[q] = (y + 2 < L) ? lat[(int64_t)(q * L + y + 2) * kLanes] : 0ull;
+                    uint64_t P = cu[0], Pu = up[0], Pd = dn[0];
+                    uint64_t Q = 0, Qu = 0, Qd = 0;
+                    if (MODEL == 1) {
+                        const bool t = fl != 0;
+                        P = t ? cu[1] : cu[0]; Pu = t ? up[1] : up[0]; Pd = t ? dn[1] : dn[0];
+                        Q = t ? cu[0] : cu[1]; Qu = t ? up[0] : up[1]; Qd = t ? dn[0] : dn[1];
+                    }
+                    // ---- per-row static bit slices ---------------------------------
+                    const uint64_t pu = P ^ Pu, pd = P ^ Pd, pr = P ^ rotr1<L>(P);
+                    const uint64_t c0 = pu ^ pd ^ pr, c1n = ~((pu & pd) | (pr & (pu ^ pd)));
+                    const uint64_t pdl = P ^ rotl1<L>(P);
+                    uint64_t m0 = 0, m1n = 0, ql = 0, xr = 0;
+                    if (MODEL == 1) {
+                        const uint64_t qu = Q ^ Qu, qd = Q ^ Qd, qr = Q ^ rotr1<L>(Q);
+                        const uint64_t xu = pu ^ qu, xd = pd ^ qd;
+                        xr = pr ^ qr;
+                        m0 = xu ^ xd ^ xr;
+                        m1n = ~((xu & xd) | (xr & (xu ^ xd)));
+                        ql = Q ^ rotl1<L>(Q);
+                    }
+                    // signed 2-bit counts, site x at bits 2 (x % 16) of word x / 16:
+                    // (c0, c1n) interleaved is c0 + 2 c1 - 2 in two's complement
+                    uint32_t w_cp[NW2], w_cq[NW2], w_pdl[NW], w_ql[NW], frev[NW];
+#pragma unroll
+                    for (int w = 0; w < NW2; ++w) {
+                        w_cp[w] = spread16((uint32_t)(c0 >> (16 * w)) & 0xffffu) |
+                                  (spread16((uint32_t)(c1n >> (16 * w)) & 0xffffu) << 1);
+                        if (MODEL == 1)
+                            w_cq[w] = spread16((uint32_t)(m0 >> (16 * w)) & 0xffffu) |
+                                      (spread16((uint32_t)(m1n >> (16 * w)) & 0xffffu) << 1);
+                    }
+#pragma unroll
+                    for (int h = 0; h < NW; ++h) {
+                        w_pdl[h] = (uint32_t)(pdl >> (32 * h));
+                        w_ql[h] = (uint32_t)(ql >> (32 * h));
+                        frev[h] = 0u;
+                    }
+                    const uint32_t rrow = ring_lane + (uint32_t)(buf * 8 * L);
+                    int acc = 0, f0 = 0;
+                    f64x2 bnd = lds_f64x2(rrow);
+#pragma unroll
+                    for (int x = 0; x < L; ++x) {
+                        const int h = x >> 5, b = x & 31;
+                        if ((x & 1) == 0 && x > 0) bnd = lds_f64x2(rrow + 8u * (uint32_t)x);
+                        const double Bx = (x & 1) ? bnd.y : bnd.x;
+                        const uint32_t dl = bit_of(w_pdl[h], b) ^ (uint32_t)acc;
+                        // Delta E = -4J (cP' + dl) [ - 4K (cQ' + (dl ^ ql)) ]
+                        int kp = (int)dl + __builtin_amdgcn_sbfe((int)w_cp[x >> 4], 2 * (x & 15), 2);
+                        int kq = 0;
+                        if (MODEL == 1)
+                            kq = (int)(dl ^ bit_of(w_ql[h], b)) + __builtin_amdgcn_sbfe((int)w_cq[x >> 4], 2 * (x & 15), 2);
+                        if (x == L - 1) {
+                            // the right neighbour is site 0 of this row: its static
+                            // disagreement toggles if it flipped
+                            const int prl = (int)bit_of((uint32_t)(pr >> (32 * h)), b);
+                            kp += f0 * (1 - 2 * prl);
+                            if (MODEL == 1) {
+                                const int xrl = (int)bit_of((uint32_t)(xr >> (32 * h)), b);
+                                kq += f0 * (1 - 2 * xrl);
+                            }
+                        }
+                        int on = __mul24(m4J, kp) + o;
+                        if (MODEL == 1) on += __mul24(m4K, kq);
+                        tuned_attempt<SIMPLE>(on, tw, o, cbo, lnf, stride16, zero_off, sent, Bx, acc);
+                        if (x == 0) f0 = acc;
+                        frev[h] = (frev[h] << 1) | (uint32_t)acc;   // site x lands at bit (sites in word - 1 - x % 32)
+                    }
+                    uint64_t fw = 0;
+#pragma unroll
+                    for (int h = 0; h < NW; ++h) {
+                        constexpr int last = L - 32 * (NW - 1);   // sites in the last word
+                        const int nh = h == NW - 1 ? last : 32;
+                        fw |= (uint64_t)(__builtin_bitreverse32(frev[h]) >> (32 - nh)) << (32 * h);
+                    }
+                    const uint64_t Pn = P ^ fw;
+                    if (MODEL == 0) {
+                        cu[0] = Pn;
+                    } else {
+                        if (fl) cu[1] = Pn; else cu[0] = Pn;
+                    }
+#pragma unroll
+                    for (int q = 0; q < NP; ++q) {
+                        lat[(int64_t)(q * L + y) * kLanes] = cu[q];
+                        if (y == 0) r0[q] = cu[q];
+                        up[q] = cu[q];
+                        cu[q] = dn[q];
+                        dn[q] = (y + 2 < L) ? nx[q] : r0[q];
+                    }
+                }
+                __syncthreads();   // this row's bounds consumed, the next row's written
+                buf ^= 1;
+            }
+            if (MODEL == 1) fl ^= 1;
+        }
+    }
+
+    if (active) {
+        for (int b = 0; b < nbins; ++b) {
+            const uint32_t off = zero_off + (uint32_t)(b * stride16);
+            a.lng[hb + (int64_t)b * kLanes] = lds_f64(off);
+            a.cnt[hb + (int64_t)b * kLanes] = lds_u32_at(off + 8);
+        }
+        a.w_E[g] = (double)(o + tw.wmin);
+        a.w_flavor[g] = fl;
+    }
+}
+
+// Producer side: the acceptance bound B (accept_bound.hpp) of every attempt
+// of one row, written into each walker's own ring row: walker j's bounds for
+// attempt x of buffer `buf` sit at ring_base + j * ring_stride + buf * 8 L + 8 x.
+// L is a multiple of 4, so a row starts on a Philox block boundary and each
+// block fills four consecutive entries (two 16-byte stores).
+template <int MODEL, int L>
+__device__ __forceinline__ void fill_row_tuned(const DevGroup& G, const DevSystem& S, int wave, int nwaves, int lane,
+                                               uint64_t sweep_base, int row, uint32_t ring_lane_buf) {
+    constexpr int NPASS = MODEL == 0 ? 1 : 2;
+    constexpr int N = L * L;
+    constexpr uint64_t BLK = (uint64_t)((MODEL == 0 ? N : 2 * N) + 3) / 4;
+    const uint32_t walker = (uint32_t)(G.first_walker - S.first_walker + lane);
+    constexpr int per_sweep = NPASS * L;
+    const int s = row / per_sweep, rem = row - s * per_sweep;
+    const int pass = rem / L, y = rem - pass * L;
+    const int b0 = (pass * N + y * L) >> 2;
+    const uint64_t blk0 = (sweep_base + (uint64_t)s) * BLK;
+    for (int j = wave; j < L / 4; j += nwaves) {
+        const uint64_t bi = blk0 + (uint64_t)(b0 + j);
+        const u32x4 r = philox4x32_10(u32x4{(uint32_t)bi, (uint32_t)(bi >> 32), DOMAIN_SWEEP, walker}, S.k0, S.k1);
+        const uint32_t dst = ring_lane_buf + 32u * (uint32_t)j;
+        f64x2 v0, v1;
+        v0.x = accept_bound(r.x);
+        v0.y = accept_bound(r.y);
+        v1.x = accept_bound(r.z);
+        v1.y = accept_bound(r.w);
+        lds_store_f64x2(dst, v0);
+        lds_store_f64x2(dst + 16u, v1);
+    }
+}
+
+template <int MODEL, int L>
+__device__ __forceinline__ void produce_bounds_tuned(const DevGroup& G, const DevSystem& S, int wave, int nwaves,
+                                                     int lane, int nsweeps, uint64_t sweep_base, uint32_t ring_lane) {
+    constexpr int NPASS = MODEL == 0 ? 1 : 2;
+    const int rows = nsweeps * NPASS * L;
+    const bool mine = lane < G.nwalkers;
+    if (mine) fill_row_tuned<MODEL, L>(G, S, wave, nwaves, lane, sweep_base, 0, ring_lane);
+    __syncthreads();   // pairs with the walkers' barrier after loading the histograms
+    for (int r = 0; r < rows; ++r) {
+        if (mine && r + 1 < rows)
+            fill_row_tuned<MODEL, L>(G, S, wave, nwaves, lane, sweep_base, r + 1,
+                                     ring_lane + (uint32_t)(((r + 1) & 1) * 8 * L));
+        __syncthreads();   // pairs with the walkers' end-of-row barrier
+    }
+}
+
+// One workgroup = one walker wavefront + 1..3 producer wavefronts; groups are
+// taken persistently in blockIdx order.  LDS: (nbins + 2 guard) x group size
+// slots of 16 bytes, then one ring row of tuned_ring_stride(L) bytes per
+// walker (two buffers of L FP64 bounds, padded by 16 bytes so that the
+// walkers' 16-byte reads fall on different banks).
+template <int MODEL, int L, bool SIMPLE>
+__global__ void __launch_bounds__(256) k_sweep_tuned(KArgs a, const int32_t* groups, int ngroups, int nsweeps,
+                                                     uint64_t sweep_base, int guard) {
+    const int tid = threadIdx.x;
+    for (int gi = blockIdx.x; gi < ngroups; gi += gridDim.x) {
+        const DevGroup G = a.grp[groups[gi]];
+        const DevSystem S = a.sys[G.sys];
+        const uint32_t ring = lds_base() + (uint32_t)((G.nbins + 2 * guard) * G.nwalkers * kTunedSlot);
+        const int lane = tid & (kLanes - 1);
+        const uint32_t ring_lane = ring + (uint32_t)(lane * tuned_ring_stride(L));
+        if (tid < kLanes)
+            sweep_walker_tuned<MODEL, L, SIMPLE>(a, G, S, tid, nsweeps, guard, ring_lane);
+        else
+            produce_bounds_tuned<MODEL, L>(G, S, tid / kLanes - 1, (int)(blockDim.x / kLanes) - 1, lane, nsweeps,
+                                           sweep_base, ring_lane);
+        __syncthreads();   // the ring and LDS histograms are reused by the next group
+    }
+}
+
+}  // namespace rewl

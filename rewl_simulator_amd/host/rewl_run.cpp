@@ -4,7 +4,7 @@
 // Ashkin-Teller, Ising otherwise, as model_hamiltonians.hpp chooses by macro)
 // and prints the normalised microcanonical ln g/N per energy bin.
 //
-//   rewl_run L windows replicas final_increment
+//   rewl_run L windows replicas final_increment [average_mode]
 #include <cstdio>
 #include <cstdlib>
 
@@ -26,6 +26,7 @@ int main(int argc, char** argv) {
     rp.num_windows = argc > 2 ? std::atoi(argv[2]) : 2;
     rp.replicas_per_window = argc > 3 ? std::atoi(argv[3]) : 2;
     rp.final_increment = argc > 4 ? std::atof(argv[4]) : 1e-4;
+    rp.average_mode = argc > 5 ? std::atoi(argv[5]) : 0;
     try {
         rewl_host::REWL_simulation<Hamiltonian_t<double>> sim(sp, rp);
         const bool done = sim.simulate(1000000);

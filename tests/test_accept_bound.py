@@ -1,0 +1,99 @@
+"""The walker's one-comparison acceptance x < B(k) (csrc/accept_bound.hpp)
+against the reference rule it replaces (wang_landau.hpp:72-84):
+d = (float)x; accept if d <= 0, else if k 2^-32 < exp(-(double)d).  The
+header is compiled for the host with g++ (glibc log and exp) and probed at
+random draws and at the floats and midpoints around every boundary."""
+import os
+import subprocess
+import tempfile
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+PROBE = r"""
+#include <cmath>
+#include <cstdio>
+#include <cstdint>
+#include <random>
+#include "accept_bound.hpp"
+using namespace rewl;
+
+static bool reference(uint32_t k, double x) {
+    const float d = (float)x;
+    if (d <= 0.f) return true;
+    return (double)k * 0x1p-32 < std::exp(-(double)d);
+}
+
+// true if exp(-d) lies so close to the rounding midpoint above u that a libm
+// exp within 0.502 ulp may round either way (long double exp as the judge)
+static bool ambiguous(uint32_t k, double x) {
+    const float d = (float)x;
+    if (d <= 0.f || k == 0u) return false;
+    const double u = (double)k * 0x1p-32;
+    int e;
+    (void)std::frexp(u, &e);
+    const long double h = std::ldexp(1.0L, e - 54);
+    const long double y = std::exp(-(long double)d);
+    return std::fabs((double)(y - ((long double)u + h))) <= 0.004 * (double)h;
+}
+
+int main() {
+    std::mt19937_64 g(12345);
+    long long checked = 0, bad = 0, amb = 0;
+    auto probe = [&](uint32_t k, double x) {
+        ++checked;
+        if ((x < accept_bound(k)) != reference(k, x)) {
+            if (ambiguous(k, x)) { ++amb; return; }
+            if (bad < 5) std::printf("mismatch k=%u x=%.17g B=%.17g\n", k, x, accept_bound(k));
+            ++bad;
+        }
+    };
+    std::vector<uint32_t> ks = {0u, 1u, 2u, 3u, 0xffffffffu, 0xfffffffeu, 0x80000000u, 0x7fffffffu};
+    for (int i = 0; i < 200000; ++i) ks.push_back((uint32_t)g());
+    for (uint32_t k : ks) {
+        const double B = accept_bound(k);
+        // the floats and midpoints around the boundary, and doubles next to them
+        const float f = (float)B;
+        for (int s = -3; s <= 3; ++s) {
+            const float fs = bits_float(float_bits(f) + s);
+            const double v = (double)fs;
+            probe(k, v);
+            probe(k, std::nextafter(v, 1e300));
+            probe(k, std::nextafter(v, -1e300));
+            const double m = 0.5 * ((double)fs + (double)bits_float(float_bits(fs) + 1u));
+            probe(k, m);
+            probe(k, std::nextafter(m, 1e300));
+            probe(k, std::nextafter(m, -1e300));
+        }
+        probe(k, B);
+        probe(k, std::nextafter(B, -1e300));
+        // random differences, including <= 0 and far beyond 22.2
+        std::uniform_real_distribution<double> U(-2.0, 30.0);
+        for (int j = 0; j < 4; ++j) probe(k, U(g));
+        probe(k, 0.0);
+        probe(k, -0.0);
+        probe(k, 1e-300);
+        probe(k, 800.0);
+    }
+    // u = 0: accepted exactly while exp(-d) > 0
+    const float fz = 0x1.74910cp+9f;
+    if (!(std::exp(-(double)fz) > 0.0) || !(std::exp(-(double)bits_float(float_bits(fz) + 1u)) == 0.0)) {
+        std::printf("exp underflow threshold moved\n");
+        return 2;
+    }
+    std::printf("checked %lld mismatches %lld ambiguous %lld\n", checked, bad, amb);
+    return bad == 0 ? 0 : 1;
+}
+"""
+
+
+def test_accept_bound_matches_reference_rule():
+    with tempfile.TemporaryDirectory() as d:
+        src = os.path.join(d, "probe.cpp")
+        exe = os.path.join(d, "probe")
+        open(src, "w").write(PROBE)
+        subprocess.run(["g++", "-O2", "-std=c++17", "-ffp-contract=off", "-I",
+                        os.path.join(ROOT, "rewl_simulator_amd", "csrc"), src, "-o", exe], check=True)
+        out = subprocess.run([exe], capture_output=True, text=True, timeout=300)
+        print(out.stdout)
+        assert out.returncode == 0, out.stdout
+        assert "mismatches 0" in out.stdout

@@ -253,3 +253,36 @@ def test_ashkin_teller_full_spectrum_adjust_bit_exact(gpu, rng):
     assert eng.energy(eng.num_walkers - 1) >= eng.windows[-1]["e_min"]
     eng.sweeps(50); ora.sweeps(50)
     assert_same(eng, ora, "after 50 sweeps", cursors=(rng == abi.RNG_REPLAY))
+
+
+@pytest.mark.parametrize("L", [16, 24, 32, 48, 64])
+@pytest.mark.parametrize("model", [abi.ISING2D, abi.ASHKIN_TELLER2D])
+def test_row_unrolled_kernel_bit_exact(gpu, model, L):
+    """The row-unrolled kernel (sweep_kernel = 2) for every lattice size it is
+    built for: narrow windows so that walkers keep proposing across the window
+    edges (guard bins), a small ln f so that acceptances are frequent, and
+    exchanges and checks in between; bit-exact with the oracle."""
+    print(f"case: row-unrolled kernel model={model} L={L}", flush=True)
+    N = L * L
+    if model == abi.ISING2D:
+        s = configs.ising_full_range(L, 8.0, 6, 2)
+    else:
+        s = configs.at_system(L, -5.0 * N, 3.0 * N + 8.0, 8.0, 6, 2)
+    p = configs.params(sweep_kernel=2, lnf_initial=0.02, flatness_mode=abi.FLAT_MIN_OVER_MEAN, flatness=0.5,
+                       schedule=abi.SCHED_HALVE, sweeps_per_check=20, sweeps_per_exchange=5,
+                       adjust_mode=1 if model == abi.ASHKIN_TELLER2D else 0)
+    eng, ora = make_pair([s], p)
+    eng.adjust_to_range(); ora.adjust_to_range()
+    for _ in range(3):
+        for _ in range(4):
+            eng.sweeps(5); ora.sweeps(5)
+            eng.exchange(); ora.exchange()
+        eng.check(); ora.check()
+    assert_same(eng, ora, "after 60 sweeps", cursors=False)
+    window_states_equal(eng, ora)
+
+
+def test_row_unrolled_kernel_needs_a_built_size(gpu):
+    s = configs.ising_full_range(12, 8.0, 2, 2)
+    with pytest.raises(abi.REWLError):
+        Engine([s], configs.params(sweep_kernel=2))

@@ -65,9 +65,11 @@ def test_config1_one_over_t_converges_to_exact(gpu):
 
 def test_host_program_matches_exact(gpu):
     """The C++ host program (rewl_host.hpp over the C ABI) with the reference's
-    default Ising range [E0, 0), bin 8, 1/t schedule."""
+    default Ising range [E0, 0), bin 8, 1/t schedule, all-visited flatness and
+    exchange every sweep; equal-weight replica averaging (the count-weighted
+    average of the reference biases the window edges once replicas exchange)."""
     exe = os.path.join(ROOT, "rewl_simulator_amd", "host", "rewl_run")
-    out = subprocess.run([exe, "8", "2", "2", "1e-5"], capture_output=True, text=True, timeout=600)
+    out = subprocess.run([exe, "8", "2", "2", "1e-5", "1"], capture_output=True, text=True, timeout=600)
     assert out.returncode == 0, out.stderr
     rows = [l for l in out.stdout.splitlines() if l and not l.startswith("#")]
     data = np.array([[float(v) for v in r.split(",")] for r in rows])
