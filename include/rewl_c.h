@@ -132,10 +132,11 @@ typedef struct rewl_params {
                                     1 = equal weights 1/R, the alternative left in
                                     rewl_simulation.hpp:401 (unbiased when replicas
                                     also exchange with other windows)               */
-    int32_t sweep_kernel;        /* Philox integer path: 0 = auto (row-unrolled kernel
-                                    for L in {16,24,32,48,64}, else the row kernel);
-                                    1 = row kernel; 2 = row-unrolled (error if L is
-                                    not one of those sizes or producers < 0)       */
+    int32_t sweep_kernel;        /* Philox integer path: 0 = auto (3 for L in
+                                    {16,24,32,48,64}, else 1); 1 = row kernel;
+                                    2 = row-unrolled; 3 = row-unrolled with the
+                                    next slot computed for both outcomes (2 and 3
+                                    are an error for other L or producers < 0)     */
     int32_t reserved[3];
 } rewl_params;
 
@@ -227,8 +228,11 @@ int rewl_thermodynamics(const double* energies, const double* logdos, int32_t nu
 
 /* ---- device self-check ----------------------------------------------------- */
 /* No counterpart in the reference.  what = 0: largest absolute error of the
- * FP32 -ln(u) the sweep kernel screens acceptances with, over every 32-bit
- * Philox word (exhaustive); the kernel's acceptance band is 2^-12. */
+ * FP32 -ln(u) the row kernel screens acceptances with, over every 32-bit
+ * Philox word (exhaustive); that kernel's acceptance band is 2^-12.
+ * what = 1: number of 32-bit words, of 2^24 sampled (and 0, 1, 2^32-1,
+ * 2^32-2), whose acceptance bound B(k) computed on the device differs from
+ * the same computation on the host (glibc log). */
 int rewl_selfcheck(int32_t what, double* out);
 
 #ifdef __cplusplus

@@ -20,8 +20,18 @@
 // a single FP64 comparison in the walker; B is computed off the walker's
 // critical path.  The rule reproduces a correctly rounded exp; a libm exp
 // within 0.502 ulp (glibc's bound) can differ from it only where exp(-d) lies
-// within 0.002 ulp of the rounding midpoint u + h, and T carries the error of
-// one FP64 log, which moves f only when a float lies within an ulp of T.
+// within 0.002 ulp of the rounding midpoint u + h.
+//
+// T is evaluated as C - ln(1 + x) with a 1537-entry table (the same table on
+// the host and the device, built once from the host's log1p) and an order-8
+// series, |x| < 2^-9, in explicit FMAs:
+//   u >= 1/2:  u0 = 1 - i 2^-10 >= u (i = top bits of 2^32 - k), C = -ln u0,
+//              x = (u - u0)/u0 <= 0, so T = C + |ln(1 + x)| has no
+//              cancellation, down to u0 = 1, C = 0 for the top 2^22 words;
+//   u <  1/2:  k = 2^e m, m0 = 1 + i 2^-10 <= m, C = (32 - e) ln 2 - ln m0
+//              (ln 2 split hi + lo, the hi part exact in the product), T >= ln 2.
+// T is within a few FP64 ulps; that moves f only where a float lies that close
+// to T (about 2^-28 of the words), and there only d == f decides differently.
 #pragma once
 #include <math.h>
 #include <stdint.h>
@@ -63,20 +73,70 @@ REWL_AB_HD double float_le_bound(float f) {
     return (fb & 1u) ? m : bits_double(double_bits(m) + 1u);
 }
 
+struct BoundEntry {
+    double c, r;   // C = -ln u0 (u >= 1/2) or ln m0 (u < 1/2), R = 1 / u0 or 1 / m0
+};
+constexpr int kBoundTabHi = 513;                 // entries for u >= 1/2: i = 0..512
+constexpr int kBoundTabN = kBoundTabHi + 1024;   // then u < 1/2: i = 0..1023
+
+// Fills the table (host, glibc log1p).
+inline void build_bound_table(BoundEntry* t) {
+    for (int i = 0; i < kBoundTabHi; ++i) {
+        const double a = (double)i * 0x1p-10;
+        t[i].c = -log1p(-a);
+        t[i].r = 1.0 / (1.0 - a);
+    }
+    for (int i = 0; i < 1024; ++i) {
+        const double a = (double)i * 0x1p-10;
+        t[kBoundTabHi + i].c = log1p(a);
+        t[kBoundTabHi + i].r = 1.0 / (1.0 + a);
+    }
+}
+
+// T = -ln(u) - h/u for u = k 2^-32, k > 0 (h = half an FP64 ulp of u).
+// Both branches are evaluated and selected, so a wavefront issues one table
+// load per word and no divergent branches.
+REWL_AB_HD double accept_threshold(uint32_t k, const BoundEntry* tab) {
+    const double ln2_hi = 0x1.62e42fee00000p-1, ln2_lo = 0x1.a39ef35793c76p-33;
+    const bool hi = k >= 0x80000000u;
+    // u >= 1/2: u0 = 1 - ia 2^-10, u - u0 = -(j - ia 2^22) 2^-32 (exact)
+    const uint32_t j = (uint32_t)(0x100000000ull - (uint64_t)k);
+    const uint32_t ia = j >> 22;
+    // u < 1/2: k = 2^ex m, m0 = 1 + ib 2^-10, m - m0 = (low 21 bits of mb) 2^-31
+    const int lz = __builtin_clz(k | 1u);
+    const int ex = 31 - lz;
+    const uint32_t mb = k << lz;
+    const uint32_t ib = (mb >> 21) & 1023u;
+    const BoundEntry e = tab[hi ? ia : kBoundTabHi + ib];
+    const double dx = hi ? -((double)(j - (ia << 22)) * 0x1p-32) : (double)(mb & 0x1fffffu) * 0x1p-31;
+    const double x = dx * e.r;
+    const double n = (double)(32 - ex);
+    const double C = hi ? e.c : (n * ln2_hi - e.c) + n * ln2_lo;
+    const double hs = hi ? 0x1p-54 : 0x1p-53;
+    // ln(1 + x) to x^8
+    double p = fma(x, -1.0 / 8.0, 1.0 / 7.0);
+    p = fma(x, p, -1.0 / 6.0);
+    p = fma(x, p, 1.0 / 5.0);
+    p = fma(x, p, -1.0 / 4.0);
+    p = fma(x, p, 1.0 / 3.0);
+    p = fma(x, p, -1.0 / 2.0);
+    p = fma(x, p, 1.0);
+    const double T = C - x * p;
+    // h / u = hs R / (1 + x) to first order in x
+    return T - (hs * e.r) * (1.0 - x);
+}
+
 // Largest float f with u < RN(exp(-(double)f)) for u = k 2^-32, k > 0.
-REWL_AB_HD float accept_float_max(uint32_t k) {
-    const double u = (double)k * 0x1p-32;
-    int e;
-    (void)frexp(u, &e);                      // u = m 2^e, m in [0.5, 1): ulp(u) = 2^(e-53)
-    const double h = ldexp(1.0, e - 54);
-    const double T = -log(u) - h / u;
+REWL_AB_HD float accept_float_max(uint32_t k, const BoundEntry* tab) {
+    const double T = accept_threshold(k, tab);
     float f = (float)T;
     if ((double)f >= T) f = bits_float(float_bits(f) - 1u);
     return f;
 }
 
-REWL_AB_HD double accept_bound(uint32_t k) {
-    return float_le_bound(k == 0u ? 0x1.74910cp+9f : accept_float_max(k));
+REWL_AB_HD double accept_bound(uint32_t k, const BoundEntry* tab) {
+    const float f = accept_float_max(k, tab);
+    return float_le_bound(k == 0u ? 0x1.74910cp+9f : f);
 }
 
 }  // namespace rewl

@@ -71,6 +71,7 @@ struct KArgs {
     uint64_t ring_cap;
     int32_t* err;
     unsigned long long* exch_accepted;
+    const void* bound_tab;     // acceptance-bound table (accept_bound.hpp), kBoundTabN entries
     int32_t num_walkers, num_windows, num_groups;
 };
 

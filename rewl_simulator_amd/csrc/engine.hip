@@ -153,6 +153,7 @@ struct rewl_ctx {
     DevBuf<double> d_lng;
     DevBuf<uint32_t> d_cnt;
     DevBuf<unsigned long long> d_exch;
+    DevBuf<BoundEntry> d_bound_tab;
 
     // replay streams
     std::vector<std::mt19937_64> engines;
@@ -187,7 +188,7 @@ void destroy_ctx(rewl_ctx* c) {
     c->d_w_hist.release(); c->d_w_lat.release(); c->d_win_iter.release(); c->d_win_sweeps.release();
     c->d_w_E.release(); c->d_win_lnf.release(); c->d_fields.release();
     c->d_w_cursor.release(); c->d_lat.release(); c->d_visits.release(); c->d_ring.release(); c->d_ring_filled.release();
-    c->d_exclude.release(); c->d_lng.release(); c->d_cnt.release(); c->d_exch.release();
+    c->d_exclude.release(); c->d_lng.release(); c->d_cnt.release(); c->d_exch.release(); c->d_bound_tab.release();
     c->d_sys.release(); c->d_win.release(); c->d_grp.release();
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -290,7 +291,9 @@ int launch_sweep_tuned(rewl_ctx* c, int li, int nsweeps) {
                        (size_t)tuned_ring_stride(L) * lanes;
     const bool simple = c->tuned_simple[MODEL][li];
     const int np = c->tuned_nprod;
-    void* kern = simple ? (void*)k_sweep_tuned<MODEL, L, true> : (void*)k_sweep_tuned<MODEL, L, false>;
+    const bool spec = c->p.sweep_kernel != 2;   // 2: row-unrolled without slot speculation
+    void* kern = simple ? (spec ? (void*)k_sweep_tuned<MODEL, L, true, true> : (void*)k_sweep_tuned<MODEL, L, true, false>)
+                        : (spec ? (void*)k_sweep_tuned<MODEL, L, false, true> : (void*)k_sweep_tuned<MODEL, L, false, false>);
     HIP_TRY(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     KArgs args = c->args;
     const int32_t* gl = c->d_tuned_groups[MODEL][li].p;
@@ -382,7 +385,7 @@ int rewl_create(const rewl_system_desc* systems, int32_t num_systems, const rewl
     const rewl_params& P = *params;
     if (P.sweeps_per_check <= 0 || P.sweeps_per_exchange <= 0 || P.lnf_initial <= 0 ||
         (P.flatness_mode < 0 || P.flatness_mode > 3) || (P.schedule < 0 || P.schedule > 1) ||
-        (P.rng_mode < 0 || P.rng_mode > 1) || (P.adjust_mode < 0 || P.adjust_mode > 1) || (P.average_mode < 0 || P.average_mode > 1) || (P.sweep_kernel < 0 || P.sweep_kernel > 2) || P.walkers_per_group < 0 || P.walkers_per_group > kLanes)
+        (P.rng_mode < 0 || P.rng_mode > 1) || (P.adjust_mode < 0 || P.adjust_mode > 1) || (P.average_mode < 0 || P.average_mode > 1) || (P.sweep_kernel < 0 || P.sweep_kernel > 3) || P.walkers_per_group < 0 || P.walkers_per_group > kLanes)
         return fail(REWL_ERR_INVALID, "invalid rewl_params");
 
     rewl_ctx* c = new rewl_ctx();
@@ -469,10 +472,10 @@ int rewl_create(const rewl_system_desc* systems, int32_t num_systems, const rewl
         // a window of many bins gets fewer walkers per wavefront.
         const bool fast = S.intpath && P.rng_mode == REWL_RNG_PHILOX;
         c->nprod = P.producers < 0 ? 0 : (P.producers == 0 ? 1 : std::min<int>(P.producers, 3));
-        c->tuned_nprod = P.producers == 0 ? 3 : std::min<int>(std::max<int>(P.producers, 1), 3);
+        c->tuned_nprod = P.producers == 0 ? 7 : std::min<int>(std::max<int>(P.producers, 1), 7);
         const int li = tuned_index(S.d.L);
         const bool tuned = fast && P.producers >= 0 && li >= 0 && P.sweep_kernel != 1;
-        if (P.sweep_kernel == 2 && !tuned)
+        if (P.sweep_kernel >= 2 && !tuned)
             return bail(fail(REWL_ERR_INVALID, "the row-unrolled kernel needs the Philox integer path, producers >= 0 "
                                                "and L in {16, 24, 32, 48, 64}"));
         int wpg = kLanes;
@@ -674,6 +677,12 @@ int rewl_create(const rewl_system_desc* systems, int32_t num_systems, const rewl
         if (c->d_ring.alloc(1) != hipSuccess) return bail(fail(REWL_ERR_DEVICE, "hipMalloc"));
     }
 
+    {
+        std::vector<BoundEntry> tab(kBoundTabN);
+        build_bound_table(tab.data());
+        if ((rc = upload(c->d_bound_tab, tab, c->stream))) return bail(rc);
+    }
+
     KArgs& a = c->args;
     a.sys = c->d_sys.p; a.win = c->d_win.p; a.grp = c->d_grp.p;
     a.w_window = c->d_w_window.p; a.w_local = c->d_w_local.p; a.w_hist = c->d_w_hist.p; a.w_lat = c->d_w_lat.p;
@@ -682,7 +691,7 @@ int rewl_create(const rewl_system_desc* systems, int32_t num_systems, const rewl
     a.win_oot = c->d_win_oot.p; a.win_done = c->d_win_done.p; a.exclude = c->d_exclude.p;
     a.lat = c->d_lat.p; a.lng = c->d_lng.p; a.cnt = c->d_cnt.p; a.visits = c->d_visits.p;
     a.ring = c->d_ring.p; a.ring_filled = c->d_ring_filled.p; a.ring_cap = c->ring_cap;
-    a.err = c->d_err.p; a.exch_accepted = c->d_exch.p;
+    a.err = c->d_err.p; a.exch_accepted = c->d_exch.p; a.bound_tab = c->d_bound_tab.p;
     a.num_walkers = W; a.num_windows = (int)nwin; a.num_groups = (int)c->groups.size();
 
     // initial energies of the all-up lattices (recalculate_state)
@@ -1070,20 +1079,56 @@ int rewl_thermodynamics(const double* E, const double* lng, int32_t nb, int32_t 
 }
 
 int rewl_selfcheck(int32_t what, double* out) {
-    if (what != 0 || !out) return fail(REWL_ERR_INVALID, "unknown self-check");
+    if ((what != 0 && what != 1) || !out) return fail(REWL_ERR_INVALID, "unknown self-check");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(REWL_ERR_NO_DEVICE, "no HIP device");
+    if (what == 1) {
+        // device acceptance bounds (device log) against the same function on the
+        // host (glibc log) for 2^24 sampled words and the extreme words
+        const uint32_t n = 1u << 24;
+        std::vector<uint32_t> ks(n);
+        for (uint32_t i = 0; i < n; ++i) ks[i] = i * 257u + 12345u;
+        ks[0] = 0u;
+        ks[1] = 1u;
+        ks[2] = 0xffffffffu;
+        ks[3] = 0xfffffffeu;
+        std::vector<BoundEntry> tab(kBoundTabN);
+        build_bound_table(tab.data());
+        uint32_t* dk = nullptr;
+        double* db = nullptr;
+        BoundEntry* dt = nullptr;
+        HIP_TRY(hipMalloc((void**)&dk, n * sizeof(uint32_t)));
+        hipError_t e = hipMalloc((void**)&db, n * sizeof(double));
+        if (e == hipSuccess) e = hipMalloc((void**)&dt, kBoundTabN * sizeof(BoundEntry));
+        if (e == hipSuccess) e = hipMemcpy(dk, ks.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(dt, tab.data(), kBoundTabN * sizeof(BoundEntry), hipMemcpyHostToDevice);
+        if (e == hipSuccess) {
+            hipLaunchKernelGGL(k_selfcheck_bounds, dim3(4096), dim3(256), 0, nullptr, dk, n, dt, db);
+            e = hipGetLastError();
+        }
+        std::vector<double> got(n);
+        if (e == hipSuccess) e = hipMemcpy(got.data(), db, n * sizeof(double), hipMemcpyDeviceToHost);
+        (void)hipFree(dk);
+        (void)hipFree(db);
+        (void)hipFree(dt);
+        if (e != hipSuccess) return fail(REWL_ERR_DEVICE, std::string("rewl_selfcheck: ") + hipGetErrorString(e));
+        uint64_t bad = 0;
+        for (uint32_t i = 0; i < n; ++i) bad += double_bits(got[i]) != double_bits(accept_bound(ks[i], tab.data()));
+        *out = (double)bad;
+        return REWL_OK;
+    }
     unsigned long long* d = nullptr;
     HIP_TRY(hipMalloc((void**)&d, sizeof(unsigned long long)));
-    HIP_TRY(hipMemset(d, 0, sizeof(unsigned long long)));
+    hipError_t e = hipMemset(d, 0, sizeof(unsigned long long));
     // all 2^32 words in 4 launches of 2^30
-    for (uint32_t q = 0; q < 4; ++q) {
+    for (uint32_t q = 0; q < 4 && e == hipSuccess; ++q) {
         hipLaunchKernelGGL(k_selfcheck_neglog, dim3(4096), dim3(256), 0, nullptr, q << 30, 1u << 30, d);
-        HIP_TRY(hipGetLastError());
+        e = hipGetLastError();
     }
     unsigned long long bits = 0;
-    HIP_TRY(hipMemcpy(&bits, d, sizeof(bits), hipMemcpyDeviceToHost));
+    if (e == hipSuccess) e = hipMemcpy(&bits, d, sizeof(bits), hipMemcpyDeviceToHost);
     (void)hipFree(d);
+    if (e != hipSuccess) return fail(REWL_ERR_DEVICE, std::string("rewl_selfcheck: ") + hipGetErrorString(e));
     double v;
     std::memcpy(&v, &bits, sizeof(v));
     *out = v;

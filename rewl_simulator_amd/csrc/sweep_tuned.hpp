@@ -100,18 +100,22 @@ struct TunedWindow {
 // range test (|Delta E| <= guard x bin size).  Otherwise the reference's
 // phi(e) = (int)(e + 0.5) range test picks the +inf sentinel slot.
 template <bool SIMPLE>
-__device__ __forceinline__ void tuned_attempt(int on, const TunedWindow& tw, int& o, uint32_t& cbo, double lnf,
-                                              int stride16, uint32_t zero_off, uint32_t sent, double B, int& a) {
-    uint32_t nbo;
+__device__ __forceinline__ uint32_t tuned_slot(int on, const TunedWindow& tw, int stride16, uint32_t zero_off,
+                                               uint32_t sent) {
     if (SIMPLE) {
         // |Delta E| <= guard bins: in range because the walker is inside its
         // window (the host refuses to sweep before rewl_adjust_to_range)
-        nbo = (uint32_t)__mul24(on >> tw.sh, stride16) + zero_off;
-    } else {
-        const int En = on + tw.wmin;
-        const bool in = (unsigned)(phi_int(En) - tw.phi_lo) < (unsigned)tw.span;
-        nbo = in ? (uint32_t)__mul24(max(on, 0) >> tw.sh, stride16) + zero_off : sent;
+        return (uint32_t)__mul24(on >> tw.sh, stride16) + zero_off;
     }
+    const int En = on + tw.wmin;
+    const bool in = (unsigned)(phi_int(En) - tw.phi_lo) < (unsigned)tw.span;
+    return in ? (uint32_t)__mul24(max(on, 0) >> tw.sh, stride16) + zero_off : sent;
+}
+
+template <bool SIMPLE>
+__device__ __forceinline__ void tuned_attempt(int on, const TunedWindow& tw, int& o, uint32_t& cbo, double lnf,
+                                              int stride16, uint32_t zero_off, uint32_t sent, double B, int& a) {
+    const uint32_t nbo = tuned_slot<SIMPLE>(on, tw, stride16, zero_off, sent);
     // the current bin is re-read rather than carried: its ln g changes only
     // through the lds_add_f64 below, which LDS orders before this read
     const double ln = lds_f64(nbo);
@@ -124,7 +128,7 @@ __device__ __forceinline__ void tuned_attempt(int on, const TunedWindow& tw, int
     a = acc ? 1 : 0;
 }
 
-template <int MODEL, int L, bool SIMPLE>
+template <int MODEL, int L, bool SIMPLE, bool SPEC>
 __device__ __forceinline__ void sweep_walker_tuned(const KArgs& a, const DevGroup& G, const DevSystem& S, int lane,
                                                    int nsweeps, int guard, uint32_t ring_lane) {
     constexpr int NW = (L + 31) / 32;    // 32-bit words per row
@@ -227,12 +231,10 @@ __device__ __forceinline__ void sweep_walker_tuned(const KArgs& a, const DevGrou
                     const uint32_t rrow = ring_lane + (uint32_t)(buf * 8 * L);
                     int acc = 0, f0 = 0;
                     f64x2 bnd = lds_f64x2(rrow);
-#pragma unroll
-                    for (int x = 0; x < L; ++x) {
+                    // proposal offset of site x when the previous site's outcome is `a`
+                    auto propose = [&](int x, int a, int from) -> int {
                         const int h = x >> 5, b = x & 31;
-                        if ((x & 1) == 0 && x > 0) bnd = lds_f64x2(rrow + 8u * (uint32_t)x);
-                        const double Bx = (x & 1) ? bnd.y : bnd.x;
-                        const uint32_t dl = bit_of(w_pdl[h], b) ^ (uint32_t)acc;
+                        const uint32_t dl = bit_of(w_pdl[h], b) ^ (uint32_t)a;
                         // Delta E = -4J (cP' + dl) [ - 4K (cQ' + (dl ^ ql)) ]
                         int kp = (int)dl + __builtin_amdgcn_sbfe((int)w_cp[x >> 4], 2 * (x & 15), 2);
                         int kq = 0;
@@ -248,11 +250,58 @@ __device__ __forceinline__ void sweep_walker_tuned(const KArgs& a, const DevGrou
                                 kq += f0 * (1 - 2 * xrl);
                             }
                         }
-                        int on = __mul24(m4J, kp) + o;
+                        int on = __mul24(m4J, kp) + from;
                         if (MODEL == 1) on += __mul24(m4K, kq);
-                        tuned_attempt<SIMPLE>(on, tw, o, cbo, lnf, stride16, zero_off, sent, Bx, acc);
-                        if (x == 0) f0 = acc;
-                        frev[h] = (frev[h] << 1) | (uint32_t)acc;   // site x lands at bit (sites in word - 1 - x % 32)
+                        return on;
+                    };
+                    if (SPEC) {
+                        // The slot of the next proposal is computed for both outcomes of
+                        // the current attempt while its LDS reads are in flight; the
+                        // outcome only selects, so the chain from one decision to the
+                        // next read is one v_cndmask.
+                        int on = propose(0, 0, o);
+                        uint32_t nbo = tuned_slot<SIMPLE>(on, tw, stride16, zero_off, sent);
+#pragma unroll
+                        for (int x = 0; x < L; ++x) {
+                            const int h = x >> 5;
+                            if ((x & 1) == 0 && x > 0) bnd = lds_f64x2(rrow + 8u * (uint32_t)x);
+                            const double Bx = (x & 1) ? bnd.y : bnd.x;
+                            const double ln = lds_f64(nbo);
+                            const double lc = lds_f64(cbo);
+                            int on0 = 0, on1 = 0;
+                            uint32_t nbo0 = 0, nbo1 = 0;
+                            if (x + 1 < L) {
+                                on0 = propose(x + 1, 0, o);
+                                on1 = propose(x + 1, 1, on);
+                                nbo0 = tuned_slot<SIMPLE>(on0, tw, stride16, zero_off, sent);
+                                nbo1 = tuned_slot<SIMPLE>(on1, tw, stride16, zero_off, sent);
+                                // keep both slots materialised: selecting the offset first
+                                // and deriving the slot after the decision would put the
+                                // shift and multiply-add back on the chain
+                                asm volatile("" : "+v"(nbo0), "+v"(nbo1));
+                            }
+                            const bool ac = (ln - lc) < Bx;
+                            o = ac ? on : o;
+                            cbo = ac ? nbo : cbo;
+                            lds_add_f64(cbo, lnf);
+                            lds_add_u32(cbo + 8, 1u);
+                            acc = ac ? 1 : 0;
+                            if (x == 0) f0 = acc;
+                            frev[h] = (frev[h] << 1) | (uint32_t)acc;
+                            on = ac ? on1 : on0;
+                            nbo = ac ? nbo1 : nbo0;
+                        }
+                    } else {
+#pragma unroll
+                        for (int x = 0; x < L; ++x) {
+                            const int h = x >> 5;
+                            if ((x & 1) == 0 && x > 0) bnd = lds_f64x2(rrow + 8u * (uint32_t)x);
+                            const double Bx = (x & 1) ? bnd.y : bnd.x;
+                            const int on = propose(x, acc, o);
+                            tuned_attempt<SIMPLE>(on, tw, o, cbo, lnf, stride16, zero_off, sent, Bx, acc);
+                            if (x == 0) f0 = acc;
+                            frev[h] = (frev[h] << 1) | (uint32_t)acc;   // site x lands at bit (sites in word - 1 - x % 32)
+                        }
                     }
                     uint64_t fw = 0;
 #pragma unroll
@@ -301,7 +350,8 @@ __device__ __forceinline__ void sweep_walker_tuned(const KArgs& a, const DevGrou
 // block fills four consecutive entries (two 16-byte stores).
 template <int MODEL, int L>
 __device__ __forceinline__ void fill_row_tuned(const DevGroup& G, const DevSystem& S, int wave, int nwaves, int lane,
-                                               uint64_t sweep_base, int row, uint32_t ring_lane_buf) {
+                                               uint64_t sweep_base, int row, uint32_t ring_lane_buf,
+                                               const BoundEntry* tab) {
     constexpr int NPASS = MODEL == 0 ? 1 : 2;
     constexpr int N = L * L;
     constexpr uint64_t BLK = (uint64_t)((MODEL == 0 ? N : 2 * N) + 3) / 4;
@@ -316,10 +366,10 @@ __device__ __forceinline__ void fill_row_tuned(const DevGroup& G, const DevSyste
         const u32x4 r = philox4x32_10(u32x4{(uint32_t)bi, (uint32_t)(bi >> 32), DOMAIN_SWEEP, walker}, S.k0, S.k1);
         const uint32_t dst = ring_lane_buf + 32u * (uint32_t)j;
         f64x2 v0, v1;
-        v0.x = accept_bound(r.x);
-        v0.y = accept_bound(r.y);
-        v1.x = accept_bound(r.z);
-        v1.y = accept_bound(r.w);
+        v0.x = accept_bound(r.x, tab);
+        v0.y = accept_bound(r.y, tab);
+        v1.x = accept_bound(r.z, tab);
+        v1.y = accept_bound(r.w, tab);
         lds_store_f64x2(dst, v0);
         lds_store_f64x2(dst + 16u, v1);
     }
@@ -327,29 +377,34 @@ __device__ __forceinline__ void fill_row_tuned(const DevGroup& G, const DevSyste
 
 template <int MODEL, int L>
 __device__ __forceinline__ void produce_bounds_tuned(const DevGroup& G, const DevSystem& S, int wave, int nwaves,
-                                                     int lane, int nsweeps, uint64_t sweep_base, uint32_t ring_lane) {
+                                                     int lane, int nsweeps, uint64_t sweep_base, uint32_t ring_lane,
+                                                     const BoundEntry* tab) {
     constexpr int NPASS = MODEL == 0 ? 1 : 2;
     const int rows = nsweeps * NPASS * L;
     const bool mine = lane < G.nwalkers;
-    if (mine) fill_row_tuned<MODEL, L>(G, S, wave, nwaves, lane, sweep_base, 0, ring_lane);
+    if (mine) fill_row_tuned<MODEL, L>(G, S, wave, nwaves, lane, sweep_base, 0, ring_lane, tab);
     __syncthreads();   // pairs with the walkers' barrier after loading the histograms
     for (int r = 0; r < rows; ++r) {
         if (mine && r + 1 < rows)
             fill_row_tuned<MODEL, L>(G, S, wave, nwaves, lane, sweep_base, r + 1,
-                                     ring_lane + (uint32_t)(((r + 1) & 1) * 8 * L));
+                                     ring_lane + (uint32_t)(((r + 1) & 1) * 8 * L), tab);
         __syncthreads();   // pairs with the walkers' end-of-row barrier
     }
 }
 
-// One workgroup = one walker wavefront + 1..3 producer wavefronts; groups are
+// One workgroup = one walker wavefront + 1..7 producer wavefronts; groups are
 // taken persistently in blockIdx order.  LDS: (nbins + 2 guard) x group size
 // slots of 16 bytes, then one ring row of tuned_ring_stride(L) bytes per
 // walker (two buffers of L FP64 bounds, padded by 16 bytes so that the
 // walkers' 16-byte reads fall on different banks).
-template <int MODEL, int L, bool SIMPLE>
-__global__ void __launch_bounds__(256) k_sweep_tuned(KArgs a, const int32_t* groups, int ngroups, int nsweeps,
+template <int MODEL, int L, bool SIMPLE, bool SPEC>
+__global__ void __launch_bounds__(512) k_sweep_tuned(KArgs a, const int32_t* groups, int ngroups, int nsweeps,
                                                      uint64_t sweep_base, int guard) {
     const int tid = threadIdx.x;
+    // The walker wavefront shares its SIMD with a producer when there are more
+    // than three producers; at the highest priority it issues whenever it is
+    // ready and the producer fills the cycles it spends waiting on LDS.
+    if (tid < kLanes) __builtin_amdgcn_s_setprio(3);
     for (int gi = blockIdx.x; gi < ngroups; gi += gridDim.x) {
         const DevGroup G = a.grp[groups[gi]];
         const DevSystem S = a.sys[G.sys];
@@ -357,12 +412,18 @@ __global__ void __launch_bounds__(256) k_sweep_tuned(KArgs a, const int32_t* gro
         const int lane = tid & (kLanes - 1);
         const uint32_t ring_lane = ring + (uint32_t)(lane * tuned_ring_stride(L));
         if (tid < kLanes)
-            sweep_walker_tuned<MODEL, L, SIMPLE>(a, G, S, tid, nsweeps, guard, ring_lane);
+            sweep_walker_tuned<MODEL, L, SIMPLE, SPEC>(a, G, S, tid, nsweeps, guard, ring_lane);
         else
             produce_bounds_tuned<MODEL, L>(G, S, tid / kLanes - 1, (int)(blockDim.x / kLanes) - 1, lane, nsweeps,
-                                           sweep_base, ring_lane);
+                                           sweep_base, ring_lane, static_cast<const BoundEntry*>(a.bound_tab));
         __syncthreads();   // the ring and LDS histograms are reused by the next group
     }
+}
+
+// B(k) for k = ks[i] (the sweep kernel's acceptance bounds, device log)
+__global__ void k_selfcheck_bounds(const uint32_t* ks, uint32_t n, const BoundEntry* tab, double* out) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        out[i] = accept_bound(ks[i], tab);
 }
 
 }  // namespace rewl

@@ -24,9 +24,10 @@ class MLP:
         h = C.c_void_p()
         abi.check(self.lib.rewl_mlp_create(C.c_uint64(seed), C.byref(h)))
         self.h = h
-        # feature standardisation, applied on the host before the network
+        # feature standardisation and target shift, applied on the host
         self.x_shift = np.zeros(4, np.float64)
         self.x_scale = np.ones(4, np.float64)
+        self.t_shift = 0.0
 
     def close(self):
         if self.h:
@@ -95,20 +96,30 @@ class MLP:
         return hist[:steps]
 
     # standardised interface
-    def fit_standardisation(self, x):
+    def fit_standardisation(self, x, t=None):
         x = np.asarray(x, np.float64).reshape(-1, 4)
         self.x_shift = x.mean(0)
         sd = x.std(0)
         self.x_scale = np.where(sd > 0, sd, 1.0)
+        if t is not None:
+            self.t_shift = float(np.mean(t))
 
     def _std(self, x):
         return (np.asarray(x, np.float64).reshape(-1, 4) - self.x_shift) / self.x_scale
 
     def predict(self, x):
-        return self.forward_raw(self._std(x)).astype(np.float64)
+        return self.forward_raw(self._std(x)).astype(np.float64) + self.t_shift
 
     def train(self, x, t, w=None, **kw):
-        return self.train_raw(self._std(x), t, w, **kw)
+        return self.train_raw(self._std(x), np.asarray(t, np.float64) - self.t_shift, w, **kw)
+
+    def save(self, path):
+        np.savez(path, params=self.params, x_shift=self.x_shift, x_scale=self.x_scale, t_shift=self.t_shift)
+
+    def load(self, path):
+        d = np.load(path)
+        self.params = d["params"]
+        self.x_shift, self.x_scale, self.t_shift = d["x_shift"], d["x_scale"], float(d["t_shift"])
 
 
 def features(E, N, L, K, sigma):
@@ -131,7 +142,8 @@ def window_bin_energies(window):
 def warm_start(eng, mlp, sigma_of, lnf):
     """Seed every walker's ln g from the network's prediction at its window's
     bin energies (lower edges, as the stitched curves the network is trained
-    on report them), and start every window at ln f = lnf.
+    on report them), and start every window at ln f = lnf: a few host calls
+    (rewl_mlp_forward, rewl_set_logdos, rewl_set_window_lnf) per window.
 
     ``sigma_of(system_index)`` gives the disorder strength of each system."""
     for k, win in enumerate(eng.windows):

@@ -1,0 +1,236 @@
+"""Learned warm start for batched random-strain Ashkin-Teller REWL.
+
+Pipeline (every step on the GPU):
+
+  1. ``generate``: converged ln g curves of random-strain AT realisations
+     (the batched configuration: 8 windows x 4 walkers, 512 bins over
+     [E(all-up), E(checkerboard)], visited-support flatness 0.8, ln f halved to
+     1e-6), many realisations per context, stitched and normalised to
+     sum g = 4^N.
+  2. ``train``: the MLP of include/rewl_mlp.h, inputs (E/N, 1/L, K, sigma),
+     target ln g/N, trained with the library's HIP forward/backward/Adam
+     kernels on every visited bin of every curve.
+  3. ``evaluate``: held-out realisations (new disorder seeds) run from flat
+     ln g at ln f = 1 (plain WL, the fallback) and from the network's ln g at
+     ln f = 1e-3 (warm start), both to ln f < 1e-6; wall time of each run and
+     the ln g error of each against a reference run taken further (ln f
+     1e-8).  ln g errors are max |difference| over the visited bins after a
+     common anchor, per realisation.
+
+    python -m rewl_simulator_amd.warmstart --out DIR [--quick]
+"""
+import argparse
+import json
+import os
+import time
+
+import numpy as np
+
+from . import abi, configs
+from .engine import Engine
+from .mlp import MLP, features, warm_start
+
+K_AT = 0.5
+STATES_PER_SITE = 4.0
+
+
+def rfat_systems(specs, bins=512, windows=8, replicas=4):
+    """One random-strain AT system per (L, sigma, seed)."""
+    systems = []
+    for L, sigma, seed in specs:
+        h = configs.gaussian_strain(L, sigma, seed)
+        e0, e1 = configs.rfat_range(L, h, K=K_AT)
+        bs = (e1 - e0) / (bins - 1)
+        systems.append(configs.at_system(L, e0, e0 + bs * bins, bs, windows, replicas, K=K_AT, field=h, seed=seed))
+    return systems
+
+
+def rfat_params(lnf_final=1e-6, sweeps_per_check=250, average_mode=1):
+    return configs.params(flatness_mode=abi.FLAT_MIN_OVER_MEAN_VISITED, flatness=0.8, schedule=abi.SCHED_HALVE,
+                          lnf_final=lnf_final, sweeps_per_check=sweeps_per_check,
+                          sweeps_per_exchange=sweeps_per_check, adjust_mode=1, average_mode=average_mode)
+
+
+def visited_bins(eng, system):
+    """Energies (lower bin edges) that some walker of the system has visited."""
+    seen = set()
+    for w in eng.windows:
+        if w["system"] != system:
+            continue
+        R = eng.systems[system]["replicas_per_window"]
+        v = np.zeros(w["num_bins"], dtype=np.uint64)
+        for r in range(R):
+            v += eng.visits(w["first_walker"] + r) + eng.counts(w["first_walker"] + r).astype(np.uint64)
+        E = w["e_min"] + w["bin_size"] * np.arange(w["num_bins"])
+        seen.update(np.round(E[v > 0], 6).tolist())
+    return seen
+
+
+def curves(eng, specs):
+    """(L, sigma, E, ln g, visited) per system, ln g normalised to 4^N."""
+    out = []
+    for i, (L, sigma, seed) in enumerate(specs):
+        E, lng = eng.stitched(i, states_per_site=STATES_PER_SITE)
+        seen = visited_bins(eng, i)
+        vis = np.array([round(e, 6) in seen for e in E])
+        out.append(dict(L=L, sigma=sigma, seed=seed, E=E, lng=lng, visited=vis))
+    return out
+
+
+def run_until_done(eng, max_checks, log=None, every=50, name=""):
+    """Run check cycles until every window is at its final ln f; returns
+    (converged, seconds).  Runs in chunks of `every` checks so that progress
+    (the slowest window's ln f) can be logged."""
+    eng.synchronize()
+    t0 = time.perf_counter()
+    done, checks = False, 0
+    while checks < max_checks and not done:
+        k = min(every, max_checks - checks)
+        done = eng.run(k)
+        checks += k
+        if log is not None:
+            lnf = max(eng.window_info(w)["lnf"] for w in range(eng.num_windows))
+            log(f"  {name}: {checks} checks, slowest ln f {lnf:.2e}, {time.perf_counter() - t0:.1f} s")
+    eng.synchronize()
+    return done, time.perf_counter() - t0
+
+
+def generate(specs, lnf_final=1e-6, batch=256, max_checks=20000, log=print):
+    """Converged curves for every spec, `batch` realisations per context."""
+    out = []
+    for b in range(0, len(specs), batch):
+        part = specs[b:b + batch]
+        t0 = time.perf_counter()
+        with Engine(rfat_systems(part), rfat_params(lnf_final)) as eng:
+            eng.adjust_to_range()
+            done, secs = run_until_done(eng, max_checks, log=log, name="generate")
+            eng.finalize()
+            out.extend(curves(eng, part))
+            st = eng.stats()
+        log(f"generate: {b + len(part)}/{len(specs)} curves, converged={done}, {secs:.1f} s run, "
+            f"{st['sweeps']} sweeps, {time.perf_counter() - t0:.1f} s total")
+    return out
+
+
+def dataset(curve_list):
+    xs, ts = [], []
+    for c in curve_list:
+        L = c["L"]
+        N = L * L
+        m = c["visited"]
+        xs.append(features(c["E"][m], N, L, K_AT, c["sigma"]))
+        ts.append(c["lng"][m] / N)
+    return np.concatenate(xs), np.concatenate(ts)
+
+
+def train(curve_list, steps=4000, batch=16384, seed=3, log=print):
+    x, t = dataset(curve_list)
+    mlp = MLP(seed=seed)
+    mlp.fit_standardisation(x, t)
+    hist = mlp.train(x, t, steps=steps, batch=batch, lr=2e-3, lr_final=1e-5, seed=seed + 1, warmup=100)
+    log(f"train: {len(x)} points, {steps} steps, {mlp.last_ms():.0f} ms on the GPU, "
+        f"loss {hist[:10].mean():.3e} -> {hist[-50:].mean():.3e}")
+    return mlp, hist
+
+
+def predict_lng(mlp, E, L, sigma):
+    N = L * L
+    return N * mlp.predict(features(E, N, L, K_AT, sigma))
+
+
+def anchored_max_error(a, b, mask):
+    d = (a - b)[mask]
+    d = d - d[0]
+    return float(np.max(np.abs(d)))
+
+
+def evaluate(mlp, specs, lnf_final=1e-6, lnf_warm=1e-3, ref_lnf=1e-8, max_checks=40000, log=print):
+    systems = rfat_systems(specs)
+    res = {}
+    # reference: plain WL taken further
+    with Engine(systems, rfat_params(ref_lnf)) as eng:
+        eng.adjust_to_range()
+        done, secs = run_until_done(eng, max_checks, log=log, name="reference")
+        eng.finalize()
+        ref = curves(eng, specs)
+    log(f"evaluate: reference run to ln f {ref_lnf:g}: converged={done}, {secs:.1f} s")
+    res["reference"] = dict(lnf=ref_lnf, seconds=secs, converged=bool(done))
+    for name in ("plain", "warm"):
+        with Engine(systems, rfat_params(lnf_final)) as eng:
+            eng.adjust_to_range()
+            if name == "warm":
+                t0 = time.perf_counter()
+                warm_start(eng, mlp, lambda i: specs[i][1], lnf_warm)
+                t_init = time.perf_counter() - t0
+            else:
+                t_init = 0.0
+            done, secs = run_until_done(eng, max_checks, log=log, name=name)
+            st = eng.stats()
+            eng.finalize()
+            got = curves(eng, specs)
+        errs = [anchored_max_error(g["lng"], r["lng"], r["visited"] & g["visited"]) for g, r in zip(got, ref)]
+        res[name] = dict(seconds=secs, init_seconds=t_init, converged=bool(done), sweeps=st["sweeps"],
+                         flat_events=st["flat_events"], lng_error_max=float(np.max(errs)),
+                         lng_error_median=float(np.median(errs)), lnf_start=lnf_warm if name == "warm" else 1.0)
+        log(f"evaluate: {name}: {secs:.1f} s (+{t_init:.2f} s init), converged={done}, {st['sweeps']} sweeps, "
+            f"max ln g error {np.max(errs):.3f} (median {np.median(errs):.3f})")
+    # the network's own error before any sampling
+    pe = []
+    for r, (L, sigma, seed) in zip(ref, specs):
+        p = predict_lng(mlp, r["E"], L, sigma)
+        pe.append(anchored_max_error(p, r["lng"], r["visited"]))
+    res["prediction_error_max"] = float(np.max(pe))
+    res["prediction_error_median"] = float(np.median(pe))
+    res["speedup"] = res["plain"]["seconds"] / (res["warm"]["seconds"] + res["warm"]["init_seconds"])
+    return res
+
+
+def training_specs(n, sizes=(16, 24, 32, 48), sigmas=(0.0, 0.25, 0.5, 0.75, 1.0), seed0=50000):
+    """n (L, sigma, seed) triples cycling sizes and disorder strengths; the
+    disorder seeds are disjoint from the held-out ones."""
+    return [(sizes[i % len(sizes)], sigmas[(i // len(sizes)) % len(sigmas)], seed0 + i) for i in range(n)]
+
+
+def heldout_specs(n, L=48, sigma=0.5, seed0=900000):
+    return [(L, sigma, seed0 + i) for i in range(n)]
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--out", default="gpurun_out/warmstart")
+    ap.add_argument("--curves", type=int, default=2048)
+    ap.add_argument("--heldout", type=int, default=64)
+    ap.add_argument("--steps", type=int, default=4000)
+    ap.add_argument("--quick", action="store_true", help="small sizes for a fast end-to-end check")
+    a = ap.parse_args()
+    os.makedirs(a.out, exist_ok=True)
+    logf = open(os.path.join(a.out, "log.txt"), "a")
+
+    def log(msg):
+        print(msg, flush=True)
+        logf.write(msg + "\n")
+        logf.flush()
+
+    if a.quick:
+        train_specs = training_specs(a.curves, sizes=(12, 16), sigmas=(0.0, 0.5, 1.0))
+        held = heldout_specs(a.heldout, L=16)
+    else:
+        train_specs = training_specs(a.curves)
+        held = heldout_specs(a.heldout)
+    t0 = time.perf_counter()
+    cl = generate(train_specs, log=log)
+    gen_s = time.perf_counter() - t0
+    mlp, hist = train(cl, steps=a.steps, log=log)
+    mlp.save(os.path.join(a.out, "mlp.npz"))
+    res = evaluate(mlp, held, log=log)
+    res["generate_seconds"] = gen_s
+    res["train_ms"] = mlp.last_ms()
+    res["curves"] = len(cl)
+    res["train_points"] = int(sum(c["visited"].sum() for c in cl))
+    res["heldout"] = dict(count=len(held), L=held[0][0], sigma=held[0][1])
+    log(json.dumps(res))
+    json.dump(res, open(os.path.join(a.out, "result.json"), "w"), indent=1)
+
+
+if __name__ == "__main__":
+    main()

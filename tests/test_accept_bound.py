@@ -1,8 +1,9 @@
 """The walker's one-comparison acceptance x < B(k) (csrc/accept_bound.hpp)
 against the reference rule it replaces (wang_landau.hpp:72-84):
 d = (float)x; accept if d <= 0, else if k 2^-32 < exp(-(double)d).  The
-header is compiled for the host with g++ (glibc log and exp) and probed at
-random draws and at the floats and midpoints around every boundary."""
+header is compiled for the host with g++ (its table from glibc log1p, the rule
+judged by glibc exp) and probed at random draws, at every word near the
+table's interval ends, and at the floats and midpoints around every bound."""
 import os
 import subprocess
 import tempfile
@@ -14,8 +15,13 @@ PROBE = r"""
 #include <cstdio>
 #include <cstdint>
 #include <random>
+#include <vector>
 #include "accept_bound.hpp"
-using namespace rewl;
+using rewl::BoundEntry;
+using rewl::kBoundTabN;
+using rewl::build_bound_table;
+using rewl::bits_float;
+using rewl::float_bits;
 
 static bool reference(uint32_t k, double x) {
     const float d = (float)x;
@@ -37,6 +43,9 @@ static bool ambiguous(uint32_t k, double x) {
 }
 
 int main() {
+    std::vector<BoundEntry> tab(kBoundTabN);
+    build_bound_table(tab.data());
+    auto accept_bound = [&](uint32_t k) { return rewl::accept_bound(k, tab.data()); };
     std::mt19937_64 g(12345);
     long long checked = 0, bad = 0, amb = 0;
     auto probe = [&](uint32_t k, double x) {
@@ -49,6 +58,15 @@ int main() {
     };
     std::vector<uint32_t> ks = {0u, 1u, 2u, 3u, 0xffffffffu, 0xfffffffeu, 0x80000000u, 0x7fffffffu};
     for (int i = 0; i < 200000; ++i) ks.push_back((uint32_t)g());
+    // words at and next to the table's interval ends (both branches)
+    for (uint32_t i = 0; i <= 512; ++i)
+        for (int dj = -2; dj <= 2; ++dj) ks.push_back((uint32_t)(0x100000000ull - ((uint64_t)i << 22) + dj));
+    for (uint32_t e = 0; e < 31; ++e)
+        for (uint32_t i = 0; i < 1024; i += 7)
+            for (int dk = -1; dk <= 1; ++dk) {
+                const uint64_t base = ((1ull << e) * (1024 + i)) >> 10;
+                ks.push_back((uint32_t)(base + dk));
+            }
     for (uint32_t k : ks) {
         const double B = accept_bound(k);
         // the floats and midpoints around the boundary, and doubles next to them

@@ -255,20 +255,22 @@ def test_ashkin_teller_full_spectrum_adjust_bit_exact(gpu, rng):
     assert_same(eng, ora, "after 50 sweeps", cursors=(rng == abi.RNG_REPLAY))
 
 
+@pytest.mark.parametrize("kernel", [2, 3])
 @pytest.mark.parametrize("L", [16, 24, 32, 48, 64])
 @pytest.mark.parametrize("model", [abi.ISING2D, abi.ASHKIN_TELLER2D])
-def test_row_unrolled_kernel_bit_exact(gpu, model, L):
-    """The row-unrolled kernel (sweep_kernel = 2) for every lattice size it is
-    built for: narrow windows so that walkers keep proposing across the window
-    edges (guard bins), a small ln f so that acceptances are frequent, and
-    exchanges and checks in between; bit-exact with the oracle."""
-    print(f"case: row-unrolled kernel model={model} L={L}", flush=True)
+def test_row_unrolled_kernel_bit_exact(gpu, model, L, kernel):
+    """The row-unrolled kernel (sweep_kernel = 2, and 3 with the next slot
+    computed for both outcomes) for every lattice size it is built for: narrow
+    windows so that walkers keep proposing across the window edges (guard
+    bins), a small ln f so that acceptances are frequent, and exchanges and
+    checks in between; bit-exact with the oracle."""
+    print(f"case: row-unrolled kernel model={model} L={L} kernel={kernel}", flush=True)
     N = L * L
     if model == abi.ISING2D:
         s = configs.ising_full_range(L, 8.0, 6, 2)
     else:
         s = configs.at_system(L, -5.0 * N, 3.0 * N + 8.0, 8.0, 6, 2)
-    p = configs.params(sweep_kernel=2, lnf_initial=0.02, flatness_mode=abi.FLAT_MIN_OVER_MEAN, flatness=0.5,
+    p = configs.params(sweep_kernel=kernel, lnf_initial=0.02, flatness_mode=abi.FLAT_MIN_OVER_MEAN, flatness=0.5,
                        schedule=abi.SCHED_HALVE, sweeps_per_check=20, sweeps_per_exchange=5,
                        adjust_mode=1 if model == abi.ASHKIN_TELLER2D else 0)
     eng, ora = make_pair([s], p)
@@ -286,3 +288,15 @@ def test_row_unrolled_kernel_needs_a_built_size(gpu):
     s = configs.ising_full_range(12, 8.0, 2, 2)
     with pytest.raises(abi.REWLError):
         Engine([s], configs.params(sweep_kernel=2))
+
+
+def test_acceptance_bounds_device_equal_host(gpu):
+    """The row-unrolled kernel's acceptance bounds B(k) (device FP64 log)
+    equal the host's (glibc log), which tests/test_accept_bound.py pins
+    against the reference's u < exp(-d) rule."""
+    import ctypes as C
+    lib = abi.load_engine()
+    bad = C.c_double()
+    abi.check(lib.rewl_selfcheck(1, C.byref(bad)))
+    print(f"bounds differing from the host's: {bad.value:.0f} of 2^24", flush=True)
+    assert bad.value == 0
