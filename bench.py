@@ -35,7 +35,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--walkers-per-group", type=int, default=0)
     ap.add_argument("--producers", type=int, default=0, help="producer wavefronts (0 auto, -1 none)")
-    ap.add_argument("--sweep-kernel", type=int, default=0, help="0 auto, 1 row kernel, 2 row-unrolled kernel")
+    ap.add_argument("--sweep-kernel", type=int, default=0,
+                    help="0 auto, 1 row kernel, 2 row-unrolled, 3 + slot speculation, 4 + read-ahead")
     ap.add_argument("--converge", type=float, default=0.0, metavar="LNF",
                     help="instead of the throughput steps, time the run from the first sweep until every "
                          "window's ln f is below LNF")
@@ -190,7 +191,8 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": None,
-            "kernel": "k_sweep (Ising, integer path)",
+            "kernel": "k_sweep_tuned<Ising, L=64> (row-unrolled sweep)" if a.sweep_kernel in (0, 2, 3, 4)
+                      else "k_sweep_fast<Ising> (row kernel)",
             "avg_launch_ms": avg_launch_s * 1e3,
             "attempts_per_launch": att_per_launch,
             "bytes_per_attempt": BYTES_PER_ATTEMPT,

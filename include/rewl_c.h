@@ -135,8 +135,9 @@ typedef struct rewl_params {
     int32_t sweep_kernel;        /* Philox integer path: 0 = auto (3 for L in
                                     {16,24,32,48,64}, else 1); 1 = row kernel;
                                     2 = row-unrolled; 3 = row-unrolled with the
-                                    next slot computed for both outcomes (2 and 3
-                                    are an error for other L or producers < 0)     */
+                                    next slot computed for both outcomes; 4 = that
+                                    and the next ln g read ahead (2-4 are an error
+                                    for other L or producers < 0)                  */
     int32_t reserved[3];
 } rewl_params;
 

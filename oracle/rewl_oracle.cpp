@@ -766,7 +766,15 @@ int ora_stitch(const ora_ctx* c, int32_t sysi, double states_per_site, int32_t n
     std::vector<std::vector<double>> et(S.W), lt(S.W);
     for (int k = 0; k < S.W; ++k) {
         const Walker& w = c->w[S.first + k * S.R];
-        for (size_t b = 0; b < w.lng.size(); ++b) { et[k].push_back(w.hi.bin_min(b)); lt[k].push_back(w.lng[b]); }
+        /* bin energies on the system's global grid: the same value for the same
+         * bin in every window (min + bs b equals it whenever bs is dyadic, as in
+         * the reference's integer bins; with other bin sizes the per-window sums
+         * round differently and the reference's exact match would fail) */
+        const long long g0 = std::llround((w.hi.min - S.d.e_min) / S.d.bin_size);
+        for (size_t b = 0; b < w.lng.size(); ++b) {
+            et[k].push_back(S.d.e_min + S.d.bin_size * (double)(g0 + (long long)b));
+            lt[k].push_back(w.lng[b]);
+        }
     }
     double shifter = 0.;
     for (size_t b = 0; b < et[0].size(); ++b) { fe.push_back(et[0][b]); fl.push_back(lt[0][b] + shifter); }

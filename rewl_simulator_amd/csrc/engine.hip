@@ -291,9 +291,13 @@ int launch_sweep_tuned(rewl_ctx* c, int li, int nsweeps) {
                        (size_t)tuned_ring_stride(L) * lanes;
     const bool simple = c->tuned_simple[MODEL][li];
     const int np = c->tuned_nprod;
-    const bool spec = c->p.sweep_kernel != 2;   // 2: row-unrolled without slot speculation
-    void* kern = simple ? (spec ? (void*)k_sweep_tuned<MODEL, L, true, true> : (void*)k_sweep_tuned<MODEL, L, true, false>)
-                        : (spec ? (void*)k_sweep_tuned<MODEL, L, false, true> : (void*)k_sweep_tuned<MODEL, L, false, false>);
+    // sweep_kernel 2: no speculation, 3 (and auto): next slot for both outcomes,
+    // 4: that and the next ln g read ahead
+    const int sk = c->p.sweep_kernel;
+    void* kern;
+    if (sk == 2) kern = simple ? (void*)k_sweep_tuned<MODEL, L, true, 0> : (void*)k_sweep_tuned<MODEL, L, false, 0>;
+    else if (sk == 4) kern = simple ? (void*)k_sweep_tuned<MODEL, L, true, 2> : (void*)k_sweep_tuned<MODEL, L, false, 2>;
+    else kern = simple ? (void*)k_sweep_tuned<MODEL, L, true, 1> : (void*)k_sweep_tuned<MODEL, L, false, 1>;
     HIP_TRY(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     KArgs args = c->args;
     const int32_t* gl = c->d_tuned_groups[MODEL][li].p;
@@ -385,7 +389,7 @@ int rewl_create(const rewl_system_desc* systems, int32_t num_systems, const rewl
     const rewl_params& P = *params;
     if (P.sweeps_per_check <= 0 || P.sweeps_per_exchange <= 0 || P.lnf_initial <= 0 ||
         (P.flatness_mode < 0 || P.flatness_mode > 3) || (P.schedule < 0 || P.schedule > 1) ||
-        (P.rng_mode < 0 || P.rng_mode > 1) || (P.adjust_mode < 0 || P.adjust_mode > 1) || (P.average_mode < 0 || P.average_mode > 1) || (P.sweep_kernel < 0 || P.sweep_kernel > 3) || P.walkers_per_group < 0 || P.walkers_per_group > kLanes)
+        (P.rng_mode < 0 || P.rng_mode > 1) || (P.adjust_mode < 0 || P.adjust_mode > 1) || (P.average_mode < 0 || P.average_mode > 1) || (P.sweep_kernel < 0 || P.sweep_kernel > 4) || P.walkers_per_group < 0 || P.walkers_per_group > kLanes)
         return fail(REWL_ERR_INVALID, "invalid rewl_params");
 
     rewl_ctx* c = new rewl_ctx();
@@ -1014,7 +1018,12 @@ int rewl_stitch(rewl_ctx* c, int32_t si, double states_per_site, double* energie
         lt[k].resize(w.nbins);
         int rc = rewl_get_logdos(c, w.first_walker, lt[k].data(), w.nbins);
         if (rc) return rc;
-        for (int b = 0; b < w.nbins; ++b) et[k].push_back(w.emin + w.bs * (double)b);
+        // bin energies on the system's global grid: the same value for the same
+        // bin in every window (emin + bs b equals it whenever bs is dyadic, as in
+        // the reference's integer bins; with other bin sizes the per-window sums
+        // round differently and the reference's exact match would fail)
+        const long long g0 = std::llround((w.emin - S.d.e_min) / S.d.bin_size);
+        for (int b = 0; b < w.nbins; ++b) et[k].push_back(S.d.e_min + S.d.bin_size * (double)(g0 + b));
     }
     // concatenate_tables_multiple_overlap (concatenate_rewl_tables.hpp:167-305)
     std::vector<double> fe = et[0], fl = lt[0];

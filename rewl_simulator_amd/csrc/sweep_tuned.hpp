@@ -128,7 +128,7 @@ __device__ __forceinline__ void tuned_attempt(int on, const TunedWindow& tw, int
     a = acc ? 1 : 0;
 }
 
-template <int MODEL, int L, bool SIMPLE, bool SPEC>
+template <int MODEL, int L, bool SIMPLE, int SPEC>
 __device__ __forceinline__ void sweep_walker_tuned(const KArgs& a, const DevGroup& G, const DevSystem& S, int lane,
                                                    int nsweeps, int guard, uint32_t ring_lane) {
     constexpr int NW = (L + 31) / 32;    // 32-bit words per row
@@ -173,6 +173,8 @@ __device__ __forceinline__ void sweep_walker_tuned(const KArgs& a, const DevGrou
     // the walker is inside its window (rewl_adjust_to_range); the clamp keeps
     // the slot in bounds regardless
     uint32_t cbo = (uint32_t)(min(max(o, 0) >> tw.sh, max(nbins - 1, 0)) * stride16) + zero_off;
+    // ln g of the current bin, carried in a register by the read-ahead form
+    double lc = active ? lds_f64(cbo) : 0.0;
 
     const int m4J = -2 * S.twoJ, m4K = -2 * S.twoK;
     int buf = 0;
@@ -230,7 +232,8 @@ __device__ __forceinline__ void sweep_walker_tuned(const KArgs& a, const DevGrou
                     }
                     const uint32_t rrow = ring_lane + (uint32_t)(buf * 8 * L);
                     int acc = 0, f0 = 0;
-                    f64x2 bnd = lds_f64x2(rrow);
+                    // bounds of attempts (x, x + 1) in bnd, of (x + 2, x + 3) already in flight
+                    f64x2 bnd = lds_f64x2(rrow), bnd_next = lds_f64x2(rrow + 16u);
                     // proposal offset of site x when the previous site's outcome is `a`
                     auto propose = [&](int x, int a, int from) -> int {
                         const int h = x >> 5, b = x & 31;
@@ -254,7 +257,55 @@ __device__ __forceinline__ void sweep_walker_tuned(const KArgs& a, const DevGrou
                         if (MODEL == 1) on += __mul24(m4K, kq);
                         return on;
                     };
-                    if (SPEC) {
+                    if (SPEC == 2) {
+                        // Read-ahead: for both outcomes of the current attempt, the next
+                        // proposal's slot is computed and its ln g read while the
+                        // current decision is made; the next decision selects.  A
+                        // read issued before this attempt's update is stale exactly
+                        // when it hit the bin just updated, whose value is lc.
+                        int onA = propose(0, 0, o), onB = onA;
+                        uint32_t nboA = tuned_slot<SIMPLE>(onA, tw, stride16, zero_off, sent), nboB = nboA;
+                        double l0 = lds_f64(nboA), l1 = l0;
+                        bool pa = false;
+#pragma unroll
+                        for (int x = 0; x < L; ++x) {
+                            // keep each attempt's reads ahead of the next attempt's use:
+                            // no instruction moves across this point
+                            __builtin_amdgcn_sched_barrier(0);
+                            const int h = x >> 5;
+                            if ((x & 1) == 0 && x > 0) {
+                                bnd = bnd_next;
+                                if (x + 2 < L) bnd_next = lds_f64x2(rrow + 8u * (uint32_t)(x + 2));
+                            }
+                            const double Bx = (x & 1) ? bnd.y : bnd.x;
+                            const int on = pa ? onB : onA;
+                            const uint32_t nbo = pa ? nboB : nboA;
+                            int on0 = 0, on1 = 0;
+                            uint32_t nbo0 = 0, nbo1 = 0;
+                            double r0 = 0.0, r1 = 0.0;
+                            if (x + 1 < L) {
+                                on0 = propose(x + 1, 0, o);
+                                on1 = propose(x + 1, 1, on);
+                                nbo0 = tuned_slot<SIMPLE>(on0, tw, stride16, zero_off, sent);
+                                nbo1 = tuned_slot<SIMPLE>(on1, tw, stride16, zero_off, sent);
+                                r0 = lds_f64(nbo0);
+                                r1 = lds_f64(nbo1);
+                            }
+                            __builtin_amdgcn_sched_barrier(0);   // the reads issue before the decision
+                            double ln = pa ? l1 : l0;
+                            ln = nbo == cbo ? lc : ln;
+                            const bool ac = (ln - lc) < Bx;
+                            o = ac ? on : o;
+                            cbo = ac ? nbo : cbo;
+                            lc = (ac ? ln : lc) + lnf;
+                            lds_store_f64(cbo, lc);
+                            lds_add_u32(cbo + 8, 1u);
+                            acc = ac ? 1 : 0;
+                            if (x == 0) f0 = acc;
+                            frev[h] = (frev[h] << 1) | (uint32_t)acc;
+                            onA = on0; onB = on1; nboA = nbo0; nboB = nbo1; l0 = r0; l1 = r1; pa = ac;
+                        }
+                    } else if (SPEC == 1) {
                         // The slot of the next proposal is computed for both outcomes of
                         // the current attempt while its LDS reads are in flight; the
                         // outcome only selects, so the chain from one decision to the
@@ -264,7 +315,10 @@ __device__ __forceinline__ void sweep_walker_tuned(const KArgs& a, const DevGrou
 #pragma unroll
                         for (int x = 0; x < L; ++x) {
                             const int h = x >> 5;
-                            if ((x & 1) == 0 && x > 0) bnd = lds_f64x2(rrow + 8u * (uint32_t)x);
+                            if ((x & 1) == 0 && x > 0) {
+                                bnd = bnd_next;
+                                if (x + 2 < L) bnd_next = lds_f64x2(rrow + 8u * (uint32_t)(x + 2));
+                            }
                             const double Bx = (x & 1) ? bnd.y : bnd.x;
                             const double ln = lds_f64(nbo);
                             const double lc = lds_f64(cbo);
@@ -295,7 +349,10 @@ __device__ __forceinline__ void sweep_walker_tuned(const KArgs& a, const DevGrou
 #pragma unroll
                         for (int x = 0; x < L; ++x) {
                             const int h = x >> 5;
-                            if ((x & 1) == 0 && x > 0) bnd = lds_f64x2(rrow + 8u * (uint32_t)x);
+                            if ((x & 1) == 0 && x > 0) {
+                                bnd = bnd_next;
+                                if (x + 2 < L) bnd_next = lds_f64x2(rrow + 8u * (uint32_t)(x + 2));
+                            }
                             const double Bx = (x & 1) ? bnd.y : bnd.x;
                             const int on = propose(x, acc, o);
                             tuned_attempt<SIMPLE>(on, tw, o, cbo, lnf, stride16, zero_off, sent, Bx, acc);
@@ -397,7 +454,7 @@ __device__ __forceinline__ void produce_bounds_tuned(const DevGroup& G, const De
 // slots of 16 bytes, then one ring row of tuned_ring_stride(L) bytes per
 // walker (two buffers of L FP64 bounds, padded by 16 bytes so that the
 // walkers' 16-byte reads fall on different banks).
-template <int MODEL, int L, bool SIMPLE, bool SPEC>
+template <int MODEL, int L, bool SIMPLE, int SPEC>
 __global__ void __launch_bounds__(512) k_sweep_tuned(KArgs a, const int32_t* groups, int ngroups, int nsweeps,
                                                      uint64_t sweep_base, int guard) {
     const int tid = threadIdx.x;

@@ -61,7 +61,9 @@ def visited_bins(eng, system):
         v = np.zeros(w["num_bins"], dtype=np.uint64)
         for r in range(R):
             v += eng.visits(w["first_walker"] + r) + eng.counts(w["first_walker"] + r).astype(np.uint64)
-        E = w["e_min"] + w["bin_size"] * np.arange(w["num_bins"])
+        s = eng.systems[system]
+        g0 = int(round((w["e_min"] - s["e_min"]) / s["bin_size"]))
+        E = s["e_min"] + s["bin_size"] * (g0 + np.arange(w["num_bins"]))
         seen.update(np.round(E[v > 0], 6).tolist())
     return seen
 
@@ -145,43 +147,73 @@ def anchored_max_error(a, b, mask):
 
 
 def evaluate(mlp, specs, lnf_final=1e-6, lnf_warm=1e-3, ref_lnf=1e-8, max_checks=40000, log=print):
+    """Plain (flat ln g, ln f = 1) and warm-started (network ln g, ln f =
+    lnf_warm) runs of the same held-out systems to ln f < lnf_final.  The
+    plain result judges the warm one (max |difference| over commonly visited
+    bins, anchored); with ref_lnf > 0 both are also compared to a plain run
+    taken to ref_lnf."""
     systems = rfat_systems(specs)
     res = {}
-    # reference: plain WL taken further
-    with Engine(systems, rfat_params(ref_lnf)) as eng:
-        eng.adjust_to_range()
-        done, secs = run_until_done(eng, max_checks, log=log, name="reference")
-        eng.finalize()
-        ref = curves(eng, specs)
-    log(f"evaluate: reference run to ln f {ref_lnf:g}: converged={done}, {secs:.1f} s")
-    res["reference"] = dict(lnf=ref_lnf, seconds=secs, converged=bool(done))
+    ref = None
+    if ref_lnf > 0:
+        with Engine(systems, rfat_params(ref_lnf)) as eng:
+            eng.adjust_to_range()
+            done, secs = run_until_done(eng, max_checks, log=log, name="reference")
+            eng.finalize()
+            ref = curves(eng, specs)
+        log(f"evaluate: reference run to ln f {ref_lnf:g}: converged={done}, {secs:.1f} s")
+        res["reference"] = dict(lnf=ref_lnf, seconds=secs, converged=bool(done))
+    out = {}
     for name in ("plain", "warm"):
         with Engine(systems, rfat_params(lnf_final)) as eng:
             eng.adjust_to_range()
+            t_init = 0.0
             if name == "warm":
                 t0 = time.perf_counter()
                 warm_start(eng, mlp, lambda i: specs[i][1], lnf_warm)
                 t_init = time.perf_counter() - t0
-            else:
-                t_init = 0.0
             done, secs = run_until_done(eng, max_checks, log=log, name=name)
             st = eng.stats()
             eng.finalize()
-            got = curves(eng, specs)
-        errs = [anchored_max_error(g["lng"], r["lng"], r["visited"] & g["visited"]) for g, r in zip(got, ref)]
+            out[name] = curves(eng, specs)
         res[name] = dict(seconds=secs, init_seconds=t_init, converged=bool(done), sweeps=st["sweeps"],
-                         flat_events=st["flat_events"], lng_error_max=float(np.max(errs)),
-                         lng_error_median=float(np.median(errs)), lnf_start=lnf_warm if name == "warm" else 1.0)
-        log(f"evaluate: {name}: {secs:.1f} s (+{t_init:.2f} s init), converged={done}, {st['sweeps']} sweeps, "
-            f"max ln g error {np.max(errs):.3f} (median {np.median(errs):.3f})")
-    # the network's own error before any sampling
-    pe = []
-    for r, (L, sigma, seed) in zip(ref, specs):
-        p = predict_lng(mlp, r["E"], L, sigma)
-        pe.append(anchored_max_error(p, r["lng"], r["visited"]))
+                         flat_events=st["flat_events"], lnf_start=lnf_warm if name == "warm" else 1.0)
+        if ref is not None:
+            errs = [anchored_max_error(g["lng"], r["lng"], r["visited"] & g["visited"]) for g, r in zip(out[name], ref)]
+            res[name].update(lng_error_max=float(np.max(errs)), lng_error_median=float(np.median(errs)))
+        log(f"evaluate: {name}: {secs:.1f} s (+{t_init:.2f} s init), converged={done}, {st['sweeps']} sweeps")
+    dev = [anchored_max_error(w["lng"], p["lng"], w["visited"] & p["visited"]) for w, p in zip(out["warm"], out["plain"])]
+    res["warm_vs_plain_max"] = float(np.max(dev))
+    res["warm_vs_plain_median"] = float(np.median(dev))
+    # the network's own error before any sampling, against the best curve at hand
+    judge = ref if ref is not None else out["plain"]
+    pe = [anchored_max_error(predict_lng(mlp, r["E"], L, sigma), r["lng"], r["visited"])
+          for r, (L, sigma, seed) in zip(judge, specs)]
     res["prediction_error_max"] = float(np.max(pe))
     res["prediction_error_median"] = float(np.median(pe))
-    res["speedup"] = res["plain"]["seconds"] / (res["warm"]["seconds"] + res["warm"]["init_seconds"])
+    res["speedup_wall"] = res["plain"]["seconds"] / (res["warm"]["seconds"] + res["warm"]["init_seconds"])
+    res["speedup_sweeps"] = res["plain"]["sweeps"] / max(res["warm"]["sweeps"], 1)
+    log(f"evaluate: speedup {res['speedup_wall']:.2f}x wall, {res['speedup_sweeps']:.2f}x sweeps; "
+        f"warm vs plain max |d ln g| {res['warm_vs_plain_max']:.3f}; network error {res['prediction_error_max']:.3f}")
+    return res
+
+
+def budget_run(mlp, specs, checks, lnf_warm=1e-3, log=print):
+    """Plain and warm-started runs of the same systems for a fixed number of
+    check cycles: the slowest and median window ln f each reaches."""
+    systems = rfat_systems(specs)
+    res = {}
+    for name in ("plain", "warm"):
+        with Engine(systems, rfat_params(1e-6)) as eng:
+            eng.adjust_to_range()
+            if name == "warm":
+                warm_start(eng, mlp, lambda i: specs[i][1], lnf_warm)
+            done, secs = run_until_done(eng, checks, log=log, name=name + " (budget)")
+            lnf = np.array([eng.window_info(k)["lnf"] for k in range(eng.num_windows)])
+        res[name] = dict(checks=checks, seconds=secs, converged=bool(done), slowest_lnf=float(lnf.max()),
+                         median_lnf=float(np.median(lnf)), frac_windows_done=float((lnf < 1e-6).mean()))
+        log(f"budget: {name}: slowest ln f {lnf.max():.2e}, median {np.median(lnf):.2e}, "
+            f"{(lnf < 1e-6).mean():.2f} of windows done after {checks} checks")
     return res
 
 
@@ -198,10 +230,16 @@ def heldout_specs(n, L=48, sigma=0.5, seed0=900000):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--out", default="gpurun_out/warmstart")
-    ap.add_argument("--curves", type=int, default=2048)
+    ap.add_argument("--curves", type=int, default=256)
+    ap.add_argument("--sizes", default="12,16,20,24")
+    ap.add_argument("--sigmas", default="0,0.25,0.5,0.75,1")
     ap.add_argument("--heldout", type=int, default=64)
+    ap.add_argument("--heldout-L", type=int, default=24)
+    ap.add_argument("--lnf-final", type=float, default=1e-6)
+    ap.add_argument("--ref-lnf", type=float, default=0.0)
     ap.add_argument("--steps", type=int, default=4000)
-    ap.add_argument("--quick", action="store_true", help="small sizes for a fast end-to-end check")
+    ap.add_argument("--budget-L", type=int, default=0, help="also a fixed-budget comparison at this L")
+    ap.add_argument("--budget-checks", type=int, default=200)
     a = ap.parse_args()
     os.makedirs(a.out, exist_ok=True)
     logf = open(os.path.join(a.out, "log.txt"), "a")
@@ -211,23 +249,25 @@ def main():
         logf.write(msg + "\n")
         logf.flush()
 
-    if a.quick:
-        train_specs = training_specs(a.curves, sizes=(12, 16), sigmas=(0.0, 0.5, 1.0))
-        held = heldout_specs(a.heldout, L=16)
-    else:
-        train_specs = training_specs(a.curves)
-        held = heldout_specs(a.heldout)
+    sizes = tuple(int(v) for v in a.sizes.split(","))
+    sigmas = tuple(float(v) for v in a.sigmas.split(","))
+    train_specs = training_specs(a.curves, sizes=sizes, sigmas=sigmas)
+    held = heldout_specs(a.heldout, L=a.heldout_L)
     t0 = time.perf_counter()
-    cl = generate(train_specs, log=log)
+    cl = generate(train_specs, lnf_final=a.lnf_final, log=log)
     gen_s = time.perf_counter() - t0
     mlp, hist = train(cl, steps=a.steps, log=log)
     mlp.save(os.path.join(a.out, "mlp.npz"))
-    res = evaluate(mlp, held, log=log)
+    res = evaluate(mlp, held, lnf_final=a.lnf_final, ref_lnf=a.ref_lnf, log=log)
     res["generate_seconds"] = gen_s
     res["train_ms"] = mlp.last_ms()
     res["curves"] = len(cl)
     res["train_points"] = int(sum(c["visited"].sum() for c in cl))
+    res["train_sizes"] = list(sizes)
     res["heldout"] = dict(count=len(held), L=held[0][0], sigma=held[0][1])
+    if a.budget_L:
+        res["budget"] = budget_run(mlp, heldout_specs(a.heldout, L=a.budget_L), a.budget_checks, log=log)
+        res["budget"]["L"] = a.budget_L
     log(json.dumps(res))
     json.dump(res, open(os.path.join(a.out, "result.json"), "w"), indent=1)
 

@@ -255,12 +255,13 @@ def test_ashkin_teller_full_spectrum_adjust_bit_exact(gpu, rng):
     assert_same(eng, ora, "after 50 sweeps", cursors=(rng == abi.RNG_REPLAY))
 
 
-@pytest.mark.parametrize("kernel", [2, 3])
+@pytest.mark.parametrize("kernel", [2, 3, 4])
 @pytest.mark.parametrize("L", [16, 24, 32, 48, 64])
 @pytest.mark.parametrize("model", [abi.ISING2D, abi.ASHKIN_TELLER2D])
 def test_row_unrolled_kernel_bit_exact(gpu, model, L, kernel):
-    """The row-unrolled kernel (sweep_kernel = 2, and 3 with the next slot
-    computed for both outcomes) for every lattice size it is built for: narrow
+    """The row-unrolled kernel (sweep_kernel = 2; 3 with the next slot
+    computed for both outcomes; 4 with the next ln g read ahead too) for every
+    lattice size it is built for: narrow
     windows so that walkers keep proposing across the window edges (guard
     bins), a small ln f so that acceptances are frequent, and exchanges and
     checks in between; bit-exact with the oracle."""
