@@ -21,7 +21,14 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# LDS, where the histograms live: 256 CUs x 128 B/clk x 2.4 GHz
+LDS_PEAK_GBS = 256 * 128 * 2.4
 BYTES_PER_ATTEMPT = 32.25   # 8 B read ln g[new] + 16 B RMW ln g[cur] + 8 B RMW count + 0.25 B lattice
+# Measured HBM bytes of one sweep launch of this workload with the default
+# kernel, keyed by sweeps per launch: rocprofv3 --pmc FETCH_SIZE (x 2.0) and
+# WRITE_SIZE (x 1.0), the scales read off a plain 256 MiB copy
+# (profiles/k_sweep_tuned_ising_L64_counters.md).
+MEASURED_TRAFFIC = {1000: (2.0 * 5618.125 + 1.0 * 7537.90625) * 1024}
 
 
 def parse():
@@ -37,10 +44,15 @@ def parse():
     ap.add_argument("--producers", type=int, default=0, help="producer wavefronts (0 auto, -1 none)")
     ap.add_argument("--sweep-kernel", type=int, default=0,
                     help="0 auto, 1 row kernel, 2 row-unrolled, 3 + slot speculation, 4 + read-ahead")
+    ap.add_argument("--saturate", type=int, default=4, metavar="R",
+                    help="also time R independent copies of the workload in one context (R x 2048 walkers, "
+                         "enough to fill the chip); 0 to skip")
     ap.add_argument("--converge", type=float, default=0.0, metavar="LNF",
                     help="instead of the throughput steps, time the run from the first sweep until every "
                          "window's ln f is below LNF")
     ap.add_argument("--max-checks", type=int, default=200000)
+    ap.add_argument("--average-mode", type=int, default=0,
+                    help="replica averaging: 0 count-weighted (the reference), 1 equal weights")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="only run a plain device-to-device copy of BYTES (counter calibration) and exit")
     return ap.parse_args()
@@ -76,10 +88,56 @@ def cpu_baseline(systems, params, target_s, threads):
                        f"initial state, ln f = 1, {att} attempts in {sec:.2f} s")
 
 
+def saturating_batch(systems, params, copies, steps=2):
+    """The same sweep on `copies` independent copies of the workload (own
+    seeds) in one context: 2048 walkers fill 63 of 256 CUs, so this shows the
+    kernel's throughput on a full chip.  Reported beside the workload's own
+    figure, never as it."""
+    from rewl_simulator_amd.engine import Engine
+    batch = [dict(s, seed=s["seed"] + 7919 * r) for r in range(copies) for s in systems]
+    eng = Engine(batch, params)
+    eng.adjust_to_range()
+    n = params["sweeps_per_check"]
+    eng.sweeps(n)
+    eng.synchronize()
+    st0 = eng.stats()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        eng.sweeps(n)
+    eng.synchronize()
+    t1 = time.perf_counter()
+    st1 = eng.stats()
+    eng.close()
+    att = st1["attempts"] - st0["attempts"]
+    kms = st1["kernel_ms_sweep"] - st0["kernel_ms_sweep"]
+    rate = att / (t1 - t0)
+    return {"copies": copies, "walkers": 2048 * copies, "sweeps_per_launch": n, "launches": steps,
+            "attempts_per_s": rate, "kernel_attempts_per_s": att / (kms / 1e3),
+            "lds_GBs": rate * BYTES_PER_ATTEMPT / 1e9, "lds_frac": rate * BYTES_PER_ATTEMPT / 1e9 / LDS_PEAK_GBS}
+
+
+def exact_binned_lng(E_bins, bin_size):
+    """Exact ln g of the L=64 periodic Ising model summed over each bin
+    [E, E + bin_size), from the committed fixture (exact integer DoS);
+    -inf for bins without states."""
+    import numpy as np
+    f = np.load(os.path.join(ROOT, "tests", "golden", "ising_L64_lng.npz"))
+    e, lg = f["E"], f["lng"]
+    out = np.full(len(E_bins), -np.inf)
+    b = np.floor((e - E_bins[0]) / bin_size).astype(np.int64)
+    for k in range(len(E_bins)):
+        m = b == k
+        if m.any():
+            mx = lg[m].max()
+            out[k] = mx + np.log(np.exp(lg[m] - mx).sum())
+    return out
+
+
 def converge(eng, params, a):
     """Time to ln f < a.converge: wall time from the first sweep until the
     slowest window's ln f drops below the threshold (the reference's
-    f < final_increment test, rewl_simulation.hpp:594, 636-671)."""
+    f < final_increment test, rewl_simulation.hpp:594, 636-671); then the
+    stitched, normalised ln g against the exact DoS."""
     per_check = params["sweeps_per_check"] // params["sweeps_per_exchange"]
     eng.synchronize()
     st0 = eng.stats()
@@ -101,14 +159,28 @@ def converge(eng, params, a):
     eng.synchronize()
     t1 = time.perf_counter()
     st1 = eng.stats()
+    import numpy as np
+    eng.finalize()
+    E, lng = eng.stitched(0, states_per_site=2.0)
+    ex = exact_binned_lng(E, params_bin_size(eng))
+    ok = np.isfinite(ex)
+    d = (lng - ex)[ok]
     print(json.dumps({
         "metric": "time to ln f < %g" % a.converge, "value": t1 - t0, "unit": "s", "higher_is_better": False,
         "converged": bool(lnf < a.converge), "checks": checks, "sweeps": st1["sweeps"] - st0["sweeps"],
         "attempts": st1["attempts"] - st0["attempts"],
         "attempts_per_s": (st1["attempts"] - st0["attempts"]) / (t1 - t0),
         "flat_events": st1["flat_events"], "max_lnf": lnf,
-        "config": {"workload": "2D Ising REWL, L=64 periodic, 32 windows x 64 walkers, flatness 0.8, halving"},
+        "lng_max_abs_error_normalised": float(np.max(np.abs(d))),
+        "lng_max_abs_error_anchored": float(np.max(np.abs(d - d[0]))),
+        "lng_bins_compared": int(ok.sum()),
+        "config": {"workload": "2D Ising REWL, L=64 periodic, 32 windows x 64 walkers, flatness 0.8, halving",
+                   "average_mode": params["average_mode"]},
     }), flush=True)
+
+
+def params_bin_size(eng):
+    return eng.systems[0]["bin_size"]
 
 
 def main():
@@ -125,7 +197,8 @@ def main():
 
     systems, params = configs.config2()
     params = dict(params, sweeps_per_check=a.sweeps_per_step, sweeps_per_exchange=a.sweeps_per_step,
-                  walkers_per_group=a.walkers_per_group, producers=a.producers, sweep_kernel=a.sweep_kernel)
+                  walkers_per_group=a.walkers_per_group, producers=a.producers, sweep_kernel=a.sweep_kernel,
+                  average_mode=a.average_mode)
     eng = Engine(systems, params)
     eng.adjust_to_range()
     W = eng.num_walkers
@@ -160,6 +233,8 @@ def main():
     att_per_launch = attempts / max(launches, 1)
     achieved = att_per_launch * BYTES_PER_ATTEMPT / avg_launch_s / 1e9
     lnf = min(w["lnf"] for w in (eng.window_info(k) for k in range(eng.num_windows)))
+    default_launch = a.sweep_kernel in (0, 3) and a.producers == 0 and a.walkers_per_group == 0
+    traffic = MEASURED_TRAFFIC.get(params["sweeps_per_check"]) if default_launch else None
 
     line = {
         "metric": "spin-flip attempts/s; time to ln f=1e-6 (learned warm start vs plain)",
@@ -190,9 +265,13 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": traffic,
             "kernel": "k_sweep_tuned<Ising, L=64> (row-unrolled sweep)" if a.sweep_kernel in (0, 2, 3, 4)
                       else "k_sweep_fast<Ising> (row kernel)",
+            "histogram_tier": "LDS",
+            "lds_peak": LDS_PEAK_GBS,
+            "lds_frac": achieved / LDS_PEAK_GBS,
+            "hbm_GBs_measured": traffic / avg_launch_s / 1e9 if traffic else None,
             "avg_launch_ms": avg_launch_s * 1e3,
             "attempts_per_launch": att_per_launch,
             "bytes_per_attempt": BYTES_PER_ATTEMPT,
@@ -202,6 +281,8 @@ def main():
                   "exchanges_accepted": st1["exchanges_accepted"]},
     }
     eng.close()
+    if a.saturate > 0:
+        line["saturating_batch"] = saturating_batch(systems, params, a.saturate)
     if not a.no_cpu:
         threads = a.cpu_threads or min(16, os.cpu_count() or 1)
         line["cpu_baseline"] = cpu_baseline(systems, params, a.cpu_seconds, threads)

@@ -99,7 +99,19 @@ struct PhiloxExchangeUrbg {
 /* ---------------- histogram_index (histogram_index.hpp:8-31) ------------- */
 struct HistIndex {
     double min, max, bs;
-    size_t operator()(double v) const { return (size_t)((v - min) / bs); }
+    size_t nbins;
+    /* (size_t)((v - min) / bs) where the reference defines it.  The +0.5
+     * range test below truncates toward zero, so with min in (-0.5, 0.5) it
+     * admits v down to -1.5: for bins narrower than that slack the quotient
+     * reaches -1 and the reference's size_t conversion is undefined (its
+     * histogram access lands before the array).  Such a v counts in bin 0,
+     * and a quotient that rounds up to nbins at the top edge in the last bin. */
+    size_t operator()(double v) const {
+        const double q = (v - min) / bs;
+        if (!(q > 0.)) return 0;
+        const size_t b = (size_t)q;
+        return b < nbins ? b : nbins - 1;
+    }
     double bin_min(size_t b) const { return min + bs * (double)b; }
     bool too_low(double v) const { return (int)(v + 0.5) < (int)(min + 0.5); }
     bool too_high(double v) const { return (int)(v + 0.5) >= (int)(max + 0.5); }
@@ -504,7 +516,7 @@ int ora_create(const rewl_system_desc* systems, int32_t nsys, const rewl_params*
         auto geo = construct_windows(S.d.e_min, S.d.e_max, S.d.bin_size, (size_t)S.W, S.d.overlap);
         for (int k = 0; k < S.W; ++k) {
             Window win;
-            win.hi = HistIndex{geo[k].min, geo[k].max, geo[k].bs};
+            win.hi = HistIndex{geo[k].min, geo[k].max, geo[k].bs, geo[k].nbins};
             win.nbins = geo[k].nbins;
             win.exclude.assign(win.nbins, 0);
             if (S.d.exclude_bins) {

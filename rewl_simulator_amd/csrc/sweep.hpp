@@ -84,8 +84,11 @@ __device__ __forceinline__ bool wl_accept_int(int En, int& E, int& cb, double& l
     return acc;
 }
 
-// FP64 variant (field models): bins and range follow histogram_index.hpp:8-31
-// literally.
+// FP64 variant (field models): bins and range follow histogram_index.hpp:8-31,
+// with the bin clamped to the window: the +0.5 range test admits energies down
+// to -1.5 when the window minimum lies in (-0.5, 0.5), where a quotient <= -1
+// makes the reference's size_t conversion undefined; such an energy counts in
+// bin 0 (and a quotient rounding up to nbins at the top edge, in the last bin).
 template <bool REPLAY, bool PACKED>
 __device__ __forceinline__ bool wl_accept_fp(double En, double& E, int& cb, double& lc, double lnf, double emin,
                                              double bs, int plo, int phi, int nbmax, double* s_lng, uint32_t* s_cnt, int lane,
@@ -93,7 +96,7 @@ __device__ __forceinline__ bool wl_accept_fp(double En, double& E, int& cb, doub
                                              int32_t* err, const u32x4& rnd, int p) {
     const int pe = trunc_half(En);
     const bool in = pe >= plo && pe < phi;
-    const int nb = in ? min((int)((En - emin) / bs), nbmax) : cb;
+    const int nb = in ? min(max((int)((En - emin) / bs), 0), nbmax) : cb;
     const double ln = s_lng[nb * kLanes + lane];
     const float d = (float)(ln - lc);
     bool acc = in && (d <= 0.f);

@@ -4,7 +4,7 @@ Pipeline (every step on the GPU):
 
   1. ``generate``: converged ln g curves of random-strain AT realisations
      (the batched configuration: 8 windows x 4 walkers, 512 bins over
-     [E(all-up), E(checkerboard)], visited-support flatness 0.8, ln f halved to
+     [E(all-up), 0), visited-support flatness 0.8, ln f halved to
      1e-6), many realisations per context, stitched and normalised to
      sum g = 4^N.
   2. ``train``: the MLP of include/rewl_mlp.h, inputs (E/N, 1/L, K, sigma),
@@ -35,13 +35,18 @@ STATES_PER_SITE = 4.0
 
 
 def rfat_systems(specs, bins=512, windows=8, replicas=4):
-    """One random-strain AT system per (L, sigma, seed)."""
+    """One random-strain AT system per (L, sigma, seed), over [E(all-up), 0):
+    the reference's default half range, which holds every energy that carries
+    weight at positive temperature.  The upper half ends in the checkerboard
+    states, single configurations a gap away from the rest of the spectrum;
+    once one replica has visited one, visited-support flatness waits for
+    every replica to revisit it in every stage."""
     systems = []
     for L, sigma, seed in specs:
         h = configs.gaussian_strain(L, sigma, seed)
-        e0, e1 = configs.rfat_range(L, h, K=K_AT)
-        bs = (e1 - e0) / (bins - 1)
-        systems.append(configs.at_system(L, e0, e0 + bs * bins, bs, windows, replicas, K=K_AT, field=h, seed=seed))
+        e0, _ = configs.rfat_range(L, h, K=K_AT)
+        bs = (0.0 - e0) / bins
+        systems.append(configs.at_system(L, e0, 0.0, bs, windows, replicas, K=K_AT, field=h, seed=seed))
     return systems
 
 
@@ -51,30 +56,66 @@ def rfat_params(lnf_final=1e-6, sweeps_per_check=250, average_mode=1):
                           sweeps_per_exchange=sweeps_per_check, adjust_mode=1, average_mode=average_mode)
 
 
-def visited_bins(eng, system):
-    """Energies (lower bin edges) that some walker of the system has visited."""
-    seen = set()
+def stitch_visited(windows, e_min, bin_size):
+    """Join window curves on the system's global bin grid, over visited bins
+    only.  `windows` holds (g0, ln g, visited) per window in energy order (g0
+    = the window's first global bin).  Each window is shifted onto the curve
+    so far by the mean difference over the bins both have visited, and takes
+    over from the middle of that common stretch.  The reference's
+    derivative-matching join (rewl_stitch) assumes every bin of the overlap
+    holds states; clean systems with bins finer than their level spacing have
+    empty bins there, whose ln g is no value at all.  Returns (E, ln g,
+    visited); unvisited bins hold NaN."""
+    nb = max(g0 + len(l) for g0, l, _ in windows)
+    lng = np.full(nb, np.nan)
+    vis = np.zeros(nb, bool)
+    for k, (g0, lk, vk) in enumerate(windows):
+        idx = g0 + np.arange(len(lk))
+        if k == 0:
+            lng[idx[vk]] = lk[vk]
+            vis[idx[vk]] = True
+            continue
+        common = vk & vis[idx]
+        if not common.any():
+            raise ValueError(f"window {k} shares no visited bin with the windows below it")
+        shift = float(np.mean(lng[idx[common]] - lk[common]))
+        take = vk & (idx >= int(np.median(idx[common])))
+        lng[idx[take]] = lk[take] + shift
+        vis[idx[take]] = True
+    E = e_min + bin_size * np.arange(nb)
+    return E, lng, vis
+
+
+def normalise(lng, vis, N):
+    """Shift ln g so that the visited bins hold STATES_PER_SITE^N states."""
+    v = lng[vis]
+    m = v.max()
+    return lng - (m + np.log(np.exp(v - m).sum()) - N * np.log(STATES_PER_SITE))
+
+
+def system_curve(eng, system):
+    """Stitched, normalised (E, ln g, visited) of one system after finalize:
+    the replicas of a window are equal then, so replica 0 carries the window."""
+    s = eng.systems[system]
+    parts = []
     for w in eng.windows:
         if w["system"] != system:
             continue
-        R = eng.systems[system]["replicas_per_window"]
         v = np.zeros(w["num_bins"], dtype=np.uint64)
-        for r in range(R):
+        for r in range(s["replicas_per_window"]):
             v += eng.visits(w["first_walker"] + r) + eng.counts(w["first_walker"] + r).astype(np.uint64)
-        s = eng.systems[system]
         g0 = int(round((w["e_min"] - s["e_min"]) / s["bin_size"]))
-        E = s["e_min"] + s["bin_size"] * (g0 + np.arange(w["num_bins"]))
-        seen.update(np.round(E[v > 0], 6).tolist())
-    return seen
+        parts.append((g0, eng.logdos(w["first_walker"]), v > 0))
+    E, lng, vis = stitch_visited(parts, s["e_min"], s["bin_size"])
+    return E, normalise(lng, vis, s["L"] ** 2), vis
 
 
 def curves(eng, specs):
-    """(L, sigma, E, ln g, visited) per system, ln g normalised to 4^N."""
+    """(L, sigma, E, ln g, visited) per system, ln g normalised to 4^N over
+    the visited bins."""
     out = []
     for i, (L, sigma, seed) in enumerate(specs):
-        E, lng = eng.stitched(i, states_per_site=STATES_PER_SITE)
-        seen = visited_bins(eng, i)
-        vis = np.array([round(e, 6) in seen for e in E])
+        E, lng, vis = system_curve(eng, i)
         out.append(dict(L=L, sigma=sigma, seed=seed, E=E, lng=lng, visited=vis))
     return out
 

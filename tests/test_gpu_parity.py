@@ -114,6 +114,30 @@ def test_random_field_fp64_path_bit_exact(gpu, model, rng):
         assert_same(eng, ora, f"after {chunk} sweeps", cursors=(rng == abi.RNG_REPLAY))
 
 
+def test_random_field_slack_below_window_bit_exact(gpu):
+    """Random-strain AT with a window starting at -0.34 and bins of width
+    1.002: proposals in the +0.5 range test's slack below it (quotient below
+    -1) count in bin 0 on both sides."""
+    from rewl_simulator_amd import warmstart
+    print("case: random-strain AT L=8, slack below window 7", flush=True)
+    h = configs.gaussian_strain(8, 0.5, 50002)
+    e0, e1 = configs.rfat_range(8, h, K=0.5)
+    bs = (e1 - e0) / 511
+    systems = [configs.at_system(8, e0, e0 + bs * 512, bs, 8, 4, K=0.5, field=h, seed=50002)]
+    assert -0.5 < Oracle(systems, warmstart.rfat_params(1e-4)).window_geometry(0, 7)[0] < 0
+    p = warmstart.rfat_params(1e-4)
+    eng, ora = make_pair(systems, p)
+    eng.adjust_to_range(); ora.adjust_to_range()
+    for chunk in (50, 250):
+        eng.sweeps(chunk); ora.sweeps(chunk)
+        assert_same(eng, ora, f"after {chunk} sweeps", cursors=False)
+    for _ in range(10):
+        eng.sweeps(250); eng.exchange(); eng.check()
+        ora.sweeps(250); ora.exchange(); ora.check()
+    assert_same(eng, ora, "after 10 check cycles", cursors=False)
+    window_states_equal(eng, ora)
+
+
 @pytest.mark.parametrize("rng", [abi.RNG_REPLAY, abi.RNG_PHILOX])
 @pytest.mark.parametrize("replicas", [1, 2, 5])
 def test_rewl_cycles_bit_exact(gpu, rng, replicas):

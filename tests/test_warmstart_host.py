@@ -1,0 +1,65 @@
+"""Host logic of the warm-start pipeline (no GPU): the visited-bin join of
+window curves, normalisation, and the network's input features."""
+import numpy as np
+import pytest
+
+from rewl_simulator_amd import warmstart as W
+from rewl_simulator_amd.mlp import features
+
+
+def _windows_of(curve, spans, shifts, holes):
+    """Cut a known curve into overlapping windows, each carrying its own
+    additive constant (as WL ln g does) and unvisited bins."""
+    out = []
+    for (a, b), c in zip(spans, shifts):
+        lk = curve[a:b] + c
+        vk = np.ones(b - a, bool)
+        vk[[h - a for h in holes if a <= h < b]] = False
+        lk = np.where(vk, lk, -1e4)   # an unvisited bin's ln g is no value at all
+        out.append((a, lk, vk))
+    return out
+
+
+@pytest.mark.parametrize("holes", [[], [5, 6, 40, 41, 42, 77], list(range(1, 120, 3))])
+def test_stitch_recovers_curve_up_to_a_constant(holes):
+    x = np.linspace(0, 1, 120)
+    curve = 300 * x * (1 - 0.3 * x) + 5 * np.sin(7 * x)
+    spans = [(0, 40), (25, 70), (55, 100), (85, 120)]
+    shifts = [0.0, -812.25, 33.5, 1e3]
+    E, lng, vis = W.stitch_visited(_windows_of(curve, spans, shifts, holes), -10.0, 0.5)
+    expect = np.ones(120, bool)
+    expect[holes] = False
+    np.testing.assert_array_equal(vis, expect)
+    d = (lng - curve)[vis]
+    assert np.max(np.abs(d - d[0])) < 1e-9
+    assert np.all(np.isnan(lng[~vis]))
+    np.testing.assert_allclose(E, -10.0 + 0.5 * np.arange(120))
+
+
+def test_stitch_refuses_disjoint_windows():
+    curve = np.arange(20.0)
+    ws = [(0, curve[:10], np.ones(10, bool)), (10, curve[10:], np.ones(10, bool))]
+    with pytest.raises(ValueError):
+        W.stitch_visited(ws, 0.0, 1.0)
+
+
+def test_normalise_counts_states_per_site():
+    N = 9
+    lng = np.array([0.0, 2.0, np.nan, 5.0, 1.0])
+    vis = ~np.isnan(lng)
+    out = W.normalise(lng, vis, N)
+    total = np.log(np.exp(out[vis]).sum())
+    assert total == pytest.approx(N * np.log(W.STATES_PER_SITE), rel=1e-12)
+    np.testing.assert_allclose(np.diff(out[[0, 1, 3, 4]]), np.diff(lng[[0, 1, 3, 4]]))
+
+
+def test_features_layout():
+    f = features(np.array([-64.0, 0.0]), 64, 8, 0.5, 0.25)
+    np.testing.assert_array_equal(f, [[-1.0, 0.125, 0.5, 0.25], [0.0, 0.125, 0.5, 0.25]])
+
+
+def test_rfat_systems_half_range():
+    s = W.rfat_systems([(8, 0.5, 3)], bins=512)[0]
+    assert s["e_max"] == 0.0
+    assert s["bin_size"] == pytest.approx(-s["e_min"] / 512)
+    assert s["num_windows"] == 8 and s["replicas_per_window"] == 4
