@@ -53,6 +53,8 @@ def parse():
     ap.add_argument("--max-checks", type=int, default=200000)
     ap.add_argument("--average-mode", type=int, default=0,
                     help="replica averaging: 0 count-weighted (the reference), 1 equal weights")
+    ap.add_argument("--one-over-t", action="store_true",
+                    help="the reference's ONE_OVER_T_ALGORITHM ln f schedule instead of halving")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="only run a plain device-to-device copy of BYTES (counter calibration) and exit")
     return ap.parse_args()
@@ -174,8 +176,8 @@ def converge(eng, params, a):
         "lng_max_abs_error_normalised": float(np.max(np.abs(d))),
         "lng_max_abs_error_anchored": float(np.max(np.abs(d - d[0]))),
         "lng_bins_compared": int(ok.sum()),
-        "config": {"workload": "2D Ising REWL, L=64 periodic, 32 windows x 64 walkers, flatness 0.8, halving",
-                   "average_mode": params["average_mode"]},
+        "config": {"workload": "2D Ising REWL, L=64 periodic, 32 windows x 64 walkers, flatness 0.8",
+                   "schedule": "1/t" if a.one_over_t else "halving", "average_mode": params["average_mode"]},
     }), flush=True)
 
 
@@ -199,6 +201,9 @@ def main():
     params = dict(params, sweeps_per_check=a.sweeps_per_step, sweeps_per_exchange=a.sweeps_per_step,
                   walkers_per_group=a.walkers_per_group, producers=a.producers, sweep_kernel=a.sweep_kernel,
                   average_mode=a.average_mode)
+    if a.one_over_t:
+        from rewl_simulator_amd import abi
+        params["schedule"] = abi.SCHED_ONE_OVER_T
     eng = Engine(systems, params)
     eng.adjust_to_range()
     W = eng.num_walkers

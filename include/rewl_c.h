@@ -53,6 +53,13 @@ extern "C" {
 #define REWL_ERR_REPLAY        -6   /* replay stream exhausted                   */
 #define REWL_ERR_RANGE         -7   /* a walker could not be brought into range  */
 
+/* observables per bin (rewl_params.observables): Ising |m|, m^2, m^4; Ashkin-
+ * Teller sigma |m|, m^2, m^4, tau |m|, m^2, m^4, phi, phi^2, phi^4, Baxter |m|,
+ * m^2, m^4, in the reference's enum order without the correlator entries
+ * (ising2d_observables.hpp:20-27, ashkin_teller2d_observables.hpp) */
+#define REWL_NUM_OBS_ISING      3
+#define REWL_NUM_OBS_AT        12
+
 /* ---- enumerations ----------------------------------------------------- */
 enum rewl_model {
     REWL_ISING2D = 0,            /* Ising2d (ising2d_hamiltonian.hpp); per-site
@@ -138,7 +145,14 @@ typedef struct rewl_params {
                                     next slot computed for both outcomes; 4 = that
                                     and the next ln g read ahead (2-4 are an error
                                     for other L or producers < 0)                  */
-    int32_t reserved[3];
+    int32_t observables;         /* 1: per-bin observable sums of every update (each
+                                    attempt and each post-exchange increment) at the
+                                    walker's current bin: |m|, m^2, m^4 (Ising,
+                                    ising2d_hamiltonian.hpp:234-256) or the 12 AT
+                                    observables (ashkin_teller2d_hamiltonian.hpp:
+                                    267-318); sweeps then run the generic kernel.
+                                    0: off                                          */
+    int32_t reserved[2];
 } rewl_params;
 
 /* Per-window bookkeeping, one record per (system, window). */
@@ -209,6 +223,12 @@ int rewl_get_spins(rewl_ctx* ctx, int32_t walker, int8_t* out, int32_t n);
 int rewl_get_energy(rewl_ctx* ctx, int32_t walker, double* energy);
 int rewl_get_flavor(rewl_ctx* ctx, int32_t walker, int32_t* flavor);
 int rewl_get_rng_cursor(rewl_ctx* ctx, int32_t walker, uint64_t* cursor);
+/* Observable sums of one walker (observables on): sums[b * nobs + o] = sum of
+ * observable o over every update of the walker in bin b.  The reference keeps
+ * running means (update_observable_average, ising2d_observables.hpp:134-145)
+ * and counts_per_bin; here the mean is sums / (visits + counts) of that bin.
+ * nobs is REWL_NUM_OBS_ISING or REWL_NUM_OBS_AT for the walker's model. */
+int rewl_get_observables(rewl_ctx* ctx, int32_t walker, double* sums, int32_t nbins, int32_t nobs);
 
 /* ---- state import (warm start, tests) ------------------------------------- */
 int rewl_set_logdos(rewl_ctx* ctx, int32_t walker, const double* lng, int32_t n);

@@ -42,6 +42,7 @@ def load(path=ORACLE_SO, build_if_missing=True):
         "ora_get_logdos": ([P, C.c_int32, C.POINTER(C.c_double)], C.c_int),
         "ora_get_counts": ([P, C.c_int32, C.POINTER(C.c_uint64)], C.c_int),
         "ora_get_visits": ([P, C.c_int32, C.POINTER(C.c_uint64)], C.c_int),
+        "ora_get_observables": ([P, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_double)], C.c_int),
         "ora_get_spins": ([P, C.c_int32, C.POINTER(C.c_int8)], C.c_int),
         "ora_get_state": ([P, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_int32),
                            C.POINTER(C.c_uint64)], C.c_int),
@@ -163,6 +164,17 @@ class Oracle:
         out = np.empty(n, dtype=np.uint64)
         self.lib.ora_get_visits(self.ctx, int(g), out.ctypes.data_as(C.POINTER(C.c_uint64)))
         return out
+
+    def observables(self, g):
+        """(running means [bin, ob], counts_per_bin [bin]) of walker g."""
+        n = self.layout(g)[3]
+        s = self.systems[self.layout(g)[0]]
+        nobs = 3 if s["model"] == self.abi.ISING2D else 12
+        means = np.empty((n, nobs), dtype=np.float64)
+        counts = np.empty(n, dtype=np.float64)
+        self._chk(self.lib.ora_get_observables(self.ctx, int(g), means.ctypes.data_as(C.POINTER(C.c_double)),
+                                               counts.ctypes.data_as(C.POINTER(C.c_double))))
+        return means, counts
 
     def spins(self, g):
         s = self.systems[self.layout(g)[0]]

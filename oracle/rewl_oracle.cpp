@@ -155,6 +155,8 @@ struct Walker {
     std::uniform_real_distribution<double> ureal{0., 1.};
     uint64_t exch_cursor = 0;
     uint64_t replay_draws = 0;             /* engine outputs consumed (replay) */
+    std::vector<double> obs;               /* running means [bin][ob] (observables on) */
+    std::vector<double> obs_counts;        /* counts_per_bin, as the reference keeps it */
 };
 
 struct Window {
@@ -291,6 +293,47 @@ inline double init_uniform(ora_ctx& c, const System& S, Walker& w) {
     return canonical_from_u64((uint64_t)o.v[0] | ((uint64_t)o.v[1] << 32));
 }
 
+/* Ising2d_Obs / Ashkin_Teller2d_Obs::update_observable_average
+ * (ising2d_observables.hpp:134-145): avg = (value + counts avg) / (counts + 1). */
+inline void obs_average(Walker& w, size_t bin, int nobs, int ob, double value) {
+    double& avg = w.obs[bin * (size_t)nobs + (size_t)ob];
+    const double counts = w.obs_counts[bin];
+    avg = (value + counts * avg) / (counts + 1);
+}
+
+/* update_observables at the walker's current bin: Ising
+ * ising2d_hamiltonian.hpp:234-256 (|m|, m^2, m^4), Ashkin-Teller
+ * ashkin_teller2d_hamiltonian.hpp:267-318 (sigma, tau, phi, Baxter: |x|, x^2,
+ * x^4 with phi = sqrt(sigma^2 + tau^2)); then counts_per_bin += 1. */
+inline void update_observables(const System& S, Walker& w) {
+    if (w.obs.empty()) return;
+    const size_t b = w.hi(w.E);
+    const double divide_N = 1. / (double)S.N;
+    if (S.d.model == REWL_ISING2D) {
+        const double m = divide_N * std::fabs(w.M);
+        obs_average(w, b, REWL_NUM_OBS_ISING, 0, m);
+        obs_average(w, b, REWL_NUM_OBS_ISING, 1, m * m);
+        obs_average(w, b, REWL_NUM_OBS_ISING, 2, m * m * m * m);
+    } else {
+        const double sv = divide_N * std::fabs(w.Ms), tv = divide_N * std::fabs(w.Mt), bv = divide_N * std::fabs(w.Bax);
+        const double phi2 = sv * sv + tv * tv;
+        const int n = REWL_NUM_OBS_AT;
+        obs_average(w, b, n, 0, sv);
+        obs_average(w, b, n, 1, sv * sv);
+        obs_average(w, b, n, 2, sv * sv * sv * sv);
+        obs_average(w, b, n, 3, tv);
+        obs_average(w, b, n, 4, tv * tv);
+        obs_average(w, b, n, 5, tv * tv * tv * tv);
+        obs_average(w, b, n, 6, std::sqrt(phi2));
+        obs_average(w, b, n, 7, phi2);
+        obs_average(w, b, n, 8, phi2 * phi2);
+        obs_average(w, b, n, 9, bv);
+        obs_average(w, b, n, 10, bv * bv);
+        obs_average(w, b, n, 11, bv * bv * bv * bv);
+    }
+    w.obs_counts[b] += 1.;
+}
+
 /* ---------------- Wang-Landau update and sweep (wang_landau.hpp:49-120) ---- */
 struct SweepDraws {           /* Philox positional stream of one walker, one sweep */
     uint32_t k0, k1, walker;
@@ -322,6 +365,7 @@ inline void wl_update(const System& S, Walker& w, const Window& win, int i, bool
     }
     w.cnt[cur] += 1;
     w.lng[cur] += win.lnf;
+    update_observables(S, w);
 }
 
 void sweep_walker(ora_ctx& c, System& S, Walker& w, uint64_t sweep_index) {
@@ -434,6 +478,7 @@ inline void update_histograms(System& S, Walker& w) {
     w.lng[b] += S.windows[w.window].lnf;
 }
 
+
 void swap_states(Walker& a, Walker& b) {
     std::swap(a.E, b.E); std::swap(a.M, b.M); std::swap(a.Ms, b.Ms);
     std::swap(a.Mt, b.Mt); std::swap(a.Bax, b.Bax);
@@ -482,7 +527,9 @@ void exchange_system(ora_ctx& c, System& S, bool even) {
             }
             if (accept) { swap_states(a, b); ++c.exchanges_accepted; }
             update_histograms(S, a);
+            update_observables(S, a);
             update_histograms(S, b);
+            update_observables(S, b);
         }
     }
 }
@@ -536,6 +583,11 @@ int ora_create(const rewl_system_desc* systems, int32_t nsys, const rewl_params*
                 w.lng.assign(S.windows[k].nbins, 0.0);
                 w.cnt.assign(S.windows[k].nbins, 0);
                 w.visits.assign(S.windows[k].nbins, 0);
+                if (c->p.observables) {
+                    const int nobs = S.d.model == REWL_ISING2D ? REWL_NUM_OBS_ISING : REWL_NUM_OBS_AT;
+                    w.obs.assign(S.windows[k].nbins * (size_t)nobs, 0.0);
+                    w.obs_counts.assign(S.windows[k].nbins, 0.0);
+                }
                 w.gen.seed(S.d.walker_seeds ? S.d.walker_seeds[w.local] : (uint64_t)1);
                 recalculate_state(S, w);
                 c->w.push_back(std::move(w));
@@ -664,6 +716,16 @@ int ora_get_visits(const ora_ctx* c, int32_t walker, uint64_t* out) {
     for (size_t b = 0; b < w.visits.size(); ++b) out[b] = w.visits[b];
     return REWL_OK;
 }
+/* running means [bin][ob] and counts_per_bin of one walker (observables on) */
+int ora_get_observables(const ora_ctx* c, int32_t walker, double* means, double* counts) {
+    if (!c || walker < 0 || walker >= (int32_t)c->w.size() || !means || !counts) return fail(REWL_ERR_INVALID, "bad argument");
+    const Walker& w = c->w[walker];
+    if (w.obs.empty()) return fail(REWL_ERR_INVALID, "observables are off");
+    std::copy(w.obs.begin(), w.obs.end(), means);
+    std::copy(w.obs_counts.begin(), w.obs_counts.end(), counts);
+    return REWL_OK;
+}
+
 int ora_get_spins(const ora_ctx* c, int32_t walker, int8_t* out) {
     const Walker& w = c->w[walker];
     for (size_t i = 0; i < w.s.size(); ++i) out[i] = w.s[i] > 0 ? 1 : -1;

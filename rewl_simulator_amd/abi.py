@@ -70,7 +70,8 @@ class Params(C.Structure):
         ("adjust_mode", C.c_int32),
         ("average_mode", C.c_int32),
         ("sweep_kernel", C.c_int32),
-        ("reserved", C.c_int32 * 3),
+        ("observables", C.c_int32),
+        ("reserved", C.c_int32 * 2),
     ]
 
 
@@ -125,6 +126,8 @@ class MlpTrainParams(C.Structure):
 
 
 MLP_NUM_PARAMS = 33793
+NUM_OBS_ISING = 3
+NUM_OBS_AT = 12
 
 SIGNATURES = {
     "rewl_last_error": ([], C.c_char_p),
@@ -150,6 +153,7 @@ SIGNATURES = {
     "rewl_get_energy": ([_P, C.c_int32, C.POINTER(C.c_double)], C.c_int),
     "rewl_get_flavor": ([_P, C.c_int32, C.POINTER(C.c_int32)], C.c_int),
     "rewl_get_rng_cursor": ([_P, C.c_int32, C.POINTER(C.c_uint64)], C.c_int),
+    "rewl_get_observables": ([_P, C.c_int32, C.POINTER(C.c_double), C.c_int32, C.c_int32], C.c_int),
     "rewl_set_logdos": ([_P, C.c_int32, C.POINTER(C.c_double), C.c_int32], C.c_int),
     "rewl_set_window_lnf": ([_P, C.c_int32, C.c_double], C.c_int),
     "rewl_set_spins": ([_P, C.c_int32, C.POINTER(C.c_int8), C.c_int32], C.c_int),

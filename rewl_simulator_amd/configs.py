@@ -29,6 +29,7 @@ REFERENCE_PARAMS = dict(
     adjust_mode=0,
     average_mode=0,
     sweep_kernel=0,
+    observables=0,
 )
 
 DEFAULT_SEED = 20240501

@@ -241,6 +241,13 @@ __global__ void __launch_bounds__(kLanes) k_exchange(KArgs a, const int32_t* pai
         a.lng[ia] += fl;
         a.cnt[ib] += 1u;
         a.lng[ib] += fu;
+        if (a.obs) {
+            int m[3];
+            lattice_magnetisations(a.lat + a.w_lat[ga], S.L, S.model, m);
+            update_observables(a.obs + ia * a.nobs, S.model, S.N, m);
+            lattice_magnetisations(a.lat + a.w_lat[gb], S.L, S.model, m);
+            update_observables(a.obs + ib * a.nobs, S.model, S.N, m);
+        }
     }
 }
 

@@ -72,6 +72,8 @@ struct KArgs {
     int32_t* err;
     unsigned long long* exch_accepted;
     const void* bound_tab;     // acceptance-bound table (accept_bound.hpp), kBoundTabN entries
+    double* obs;               // observable sums [(hist index) * nobs + ob], or null
+    int32_t nobs;              // observables per bin slot (3 Ising, 12 with any AT system)
     int32_t num_walkers, num_windows, num_groups;
 };
 
@@ -129,6 +131,59 @@ __device__ __forceinline__ double at_local_energy_mem(const DevSystem& S, const 
         en += S.K * sg * ta * sn * tn;
     }
     return -en;
+}
+
+// Magnetisations of a walker's lattice (bit 1 = spin -1): Ising m[0] = M;
+// Ashkin-Teller m[0] = sum sigma, m[1] = sum tau, m[2] = sum sigma tau.
+__device__ __forceinline__ void lattice_magnetisations(const uint64_t* lat, int L, int model, int m[3]) {
+    int ps = 0, pt = 0, px = 0;
+    for (int y = 0; y < L; ++y) {
+        const uint64_t s = lat[(int64_t)y * kLanes];
+        ps += __popcll(s);
+        if (model == 1) {
+            const uint64_t t = lat[(int64_t)(L + y) * kLanes];
+            pt += __popcll(t);
+            px += __popcll(s ^ t);
+        }
+    }
+    const int N = L * L;
+    m[0] = N - 2 * ps;
+    m[1] = N - 2 * pt;
+    m[2] = N - 2 * px;
+}
+
+// update_observables at bin b (ising2d_hamiltonian.hpp:234-256,
+// ashkin_teller2d_hamiltonian.hpp:267-318): the observable values of the
+// current state, added to the walker's per-bin sums (walker-private, so the
+// adds need no ordering; the no-return atomic keeps them off the walker's
+// dependency chain).
+__device__ __forceinline__ void obs_add(double* p, double v) {
+    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void update_observables(double* obs, int model, int N, const int m[3]) {
+    const double divide_N = 1. / (double)N;
+    if (model == 0) {
+        const double v = divide_N * fabs((double)m[0]);
+        obs_add(obs + 0, v);
+        obs_add(obs + 1, v * v);
+        obs_add(obs + 2, v * v * v * v);
+    } else {
+        const double sv = divide_N * fabs((double)m[0]), tv = divide_N * fabs((double)m[1]);
+        const double bv = divide_N * fabs((double)m[2]);
+        const double phi2 = sv * sv + tv * tv;
+        obs_add(obs + 0, sv);
+        obs_add(obs + 1, sv * sv);
+        obs_add(obs + 2, sv * sv * sv * sv);
+        obs_add(obs + 3, tv);
+        obs_add(obs + 4, tv * tv);
+        obs_add(obs + 5, tv * tv * tv * tv);
+        obs_add(obs + 6, sqrt(phi2));
+        obs_add(obs + 7, phi2);
+        obs_add(obs + 8, phi2 * phi2);
+        obs_add(obs + 9, bv);
+        obs_add(obs + 10, bv * bv);
+        obs_add(obs + 11, bv * bv * bv * bv);
+    }
 }
 
 }  // namespace rewl

@@ -22,6 +22,7 @@
 #include "aux_kernels.hpp"
 #include "sweep.hpp"
 #include "sweep_fast.hpp"
+#include "sweep_field.hpp"
 #include "sweep_tuned.hpp"
 
 using namespace rewl;
@@ -128,6 +129,8 @@ struct rewl_ctx {
     bool class_simple[4] = {true, true, true, true};   // plain [min, max) range test is exact
     int class_maxlanes[4] = {0, 0, 0, 0};
     int class_maxL[4] = {0, 0, 0, 0};
+    bool class_field[4] = {false, false, false, false};   // FP64 class swept by k_sweep_field
+    size_t class_field_lds[4] = {0, 0, 0, 0};             // its largest group's LDS
     int nprod = 1;                                      // producer wavefronts of the fast path
     DevBuf<int32_t> d_class_groups[4];
     // row-unrolled kernel: group lists per (model, lattice size kTunedL[i])
@@ -154,6 +157,8 @@ struct rewl_ctx {
     DevBuf<uint32_t> d_cnt;
     DevBuf<unsigned long long> d_exch;
     DevBuf<BoundEntry> d_bound_tab;
+    DevBuf<double> d_obs;       // observable sums (observables on)
+    int nobs = 0;
 
     // replay streams
     std::vector<std::mt19937_64> engines;
@@ -188,7 +193,7 @@ void destroy_ctx(rewl_ctx* c) {
     c->d_w_hist.release(); c->d_w_lat.release(); c->d_win_iter.release(); c->d_win_sweeps.release();
     c->d_w_E.release(); c->d_win_lnf.release(); c->d_fields.release();
     c->d_w_cursor.release(); c->d_lat.release(); c->d_visits.release(); c->d_ring.release(); c->d_ring_filled.release();
-    c->d_exclude.release(); c->d_lng.release(); c->d_cnt.release(); c->d_exch.release(); c->d_bound_tab.release();
+    c->d_exclude.release(); c->d_lng.release(); c->d_cnt.release(); c->d_exch.release(); c->d_bound_tab.release(); c->d_obs.release();
     c->d_sys.release(); c->d_win.release(); c->d_grp.release();
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -279,6 +284,31 @@ int launch_sweep_fast(rewl_ctx* c, int cls, int nsweeps) {
     return REWL_OK;
 }
 
+// Philox mode, FP64 energies (field models): sweep_field.hpp.
+constexpr int kFieldProducers = 3;
+
+// LDS of one k_sweep_field group: ln g and counts, the field, the bound ring.
+size_t field_group_lds(int nbins, int nwalkers, int N, int L) {
+    return (((size_t)nbins * nwalkers * 12 + 15) & ~(size_t)15) + (size_t)N * 8 + 2 * (size_t)L * nwalkers * 8;
+}
+
+size_t field_lds(const rewl_ctx* c, int cls) { return c->class_field_lds[cls]; }
+
+template <int MODEL>
+int launch_sweep_field(rewl_ctx* c, int cls, int nsweeps) {
+    const int ng = (int)c->class_groups[cls].size();
+    const size_t lds = field_lds(c, cls);
+    const void* kern = (const void*)k_sweep_field<MODEL, kFieldProducers>;
+    HIP_TRY(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    KArgs args = c->args;
+    const int32_t* gl = c->d_class_groups[cls].p;
+    uint64_t base = c->sweep_index;
+    void* kargs[] = {&args, (void*)&gl, (void*)&ng, (void*)&nsweeps, (void*)&base};
+    HIP_TRY(hipLaunchKernel(kern, dim3((unsigned)ng), dim3(kLanes * (1 + kFieldProducers)), kargs, lds, c->stream));
+    HIP_TRY(hipGetLastError());
+    return REWL_OK;
+}
+
 // Philox mode, integer path, L in kTunedL: the row-unrolled kernel (sweep_tuned.hpp).
 template <int MODEL, int L>
 int launch_sweep_tuned(rewl_ctx* c, int li, int nsweeps) {
@@ -323,7 +353,9 @@ template <int MODEL, bool INTP>
 int launch_sweep_class(rewl_ctx* c, int cls, int nsweeps) {
     const int ng = (int)c->class_groups[cls].size();
     if (ng == 0) return REWL_OK;
-    if (INTP && c->p.rng_mode == REWL_RNG_PHILOX) return launch_sweep_fast<MODEL>(c, cls, nsweeps);
+    const bool obs = c->p.observables != 0;
+    if (INTP && c->p.rng_mode == REWL_RNG_PHILOX && !obs) return launch_sweep_fast<MODEL>(c, cls, nsweeps);
+    if (!INTP && c->class_field[cls]) return launch_sweep_field<MODEL>(c, cls, nsweeps);
     const bool packed = c->class_packed[cls];
     const int nb = c->class_maxbins[cls];
     const size_t lds = (size_t)nb * kLanes * sizeof(double) +
@@ -333,8 +365,13 @@ int launch_sweep_class(rewl_ctx* c, int cls, int nsweeps) {
     const int32_t* gl = c->d_class_groups[cls].p;
     const int fe = c->flush_every;
     void (*kern)(KArgs, const int32_t*, int, int, uint64_t, int);
-    if (replay) kern = packed ? k_sweep<MODEL, INTP, true, true> : k_sweep<MODEL, INTP, true, false>;
-    else kern = packed ? k_sweep<MODEL, INTP, false, true> : k_sweep<MODEL, INTP, false, false>;
+    if (obs) {
+        if (replay) kern = packed ? k_sweep<MODEL, INTP, true, true, true> : k_sweep<MODEL, INTP, true, false, true>;
+        else kern = packed ? k_sweep<MODEL, INTP, false, true, true> : k_sweep<MODEL, INTP, false, false, true>;
+    } else {
+        if (replay) kern = packed ? k_sweep<MODEL, INTP, true, true, false> : k_sweep<MODEL, INTP, true, false, false>;
+        else kern = packed ? k_sweep<MODEL, INTP, false, true, false> : k_sweep<MODEL, INTP, false, false, false>;
+    }
     HIP_TRY(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     hipLaunchKernelGGL(kern, grid, block, lds, c->stream, c->args, gl, ng, nsweeps, c->sweep_index, fe);
     HIP_TRY(hipGetLastError());
@@ -474,7 +511,12 @@ int rewl_create(const rewl_system_desc* systems, int32_t num_systems, const rewl
         // The throughput kernel keeps FP64 ln g and u32 counts of every walker
         // of the group in LDS with the bin stride equal to the group size, so
         // a window of many bins gets fewer walkers per wavefront.
-        const bool fast = S.intpath && P.rng_mode == REWL_RNG_PHILOX;
+        // observables on: every class runs k_sweep, which tracks the magnetisations
+        const bool obs = P.observables != 0;
+        const bool fast = S.intpath && P.rng_mode == REWL_RNG_PHILOX && !obs;
+        // field models in Philox mode: k_sweep_field (sweep_kernel 1 or producers < 0 keep k_sweep)
+        const bool field_fast = !S.intpath && P.rng_mode == REWL_RNG_PHILOX && P.producers >= 0 &&
+                                P.sweep_kernel != 1 && !obs;
         c->nprod = P.producers < 0 ? 0 : (P.producers == 0 ? 1 : std::min<int>(P.producers, 3));
         c->tuned_nprod = P.producers == 0 ? 7 : std::min<int>(std::max<int>(P.producers, 1), 7);
         const int li = tuned_index(S.d.L);
@@ -495,6 +537,10 @@ int rewl_create(const rewl_system_desc* systems, int32_t num_systems, const rewl
         else if (fast)
             wpg = std::min(kLanes, (int)((size_t)c->lds_max /
                                          ((size_t)(maxbins + 1) * 12 + (c->nprod ? 8 * (size_t)S.d.L : 0))));
+        else if (field_fast) {
+            const long long room = (long long)c->lds_max - 16 - 8LL * S.N;
+            wpg = room <= 0 ? 0 : std::min<long long>(kLanes, room / (12LL * maxbins + 16LL * S.d.L));
+        }
         if (P.walkers_per_group > 0) wpg = std::min(wpg, (int)P.walkers_per_group);
         if (wpg < 1) return bail(fail(REWL_ERR_INVALID, "window too wide for LDS: " + std::to_string(maxbins) + " bins"));
         {
@@ -517,6 +563,7 @@ int rewl_create(const rewl_system_desc* systems, int32_t num_systems, const rewl
                 if (zero_bound && m1_reachable) c->class_simple[cls] = false;
                 c->class_maxlanes[cls] = std::max(c->class_maxlanes[cls], wpg);
                 c->class_maxL[cls] = std::max(c->class_maxL[cls], S.d.L);
+                if (field_fast) c->class_field[cls] = true;
             }
         }
         for (int off = 0; off < nw; off += wpg) {
@@ -546,6 +593,8 @@ int rewl_create(const rewl_system_desc* systems, int32_t num_systems, const rewl
                 const int cls = S.d.model * 2 + (S.intpath ? 1 : 0);
                 c->class_groups[cls].push_back(gid);
                 c->class_maxbins[cls] = std::max(c->class_maxbins[cls], nb);
+                if (field_fast)
+                    c->class_field_lds[cls] = std::max(c->class_field_lds[cls], field_group_lds(nb, G.nwalkers, S.N, S.d.L));
             }
             c->groups.push_back(G);
         }
@@ -555,7 +604,12 @@ int rewl_create(const rewl_system_desc* systems, int32_t num_systems, const rewl
         const size_t full = (size_t)nb * kLanes * 12;
         const size_t packed = (size_t)nb * kLanes * 8 + (size_t)((nb + 1) / 2) * kLanes * 4;
         if (c->class_groups[cls].empty()) continue;
-        if ((cls & 1) && P.rng_mode == REWL_RNG_PHILOX) {
+        if (c->class_field[cls]) {
+            if (field_lds(c, cls) > (size_t)c->lds_max)
+                return bail(fail(REWL_ERR_INVALID, "field systems of this batch do not fit LDS together"));
+            continue;
+        }
+        if ((cls & 1) && P.rng_mode == REWL_RNG_PHILOX && !P.observables) {
             // throughput kernel: u32 counts, stride = group size (checked above)
             if ((size_t)nb * c->class_maxlanes[cls] * 12 +
                     (c->nprod ? 8 * (size_t)c->class_maxL[cls] * c->class_maxlanes[cls] : 0) > (size_t)c->lds_max)
@@ -661,6 +715,15 @@ int rewl_create(const rewl_system_desc* systems, int32_t num_systems, const rewl
         hipMemsetAsync(c->d_cnt.p, 0, c->hist_total * 4, c->stream) != hipSuccess ||
         hipMemsetAsync(c->d_visits.p, 0, c->hist_total * 8, c->stream) != hipSuccess)
         return bail(fail(REWL_ERR_DEVICE, "hipMemset failed"));
+    if (P.observables) {
+        bool any_at = false;
+        for (const auto& S : c->sys) any_at = any_at || S.d.model == REWL_ASHKIN_TELLER2D;
+        c->nobs = any_at ? REWL_NUM_OBS_AT : REWL_NUM_OBS_ISING;
+        const size_t n = c->hist_total * (size_t)c->nobs;
+        if (c->d_obs.alloc(n) != hipSuccess) return bail(fail(REWL_ERR_DEVICE, "hipMalloc failed for observables"));
+        if (hipMemsetAsync(c->d_obs.p, 0, n * 8, c->stream) != hipSuccess)
+            return bail(fail(REWL_ERR_DEVICE, "hipMemset failed"));
+    }
 
     // ---- replay streams ---------------------------------------------------------
     if (P.rng_mode == REWL_RNG_REPLAY) {
@@ -694,6 +757,7 @@ int rewl_create(const rewl_system_desc* systems, int32_t num_systems, const rewl
     a.win_lnf = c->d_win_lnf.p; a.win_iter = c->d_win_iter.p; a.win_sweeps = c->d_win_sweeps.p;
     a.win_oot = c->d_win_oot.p; a.win_done = c->d_win_done.p; a.exclude = c->d_exclude.p;
     a.lat = c->d_lat.p; a.lng = c->d_lng.p; a.cnt = c->d_cnt.p; a.visits = c->d_visits.p;
+    a.obs = c->d_obs.p; a.nobs = c->nobs;
     a.ring = c->d_ring.p; a.ring_filled = c->d_ring_filled.p; a.ring_cap = c->ring_cap;
     a.err = c->d_err.p; a.exch_accepted = c->d_exch.p; a.bound_tab = c->d_bound_tab.p;
     a.num_walkers = W; a.num_windows = (int)nwin; a.num_groups = (int)c->groups.size();
@@ -909,6 +973,21 @@ int rewl_get_visits(rewl_ctx* c, int32_t g, uint64_t* out, int32_t n) {
     if (n < nb) return fail(REWL_ERR_INVALID, "output buffer too small");
     HIP_TRY(hipMemcpy2DAsync(out, sizeof(uint64_t), c->d_visits.p + c->w_hist[g], kLanes * sizeof(uint64_t),
                              sizeof(uint64_t), (size_t)nb, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return REWL_OK;
+}
+
+int rewl_get_observables(rewl_ctx* c, int32_t g, double* sums, int32_t n, int32_t nobs) {
+    if (!walker_ok(c, g) || !sums) return fail(REWL_ERR_INVALID, "bad walker");
+    if (!c->d_obs.p) return fail(REWL_ERR_INVALID, "observables are off (rewl_params.observables)");
+    const HostSystem& S = c->sys[c->win[c->w_window[g]].sys];
+    const int want = S.d.model == REWL_ISING2D ? REWL_NUM_OBS_ISING : REWL_NUM_OBS_AT;
+    if (nobs != want) return fail(REWL_ERR_INVALID, "nobs does not match the walker's model");
+    const int nb = c->win[c->w_window[g]].nbins;
+    if (n < nb) return fail(REWL_ERR_INVALID, "output buffer too small");
+    HIP_TRY(hipMemcpy2DAsync(sums, (size_t)nobs * sizeof(double), c->d_obs.p + c->w_hist[g] * c->nobs,
+                             (size_t)kLanes * c->nobs * sizeof(double), (size_t)nobs * sizeof(double), (size_t)nb,
+                             hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     return REWL_OK;
 }

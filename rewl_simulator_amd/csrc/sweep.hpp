@@ -114,8 +114,10 @@ __device__ __forceinline__ bool wl_accept_fp(double En, double& E, int& cb, doub
 __device__ __forceinline__ uint32_t bit(uint64_t w, int x) { return (uint32_t)(w >> x) & 1u; }
 __device__ __forceinline__ double pm(uint32_t b) { return b ? -1.0 : 1.0; }
 
-// MODEL: 0 Ising2d, 1 Ashkin-Teller2d.  INTP: integer energies.
-template <int MODEL, bool INTP, bool REPLAY, bool PACKED>
+
+// MODEL: 0 Ising2d, 1 Ashkin-Teller2d.  INTP: integer energies.  OBS: per-bin
+// observable sums (the walker tracks its magnetisations).
+template <int MODEL, bool INTP, bool REPLAY, bool PACKED, bool OBS>
 __device__ void sweep_walker(const KArgs& a, const DevGroup& G, const DevSystem& S, int lane, int nsweeps,
                              uint64_t sweep_base, int flush_every, double* s_lng, uint32_t* s_cnt) {
     const int g = G.first_walker + lane;
@@ -153,6 +155,10 @@ __device__ void sweep_walker(const KArgs& a, const DevGroup& G, const DevSystem&
     }
     double lc = s_lng[cb * kLanes + lane];
     const int pspan = Wn.phi_hi - Wn.phi_lo;
+    int mag[3] = {0, 0, 0};
+    double* obs = OBS ? a.obs + hb * a.nobs : nullptr;
+    const int64_t obs_bin = (int64_t)kLanes * a.nobs;
+    if (OBS) lattice_magnetisations(lat, S.L, MODEL, mag);
     constexpr int NP = MODEL == 0 ? 1 : 2;   // bit planes
     constexpr int NPASS = MODEL == 0 ? 1 : 2;
 
@@ -186,8 +192,11 @@ __device__ void sweep_walker(const KArgs& a, const DevGroup& G, const DevSystem&
                                                 (si ^ bit(dn[0], x));
                             const int En = Ei + S.twoJ * (4 - 2 * (int)na);
                             if (wl_accept_int<REPLAY, PACKED>(En, Ei, cb, lc, lnf, Wn.imin, Wn.phi_lo, pspan, Wn.shift,
-                                                             s_lng, s_cnt, lane, ring, cap, filled, cursor, a.err, rnd, p))
+                                                             s_lng, s_cnt, lane, ring, cap, filled, cursor, a.err, rnd, p)) {
                                 cu[0] ^= (1ull << x);
+                                if (OBS) mag[0] += si ? 2 : -2;
+                            }
+                            if (OBS) update_observables(obs + cb * obs_bin, MODEL, S.N, mag);
                         } else {
                             const int site = y * L + x;
                             const double sv = pm(si);
@@ -199,8 +208,11 @@ __device__ void sweep_walker(const KArgs& a, const DevGroup& G, const DevSystem&
                             const double t = -sv;
                             const double En = Ed + -1. * t * f - -1. * sv * f;
                             if (wl_accept_fp<REPLAY, PACKED>(En, Ed, cb, lc, lnf, Wn.emin, Wn.bs, Wn.phi_lo, Wn.phi_hi,
-                                                            Wn.nbins - 1, s_lng, s_cnt, lane, ring, cap, filled, cursor, a.err, rnd, p))
+                                                            Wn.nbins - 1, s_lng, s_cnt, lane, ring, cap, filled, cursor, a.err, rnd, p)) {
                                 cu[0] ^= (1ull << x);
+                                if (OBS) mag[0] += si ? 2 : -2;
+                            }
+                            if (OBS) update_observables(obs + cb * obs_bin, MODEL, S.N, mag);
                         }
                     } else {
                         const uint32_t sb = bit(cu[0], x), tb = bit(cu[1], x);
@@ -215,7 +227,12 @@ __device__ void sweep_walker(const KArgs& a, const DevGroup& G, const DevSystem&
                             if (wl_accept_int<REPLAY, PACKED>(En, Ei, cb, lc, lnf, Wn.imin, Wn.phi_lo, pspan, Wn.shift,
                                                              s_lng, s_cnt, lane, ring, cap, filled, cursor, a.err, rnd, p)) {
                                 if (fl) cu[1] ^= (1ull << x); else cu[0] ^= (1ull << x);
+                                if (OBS) {
+                                    mag[fl] += (fl ? tb : sb) ? 2 : -2;
+                                    mag[2] += (sb ^ tb) ? 2 : -2;
+                                }
                             }
+                            if (OBS) update_observables(obs + cb * obs_bin, MODEL, S.N, mag);
                         } else {
                             const int site = y * L + x;
                             const double sg = pm(sb), ta = pm(tb);
@@ -240,7 +257,12 @@ __device__ void sweep_walker(const KArgs& a, const DevGroup& G, const DevSystem&
                             if (wl_accept_fp<REPLAY, PACKED>(En, Ed, cb, lc, lnf, Wn.emin, Wn.bs, Wn.phi_lo, Wn.phi_hi,
                                                             Wn.nbins - 1, s_lng, s_cnt, lane, ring, cap, filled, cursor, a.err, rnd, p)) {
                                 if (fl) cu[1] ^= (1ull << x); else cu[0] ^= (1ull << x);
+                                if (OBS) {
+                                    mag[fl] += (fl ? tb : sb) ? 2 : -2;
+                                    mag[2] += (sb ^ tb) ? 2 : -2;
+                                }
                             }
+                            if (OBS) update_observables(obs + cb * obs_bin, MODEL, S.N, mag);
                         }
                     }
                 }
@@ -269,7 +291,7 @@ __device__ void sweep_walker(const KArgs& a, const DevGroup& G, const DevSystem&
     if (REPLAY) a.w_cursor[g] = cursor;
 }
 
-template <int MODEL, bool INTP, bool REPLAY, bool PACKED>
+template <int MODEL, bool INTP, bool REPLAY, bool PACKED, bool OBS>
 __global__ void __launch_bounds__(kLanes) k_sweep(KArgs a, const int32_t* groups, int ngroups, int nsweeps,
                                                   uint64_t sweep_base, int flush_every) {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -280,7 +302,7 @@ __global__ void __launch_bounds__(kLanes) k_sweep(KArgs a, const int32_t* groups
         double* s_lng = reinterpret_cast<double*>(smem);
         uint32_t* s_cnt = reinterpret_cast<uint32_t*>(s_lng + (size_t)G.nbins * kLanes);
         if (lane < G.nwalkers)
-            sweep_walker<MODEL, INTP, REPLAY, PACKED>(a, G, S, lane, nsweeps, sweep_base, flush_every, s_lng, s_cnt);
+            sweep_walker<MODEL, INTP, REPLAY, PACKED, OBS>(a, G, S, lane, nsweeps, sweep_base, flush_every, s_lng, s_cnt);
     }
 }
 

@@ -1,0 +1,227 @@
+// Wang-Landau sweep, Philox mode, FP64 energies (random-field Ising and
+// random-strain Ashkin-Teller): the throughput kernel for field models.
+//
+// Same Markov chain, draws and FP64 arithmetic as k_sweep (and so as the
+// reference's wang_landau_update, wang_landau.hpp:49-93, with the local
+// energies of ising2d_hamiltonian.hpp:162-188 and
+// ashkin_teller2d_hamiltonian.hpp:169-191 summed in the reference's order).
+// What changes is what the walker wavefront waits for:
+//
+//  * the per-site field is staged in LDS once per launch, so its read is an
+//    LDS access issued ahead of its use instead of a global load on the
+//    dependency chain of every attempt;
+//  * acceptance is the single comparison ln g[new] - ln g[cur] < B(k), with B
+//    the FP64 bound of accept_bound.hpp for the attempt's Philox word k
+//    (equivalent to d = (float)(ln g[new] - ln g[cur]) <= 0 or
+//    u < exp(-(double)d) for a correctly rounded exp); producer wavefronts
+//    compute B for every attempt of the next row into a double-buffered LDS
+//    ring while the walker sweeps the current one (one barrier per row).
+//
+// LDS per group: ln g [nbins][stride] FP64, counts [nbins][stride] u32, the
+// field [N] FP64, the ring [2][L][stride] FP64, stride = walkers in the group.
+#pragma once
+#include "accept_bound.hpp"
+#include "device.hpp"
+
+namespace rewl {
+
+__device__ __forceinline__ double pm_bit(uint64_t w, int x) { return ((w >> x) & 1ull) ? -1.0 : 1.0; }
+
+// Bounds for every attempt position of one row of one sweep pass, for every
+// walker of the group (producer lane j serves walker j); the producer
+// wavefronts split the row's Philox blocks.
+template <int MODEL>
+__device__ void field_fill_row(const DevGroup& G, const DevSystem& S, int wave, int nwaves, int lane,
+                               uint64_t sweep_base, int row, double* ring_buf, const BoundEntry* tab) {
+    constexpr int NPASS = MODEL == 0 ? 1 : 2;
+    const int L = S.L, stride = G.nwalkers;
+    if (lane >= stride) return;
+    const uint32_t walker = (uint32_t)(G.first_walker - S.first_walker + lane);
+    const int per_sweep = NPASS * L;
+    const int s = row / per_sweep, rem = row - s * per_sweep;
+    const int pass = rem / L, y = rem - pass * L;
+    const int p0 = pass * S.N + y * L;
+    const int b0 = p0 >> 2, b1 = (p0 + L - 1) >> 2;
+    const uint64_t B = (uint64_t)((S.ndof + 3) / 4);
+    const uint64_t blk0 = (sweep_base + (uint64_t)s) * B;
+    for (int blk = b0 + wave; blk <= b1; blk += nwaves) {
+        const uint64_t bi = blk0 + (uint64_t)blk;
+        const u32x4 r = philox4x32_10(u32x4{(uint32_t)bi, (uint32_t)(bi >> 32), DOMAIN_SWEEP, walker}, S.k0, S.k1);
+        const int x0 = blk * 4 - p0;
+        // all four bounds first (their table reads issue together), then the
+        // positions of this row
+        const double b4[4] = {accept_bound(r.x, tab), accept_bound(r.y, tab), accept_bound(r.z, tab),
+                              accept_bound(r.w, tab)};
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int x = x0 + j;
+            if (x >= 0 && x < L) ring_buf[x * stride + lane] = b4[j];
+        }
+    }
+}
+
+template <int MODEL>
+__device__ void field_produce(const DevGroup& G, const DevSystem& S, int wave, int nwaves, int lane, int nsweeps,
+                              uint64_t sweep_base, double* ring, const BoundEntry* tab) {
+    constexpr int NPASS = MODEL == 0 ? 1 : 2;
+    const int rows = nsweeps * NPASS * S.L;
+    const size_t bufsz = (size_t)S.L * G.nwalkers;
+    field_fill_row<MODEL>(G, S, wave, nwaves, lane, sweep_base, 0, ring, tab);
+    __syncthreads();   // pairs with the walker's barrier after loading the histograms
+    for (int r = 0; r < rows; ++r) {
+        if (r + 1 < rows) field_fill_row<MODEL>(G, S, wave, nwaves, lane, sweep_base, r + 1, ring + ((r + 1) & 1) * bufsz, tab);
+        __syncthreads();   // pairs with the walker's end-of-row barrier
+    }
+}
+
+template <int MODEL>
+__device__ void sweep_walker_field(const KArgs& a, const DevGroup& G, const DevSystem& S, int lane, int nsweeps,
+                                   double* s_lng, uint32_t* s_cnt, const double* s_field, const double* ring) {
+    const bool active = lane < G.nwalkers;
+    const int g = G.first_walker + (active ? lane : 0);
+    const int wi = a.w_window[g];
+    const DevWindow Wn = a.win[wi];
+    const double lnf = a.win_lnf[wi];
+    const int64_t hb = G.hist_base + lane;
+    const int nbins = G.nbins;
+    const int stride = G.nwalkers;
+
+    if (active)
+        for (int b = 0; b < nbins; ++b) {
+            s_lng[b * stride + lane] = a.lng[hb + (int64_t)b * kLanes];
+            s_cnt[b * stride + lane] = a.cnt[hb + (int64_t)b * kLanes];
+        }
+    __syncthreads();   // histograms and field loaded, first row of bounds ready
+
+    const int L = S.L;
+    uint64_t* lat = a.lat + G.lat_base + lane;
+    int fl = active ? a.w_flavor[g] : 0;
+    double Ed = active ? a.w_E[g] : 0.0;
+    int cb = min(max((int)((Ed - Wn.emin) / Wn.bs), 0), Wn.nbins - 1);
+    double lc = active ? s_lng[cb * stride + lane] : 0.0;
+    const int plo = Wn.phi_lo, phi = Wn.phi_hi, nbmax = Wn.nbins - 1;
+    const double emin = Wn.emin, bs = Wn.bs;
+    constexpr int NP = MODEL == 0 ? 1 : 2;
+    constexpr int NPASS = MODEL == 0 ? 1 : 2;
+    int buf = 0;
+
+    for (int s = 0; s < nsweeps; ++s) {
+        for (int pass = 0; pass < NPASS; ++pass) {
+            uint64_t up[NP], cu[NP], dn[NP], r0[NP], nx[NP];
+#pragma unroll
+            for (int q = 0; q < NP; ++q) {
+                up[q] = active ? lat[(int64_t)(q * L + L - 1) * kLanes] : 0ull;
+                cu[q] = active ? lat[(int64_t)(q * L) * kLanes] : 0ull;
+                dn[q] = active ? lat[(int64_t)(q * L + (L > 1 ? 1 : 0)) * kLanes] : 0ull;
+                r0[q] = 0ull;
+            }
+            for (int y = 0; y < L; ++y) {
+              if (active) {
+#pragma unroll
+                for (int q = 0; q < NP; ++q) nx[q] = (y + 2 < L) ? lat[(int64_t)(q * L + y + 2) * kLanes] : 0ull;
+                const double* brow = ring + (size_t)buf * L * stride + lane;
+                const double* frow = s_field + y * L;
+                for (int x = 0; x < L; ++x) {
+                    const double B = brow[x * stride];
+                    const double hfield = frow[x];
+                    const int xl = (x == 0) ? L - 1 : x - 1;
+                    const int xr = (x == L - 1) ? 0 : x + 1;
+                    double En;
+                    if (MODEL == 0) {
+                        const double sv = pm_bit(cu[0], x);
+                        double f = hfield;
+                        f += S.J * pm_bit(up[0], x);
+                        f += S.J * pm_bit(cu[0], xl);
+                        f += S.J * pm_bit(cu[0], xr);
+                        f += S.J * pm_bit(dn[0], x);
+                        const double t = -sv;
+                        En = Ed + -1. * t * f - -1. * sv * f;
+                    } else {
+                        const double sg = pm_bit(cu[0], x), ta = pm_bit(cu[1], x);
+                        double ts = sg, tt = ta;
+                        ts += -2. * ts * (double)(fl == 0);
+                        tt += -2. * tt * (double)(fl == 1);
+                        const double ns[4] = {pm_bit(up[0], x), pm_bit(cu[0], xl), pm_bit(cu[0], xr), pm_bit(dn[0], x)};
+                        const double nt[4] = {pm_bit(up[1], x), pm_bit(cu[1], xl), pm_bit(cu[1], xr), pm_bit(dn[1], x)};
+                        double en_new = 0., en_old = 0.;
+                        if (S.has_field) { en_new += hfield * (ts * tt); en_old += hfield * (sg * ta); }
+#pragma unroll
+                        for (int k = 0; k < 4; ++k) {
+                            en_new += S.J * ts * ns[k];
+                            en_new += S.J * tt * nt[k];
+                            en_new += S.K * ts * tt * ns[k] * nt[k];
+                            en_old += S.J * sg * ns[k];
+                            en_old += S.J * ta * nt[k];
+                            en_old += S.K * sg * ta * ns[k] * nt[k];
+                        }
+                        En = Ed + -en_new - -en_old;
+                    }
+                    // histogram_index.hpp:8-31, the bin clamped to the window
+                    const int pe = trunc_half(En);
+                    const bool in = pe >= plo && pe < phi;
+                    const int nb = in ? min(max((int)((En - emin) / bs), 0), nbmax) : cb;
+                    const double ln = s_lng[nb * stride + lane];
+                    const bool acc = in && (ln - lc < B);
+                    if (acc) { Ed = En; cb = nb; lc = ln; }
+                    lc += lnf;
+                    s_lng[cb * stride + lane] = lc;
+                    atomicAdd(&s_cnt[cb * stride + lane], 1u);
+                    if (acc) {
+                        if (MODEL == 1 && fl) cu[1] ^= (1ull << x);
+                        else cu[0] ^= (1ull << x);
+                    }
+                }
+#pragma unroll
+                for (int q = 0; q < NP; ++q) {
+                    lat[(int64_t)(q * L + y) * kLanes] = cu[q];
+                    if (y == 0) r0[q] = cu[q];
+                    up[q] = cu[q];
+                    cu[q] = dn[q];
+                    dn[q] = (y + 2 < L) ? nx[q] : r0[q];
+                }
+              }
+              __syncthreads();   // this row's ring buffer consumed, the next one filled
+              buf ^= 1;
+            }
+            if (MODEL == 1) fl ^= 1;   // flavour switches after site N-1
+        }
+    }
+
+    if (active) {
+        for (int b = 0; b < nbins; ++b) {
+            a.lng[hb + (int64_t)b * kLanes] = s_lng[b * stride + lane];
+            a.cnt[hb + (int64_t)b * kLanes] = s_cnt[b * stride + lane];
+        }
+        a.w_E[g] = Ed;
+        a.w_flavor[g] = fl;
+    }
+}
+
+// One workgroup = one walker wavefront + NPROD producer wavefronts; groups are
+// taken persistently in blockIdx order.
+template <int MODEL, int NPROD>
+__global__ void __launch_bounds__(64 * (1 + NPROD)) k_sweep_field(KArgs a, const int32_t* groups, int ngroups,
+                                                                  int nsweeps, uint64_t sweep_base) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int tid = threadIdx.x;
+    const BoundEntry* tab = static_cast<const BoundEntry*>(a.bound_tab);
+    // the walker wavefront issues whenever it is ready; a producer sharing its
+    // SIMD fills the cycles it spends waiting
+    if (tid < kLanes) __builtin_amdgcn_s_setprio(3);
+    for (int gi = blockIdx.x; gi < ngroups; gi += gridDim.x) {
+        const DevGroup G = a.grp[groups[gi]];
+        const DevSystem S = a.sys[G.sys];
+        double* s_lng = reinterpret_cast<double*>(smem);
+        uint32_t* s_cnt = reinterpret_cast<uint32_t*>(s_lng + (size_t)G.nbins * G.nwalkers);
+        double* s_field = reinterpret_cast<double*>(smem + (((size_t)G.nbins * G.nwalkers * 12 + 15) & ~(size_t)15));
+        double* ring = s_field + S.N;
+        for (int i = tid; i < S.N; i += blockDim.x) s_field[i] = field_at(S, i);
+        if (tid < kLanes)
+            sweep_walker_field<MODEL>(a, G, S, tid, nsweeps, s_lng, s_cnt, s_field, ring);
+        else
+            field_produce<MODEL>(G, S, tid / kLanes - 1, NPROD, tid & (kLanes - 1), nsweeps, sweep_base, ring, tab);
+        __syncthreads();   // the ring, field and LDS histograms are reused by the next group
+    }
+}
+
+}  // namespace rewl

@@ -165,6 +165,16 @@ class Engine:
         abi.check(self.lib.rewl_get_flavor(self.ctx, int(g), C.byref(f)))
         return f.value
 
+    def observables(self, g):
+        """Observable sums [bin, ob] of walker g (observables on); the mean of a
+        bin is the sum over visits(g) + counts(g) of that bin."""
+        n = self._nbins(g)
+        s = self.systems[self.windows[self.walker_window[g]]["system"]]
+        nobs = abi.NUM_OBS_ISING if s["model"] == abi.ISING2D else abi.NUM_OBS_AT
+        out = np.empty((n, nobs), dtype=np.float64)
+        abi.check(self.lib.rewl_get_observables(self.ctx, int(g), out.ctypes.data_as(C.POINTER(C.c_double)), n, nobs))
+        return out
+
     def rng_cursor(self, g):
         c = C.c_uint64()
         abi.check(self.lib.rewl_get_rng_cursor(self.ctx, int(g), C.byref(c)))
