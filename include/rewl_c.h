@@ -32,6 +32,8 @@
  *   rewl_stitch            concatenate_tables_multiple_overlap + normalize_logDoS
  *                          (concatenate_rewl_tables.hpp:167-305, array_shift.hpp:15-43)
  *   rewl_thermodynamics    Thermodynamics::calculate_thermodynamics (thermodynamics.hpp:156-236)
+ *   rewl_random_field      generate_random_field with an explicit seed (random_fields.hpp:41-75,
+ *                          random_number_generators.hpp:33-57)
  */
 #ifndef REWL_C_H
 #define REWL_C_H
@@ -246,6 +248,16 @@ int rewl_stitch(rewl_ctx* ctx, int32_t system, double states_per_site,
 int rewl_thermodynamics(const double* energies, const double* logdos, int32_t num_bins,
                         int32_t system_size, double t_min, double t_max, int32_t num_t,
                         double* out);
+
+/* ---- disorder ------------------------------------------------------------- */
+enum rewl_disorder { REWL_DISORDER_UNIFORM = 0, REWL_DISORDER_GAUSSIAN = 1 };
+/* generate_random_field (random_fields.hpp:41-75) with the clock seed replaced
+ * by `seed`: one std::mt19937_64(seed); per site 20 discarded values and one
+ * kept value of the chosen distribution: uniform 2 strength (-1 + 2 u), or
+ * get_gaussian(0, strength) (random_number_generators.hpp:33-57: Box-Muller
+ * on (u, v) redrawn while u <= DBL_EPSILON, then the cosine or the sine branch
+ * by a third draw < 0.5).  Host only: needs no GPU. */
+int rewl_random_field(uint64_t seed, int32_t distribution, double strength, int32_t num_sites, double* out);
 
 /* ---- device self-check ----------------------------------------------------- */
 /* No counterpart in the reference.  what = 0: largest absolute error of the

@@ -127,6 +127,8 @@ class MlpTrainParams(C.Structure):
 
 MLP_NUM_PARAMS = 33793
 NUM_OBS_ISING = 3
+DISORDER_UNIFORM = 0
+DISORDER_GAUSSIAN = 1
 NUM_OBS_AT = 12
 
 SIGNATURES = {
@@ -154,6 +156,7 @@ SIGNATURES = {
     "rewl_get_flavor": ([_P, C.c_int32, C.POINTER(C.c_int32)], C.c_int),
     "rewl_get_rng_cursor": ([_P, C.c_int32, C.POINTER(C.c_uint64)], C.c_int),
     "rewl_get_observables": ([_P, C.c_int32, C.POINTER(C.c_double), C.c_int32, C.c_int32], C.c_int),
+    "rewl_random_field": ([C.c_uint64, C.c_int32, C.c_double, C.c_int32, C.POINTER(C.c_double)], C.c_int),
     "rewl_set_logdos": ([_P, C.c_int32, C.POINTER(C.c_double), C.c_int32], C.c_int),
     "rewl_set_window_lnf": ([_P, C.c_int32, C.c_double], C.c_int),
     "rewl_set_spins": ([_P, C.c_int32, C.POINTER(C.c_int8), C.c_int32], C.c_int),

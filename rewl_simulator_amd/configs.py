@@ -116,6 +116,18 @@ def gaussian_strain(L, sigma, seed):
     return rng.normal(0.0, sigma, size=L * L)
 
 
+def reference_random_field(L, strength, seed, distribution=abi.DISORDER_GAUSSIAN):
+    """The reference's generate_random_field (random_fields.hpp:41-75) for an
+    explicit seed, computed by the engine library (rewl_random_field): with
+    the seed the reference drew from the clock, this is its field."""
+    import ctypes as C
+    lib = abi.load_engine()
+    out = np.empty(L * L, dtype=np.float64)
+    abi.check(lib.rewl_random_field(C.c_uint64(seed), int(distribution), float(strength), L * L,
+                                    out.ctypes.data_as(C.POINTER(C.c_double))))
+    return out
+
+
 def at_energy(L, sigma, tau, J=1.0, K=0.5, field=None):
     """Ashkin-Teller energy of a configuration (numpy bond sum)."""
     s = np.asarray(sigma, float).reshape(L, L)

@@ -12,6 +12,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <limits>
 #include <mutex>
 #include <numeric>
 #include <random>
@@ -1163,6 +1164,33 @@ int rewl_thermodynamics(const double* E, const double* lng, int32_t nb, int32_t 
     (void)hipFree(dE);
     (void)hipFree(dl);
     (void)hipFree(dout);
+    return REWL_OK;
+}
+
+int rewl_random_field(uint64_t seed, int32_t distribution, double strength, int32_t num_sites, double* out) {
+    if (!out || num_sites < 0) return fail(REWL_ERR_INVALID, "rewl_random_field: bad argument");
+    if (distribution != REWL_DISORDER_UNIFORM && distribution != REWL_DISORDER_GAUSSIAN)
+        return fail(REWL_ERR_INVALID, "rewl_random_field: unknown distribution");
+    std::mt19937_64 gen(seed);
+    std::uniform_real_distribution<double> u_real(0., 1.);
+    const double eps = std::numeric_limits<double>::epsilon();
+    const double two_pi = 2.0 * std::acos(-1.0);
+    auto draw = [&]() -> double {
+        if (distribution == REWL_DISORDER_UNIFORM) return 2. * strength * (-1. + 2. * u_real(gen));
+        double u = u_real(gen), v = u_real(gen);
+        while (u <= eps) { u = u_real(gen); v = u_real(gen); }
+        const double magnitude = strength * std::sqrt(-2.0 * std::log(u));
+        const double z0 = magnitude * std::cos(two_pi * v) + 0.;
+        const double z1 = magnitude * std::sin(two_pi * v) + 0.;
+        const bool return_cos = u_real(gen) < 0.5;
+        return z0 * return_cos + z1 * (!return_cos);
+    };
+    for (int32_t i = 0; i < num_sites; ++i) {
+        double h = 0.;
+        for (int k = 0; k < 20; ++k) h = draw();   // WAIT_BETWEEN_FIELD_GENERATION (random_fields.hpp:12)
+        h = draw();
+        out[i] = h;
+    }
     return REWL_OK;
 }
 

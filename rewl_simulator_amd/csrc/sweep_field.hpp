@@ -27,6 +27,12 @@ namespace rewl {
 
 __device__ __forceinline__ double pm_bit(uint64_t w, int x) { return ((w >> x) & 1ull) ? -1.0 : 1.0; }
 
+// The reference's sums add products c * s1 * ... of a coupling (or field) c
+// and spins s = +-1 one by one: en += c s1 s2.  The product is exact, so
+// en = fma(c s1, s2, en) rounds once exactly where en + (c s1 s2) does: one
+// FMA per term in place of two multiplies and an add.
+__device__ __forceinline__ double add_term(double en, double cs, double s) { return fma(cs, s, en); }
+
 // Bounds for every attempt position of one row of one sweep pass, for every
 // walker of the group (producer lane j serves walker j); the producer
 // wavefronts split the row's Philox blocks.
@@ -128,31 +134,37 @@ __device__ void sweep_walker_field(const KArgs& a, const DevGroup& G, const DevS
                     const int xr = (x == L - 1) ? 0 : x + 1;
                     double En;
                     if (MODEL == 0) {
+                        // f = h + J s_up + J s_left + J s_right + J s_down, then
+                        // E' = E + (-1 t) f - (-1 s) f with t = -s (reference order)
                         const double sv = pm_bit(cu[0], x);
                         double f = hfield;
-                        f += S.J * pm_bit(up[0], x);
-                        f += S.J * pm_bit(cu[0], xl);
-                        f += S.J * pm_bit(cu[0], xr);
-                        f += S.J * pm_bit(dn[0], x);
+                        f = add_term(f, S.J, pm_bit(up[0], x));
+                        f = add_term(f, S.J, pm_bit(cu[0], xl));
+                        f = add_term(f, S.J, pm_bit(cu[0], xr));
+                        f = add_term(f, S.J, pm_bit(dn[0], x));
                         const double t = -sv;
                         En = Ed + -1. * t * f - -1. * sv * f;
                     } else {
+                        // en = h (s t) + sum_k [J s s_k + J t t_k + K s t s_k t_k] for the
+                        // proposed (ts, tt) and the current (sg, ta) spins, then
+                        // E' = E + -en_new - -en_old (reference order)
                         const double sg = pm_bit(cu[0], x), ta = pm_bit(cu[1], x);
-                        double ts = sg, tt = ta;
-                        ts += -2. * ts * (double)(fl == 0);
-                        tt += -2. * tt * (double)(fl == 1);
+                        const double ts = fl == 0 ? -sg : sg, tt = fl == 1 ? -ta : ta;
                         const double ns[4] = {pm_bit(up[0], x), pm_bit(cu[0], xl), pm_bit(cu[0], xr), pm_bit(dn[0], x)};
                         const double nt[4] = {pm_bit(up[1], x), pm_bit(cu[1], xl), pm_bit(cu[1], xr), pm_bit(dn[1], x)};
+                        const double Jts = S.J * ts, Jtt = S.J * tt, Ktt = S.K * ts * tt;
+                        const double Jsg = S.J * sg, Jta = S.J * ta, Kst = S.K * sg * ta;
                         double en_new = 0., en_old = 0.;
                         if (S.has_field) { en_new += hfield * (ts * tt); en_old += hfield * (sg * ta); }
 #pragma unroll
                         for (int k = 0; k < 4; ++k) {
-                            en_new += S.J * ts * ns[k];
-                            en_new += S.J * tt * nt[k];
-                            en_new += S.K * ts * tt * ns[k] * nt[k];
-                            en_old += S.J * sg * ns[k];
-                            en_old += S.J * ta * nt[k];
-                            en_old += S.K * sg * ta * ns[k] * nt[k];
+                            const double nn = ns[k] * nt[k];
+                            en_new = add_term(en_new, Jts, ns[k]);
+                            en_new = add_term(en_new, Jtt, nt[k]);
+                            en_new = add_term(en_new, Ktt, nn);
+                            en_old = add_term(en_old, Jsg, ns[k]);
+                            en_old = add_term(en_old, Jta, nt[k]);
+                            en_old = add_term(en_old, Kst, nn);
                         }
                         En = Ed + -en_new - -en_old;
                     }
