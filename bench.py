@@ -53,6 +53,8 @@ def parse():
     ap.add_argument("--max-checks", type=int, default=200000)
     ap.add_argument("--average-mode", type=int, default=0,
                     help="replica averaging: 0 count-weighted (the reference), 1 equal weights")
+    ap.add_argument("--dump", default="", metavar="PATH",
+                    help="with --converge: also save E, ln g and the exact ln g to PATH (.npz)")
     ap.add_argument("--one-over-t", action="store_true",
                     help="the reference's ONE_OVER_T_ALGORITHM ln f schedule instead of halving")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
@@ -167,6 +169,16 @@ def converge(eng, params, a):
     ex = exact_binned_lng(E, params_bin_size(eng))
     ok = np.isfinite(ex)
     d = (lng - ex)[ok]
+    E2, lng2 = eng.stitched(0, states_per_site=2.0, join="overlap")
+    ok2 = np.isfinite(ex[:len(lng2)]) & np.isfinite(lng2)
+    d2 = (lng2 - ex[:len(lng2)])[ok2]
+    if a.dump:
+        nbmax = max(w["num_bins"] for w in eng.windows)
+        wl = np.full((eng.num_windows, nbmax), np.nan)
+        for k, w in enumerate(eng.windows):
+            wl[k, :w["num_bins"]] = eng.logdos(w["first_walker"])
+        np.savez(a.dump, E=E, lng=lng, exact=ex, window_emin=np.array([w["e_min"] for w in eng.windows]),
+                 window_lng=wl)
     print(json.dumps({
         "metric": "time to ln f < %g" % a.converge, "value": t1 - t0, "unit": "s", "higher_is_better": False,
         "converged": bool(lnf < a.converge), "checks": checks, "sweeps": st1["sweeps"] - st0["sweeps"],
@@ -176,6 +188,7 @@ def converge(eng, params, a):
         "lng_max_abs_error_normalised": float(np.max(np.abs(d))),
         "lng_max_abs_error_anchored": float(np.max(np.abs(d - d[0]))),
         "lng_bins_compared": int(ok.sum()),
+        "lng_max_abs_error_overlap_join": float(np.max(np.abs(d2))),
         "config": {"workload": "2D Ising REWL, L=64 periodic, 32 windows x 64 walkers, flatness 0.8",
                    "schedule": "1/t" if a.one_over_t else "halving", "average_mode": params["average_mode"]},
     }), flush=True)

@@ -243,6 +243,14 @@ int rewl_set_spins(rewl_ctx* ctx, int32_t walker, const int8_t* spins, int32_t n
  * most capacity entries; *n_out receives the curve length. */
 int rewl_stitch(rewl_ctx* ctx, int32_t system, double states_per_site,
                 double* energies, double* logdos, int32_t capacity, int32_t* n_out);
+/* No counterpart in the reference: the same curve joined by the mean
+ * difference of adjacent windows over every bin both have visited (replica 0
+ * after rewl_finalize; a window takes over from the middle of that stretch),
+ * on the system's global bin grid (bins no walker visited are NaN), then
+ * normalised like rewl_stitch.  The reference's join takes the shift from one
+ * bin pair, the first of the overlap when its derivatives match to an int. */
+int rewl_stitch_overlap(rewl_ctx* ctx, int32_t system, double states_per_site,
+                        double* energies, double* logdos, int32_t capacity, int32_t* n_out);
 /* Canonical thermodynamics from (E, ln g) on the GPU, FP64, trapezoid weights:
  * out[t*5 + {0..4}] = free energy, U, U^2, S, C per site at T = Tmin + dT*t. */
 int rewl_thermodynamics(const double* energies, const double* logdos, int32_t num_bins,

@@ -162,6 +162,8 @@ SIGNATURES = {
     "rewl_set_spins": ([_P, C.c_int32, C.POINTER(C.c_int8), C.c_int32], C.c_int),
     "rewl_stitch": ([_P, C.c_int32, C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int32,
                      C.POINTER(C.c_int32)], C.c_int),
+    "rewl_stitch_overlap": ([_P, C.c_int32, C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int32,
+                     C.POINTER(C.c_int32)], C.c_int),
     "rewl_thermodynamics": ([C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int32, C.c_int32, C.c_double,
                              C.c_double, C.c_int32, C.POINTER(C.c_double)], C.c_int),
     "rewl_selfcheck": ([C.c_int32, C.POINTER(C.c_double)], C.c_int),

@@ -1167,6 +1167,70 @@ int rewl_thermodynamics(const double* E, const double* lng, int32_t nb, int32_t 
     return REWL_OK;
 }
 
+int rewl_stitch_overlap(rewl_ctx* c, int32_t si, double states_per_site, double* energies, double* logdos,
+                        int32_t capacity, int32_t* n_out) {
+    if (!c || si < 0 || si >= (int32_t)c->sys.size() || !n_out) return fail(REWL_ERR_INVALID, "bad system");
+    const HostSystem& S = c->sys[si];
+    const int nw = S.d.num_windows;
+    // the system's global bin grid; unvisited bins stay NaN
+    int nb_total = 0;
+    std::vector<long long> g0(nw);
+    for (int k = 0; k < nw; ++k) {
+        const HostWindow& w = c->win[S.first_window + k];
+        g0[k] = std::llround((w.emin - S.d.e_min) / S.d.bin_size);
+        nb_total = std::max<int>(nb_total, (int)(g0[k] + w.nbins));
+    }
+    std::vector<double> lng((size_t)nb_total, std::nan(""));
+    std::vector<char> vis((size_t)nb_total, 0);
+    for (int k = 0; k < nw; ++k) {
+        const HostWindow& w = c->win[S.first_window + k];
+        std::vector<double> lk(w.nbins);
+        std::vector<uint32_t> ck(w.nbins);
+        std::vector<uint64_t> vk(w.nbins), vsum(w.nbins, 0);
+        int rc = rewl_get_logdos(c, w.first_walker, lk.data(), w.nbins);
+        if (rc) return rc;
+        for (int r = 0; r < w.R; ++r) {
+            if ((rc = rewl_get_counts(c, w.first_walker + r, ck.data(), w.nbins))) return rc;
+            if ((rc = rewl_get_visits(c, w.first_walker + r, vk.data(), w.nbins))) return rc;
+            for (int b = 0; b < w.nbins; ++b) vsum[b] += vk[b] + ck[b];
+        }
+        if (k == 0) {
+            for (int b = 0; b < w.nbins; ++b)
+                if (vsum[b]) { lng[(size_t)(g0[k] + b)] = lk[b]; vis[(size_t)(g0[k] + b)] = 1; }
+            continue;
+        }
+        // shift onto the curve so far: mean difference over the bins both visited
+        double sum = 0.;
+        std::vector<long long> common;
+        for (int b = 0; b < w.nbins; ++b) {
+            const size_t i = (size_t)(g0[k] + b);
+            if (vsum[b] && vis[i]) { sum += lng[i] - lk[b]; common.push_back((long long)i); }
+        }
+        if (common.empty()) return fail(REWL_ERR_INVALID, "adjacent windows share no visited bin");
+        const double shift = sum / (double)common.size();
+        const long long from = common[common.size() / 2];   // take over from the middle of the common stretch
+        for (int b = 0; b < w.nbins; ++b) {
+            const long long i = g0[k] + b;
+            if (vsum[b] && i >= from) { lng[(size_t)i] = lk[b] + shift; vis[(size_t)i] = 1; }
+        }
+    }
+    // normalise over the visited bins as normalize_logDoS does (array_shift.hpp:15-43)
+    if (states_per_site > 0) {
+        double mx = -INFINITY;
+        for (size_t i = 0; i < lng.size(); ++i) if (vis[i]) mx = std::max(mx, lng[i]);
+        double sum = 0.;
+        for (size_t i = 0; i < lng.size(); ++i) if (vis[i]) sum += exp(lng[i] - mx);
+        const double sh = (double)S.N * log(states_per_site) - (log(sum) + mx) - log(2.) * (double)(S.d.e_max == 0.);
+        for (size_t i = 0; i < lng.size(); ++i) if (vis[i]) lng[i] += sh;
+    }
+    *n_out = nb_total;
+    for (int i = 0; i < nb_total && i < capacity; ++i) {
+        if (energies) energies[i] = S.d.e_min + S.d.bin_size * (double)i;
+        if (logdos) logdos[i] = lng[(size_t)i];
+    }
+    return REWL_OK;
+}
+
 int rewl_random_field(uint64_t seed, int32_t distribution, double strength, int32_t num_sites, double* out) {
     if (!out || num_sites < 0) return fail(REWL_ERR_INVALID, "rewl_random_field: bad argument");
     if (distribution != REWL_DISORDER_UNIFORM && distribution != REWL_DISORDER_GAUSSIAN)

@@ -193,15 +193,19 @@ class Engine:
         abi.check(self.lib.rewl_set_spins(self.ctx, int(g), buf.ctypes.data_as(C.POINTER(C.c_int8)), buf.size))
 
     # ---- results ------------------------------------------------------------------
-    def stitched(self, system=0, states_per_site=None):
-        """Stitched ln g of one system; normalised to states_per_site^N when given."""
+    def stitched(self, system=0, states_per_site=None, join="reference"):
+        """Stitched ln g of one system; normalised to states_per_site^N when
+        given.  join="reference": the reference's derivative-matching join
+        (rewl_stitch); "overlap": the mean difference over each overlap
+        (rewl_stitch_overlap, unvisited bins NaN)."""
         cap = sum(w["num_bins"] for w in self.windows if w["system"] == system)
         E = np.empty(cap, dtype=np.float64)
         lg = np.empty(cap, dtype=np.float64)
         n = C.c_int32()
         sps = -1.0 if states_per_site is None else float(states_per_site)
-        abi.check(self.lib.rewl_stitch(self.ctx, int(system), sps, E.ctypes.data_as(C.POINTER(C.c_double)),
-                                       lg.ctypes.data_as(C.POINTER(C.c_double)), cap, C.byref(n)))
+        fn = {"reference": self.lib.rewl_stitch, "overlap": self.lib.rewl_stitch_overlap}[join]
+        abi.check(fn(self.ctx, int(system), sps, E.ctypes.data_as(C.POINTER(C.c_double)),
+                     lg.ctypes.data_as(C.POINTER(C.c_double)), cap, C.byref(n)))
         return E[:n.value].copy(), lg[:n.value].copy()
 
 

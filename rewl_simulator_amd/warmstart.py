@@ -197,7 +197,10 @@ def evaluate(mlp, specs, lnf_final=1e-6, lnf_warm=1e-3, ref_lnf=1e-8, max_checks
     res = {}
     ref = None
     if ref_lnf > 0:
-        with Engine(systems, rfat_params(ref_lnf)) as eng:
+        # the same disorder with independent walker streams: a reference that
+        # shares the main runs' streams would repeat their errors
+        ref_systems = [dict(s, seed=s["seed"] + 0x9E3779B9) for s in systems]
+        with Engine(ref_systems, rfat_params(ref_lnf)) as eng:
             eng.adjust_to_range()
             done, secs = run_until_done(eng, max_checks, log=log, name="reference")
             eng.finalize()
@@ -278,6 +281,7 @@ def main():
     ap.add_argument("--heldout-L", type=int, default=24)
     ap.add_argument("--lnf-final", type=float, default=1e-6)
     ap.add_argument("--ref-lnf", type=float, default=0.0)
+    ap.add_argument("--lnf-warm", type=float, default=1e-3, help="ln f the warm-started windows begin at")
     ap.add_argument("--steps", type=int, default=4000)
     ap.add_argument("--budget-L", type=int, default=0, help="also a fixed-budget comparison at this L")
     ap.add_argument("--budget-checks", type=int, default=200)
@@ -298,10 +302,11 @@ def main():
     cl = generate(train_specs, lnf_final=a.lnf_final, log=log)
     gen_s = time.perf_counter() - t0
     mlp, hist = train(cl, steps=a.steps, log=log)
+    train_ms = mlp.last_ms()
     mlp.save(os.path.join(a.out, "mlp.npz"))
-    res = evaluate(mlp, held, lnf_final=a.lnf_final, ref_lnf=a.ref_lnf, log=log)
+    res = evaluate(mlp, held, lnf_final=a.lnf_final, lnf_warm=a.lnf_warm, ref_lnf=a.ref_lnf, log=log)
     res["generate_seconds"] = gen_s
-    res["train_ms"] = mlp.last_ms()
+    res["train_ms"] = train_ms
     res["curves"] = len(cl)
     res["train_points"] = int(sum(c["visited"].sum() for c in cl))
     res["train_sizes"] = list(sizes)

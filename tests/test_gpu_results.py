@@ -86,3 +86,26 @@ def test_thermodynamics_kernel_matches_oracle(gpu):
     g = thermodynamics(E, lng, 256, 0.01, 50.01, 5000)
     o = O.thermodynamics(E, lng, 256, 0.01, 50.01, 5000)
     np.testing.assert_allclose(g, o, rtol=1e-11, atol=1e-12)
+
+
+def test_overlap_join_recovers_exact_curve(gpu):
+    """rewl_stitch_overlap on windows holding the exact L=16 ln g, each shifted
+    by its own constant: the joined, normalised curve is the exact one."""
+    systems, params = configs.config1()
+    with Engine(systems, params) as eng:
+        eng.adjust_to_range()
+        eng.sweeps(200)   # every reachable bin visited
+        s = systems[0]
+        E_all = s["e_min"] + s["bin_size"] * np.arange(int(round((s["e_max"] - s["e_min"]) / s["bin_size"])))
+        ex = exact.binned_lng(16, s["e_min"], s["bin_size"], len(E_all))
+        rng = np.random.default_rng(3)
+        for w in eng.windows:
+            g0 = int(round((w["e_min"] - s["e_min"]) / s["bin_size"]))
+            lw = ex[g0:g0 + w["num_bins"]] + rng.normal(0, 300)
+            lw = np.where(np.isfinite(lw), lw, 0.0)
+            for r in range(s["replicas_per_window"]):
+                eng.set_logdos(w["first_walker"] + r, lw)
+        E, lng = eng.stitched(0, states_per_site=2.0, join="overlap")
+        ok = np.isfinite(lng) & np.isfinite(ex[:len(lng)])
+        assert ok.sum() > 200
+        np.testing.assert_allclose(lng[ok], ex[:len(lng)][ok], atol=1e-9)
