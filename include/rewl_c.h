@@ -257,6 +257,16 @@ int rewl_thermodynamics(const double* energies, const double* logdos, int32_t nu
                         int32_t system_size, double t_min, double t_max, int32_t num_t,
                         double* out);
 
+/* Canonical averages of per-bin observables (the observable part of
+ * Thermodynamics::calculate_thermodynamics, thermodynamics.hpp:156-236): on the
+ * GPU, FP64, one thread per temperature, trapezoid weights (1/2 at the two end
+ * bins); observables[b * num_obs + o] is the microcanonical mean of observable
+ * o in bin b; out[t * num_obs + o] = <o> at T = t_min + dT t, dT = (t_max -
+ * t_min)/num_t.  num_obs <= 16. */
+int rewl_thermo_observables(const double* energies, const double* logdos, const double* observables,
+                            int32_t num_bins, int32_t num_obs, double t_min, double t_max, int32_t num_t,
+                            double* out);
+
 /* ---- disorder ------------------------------------------------------------- */
 enum rewl_disorder { REWL_DISORDER_UNIFORM = 0, REWL_DISORDER_GAUSSIAN = 1 };
 /* generate_random_field (random_fields.hpp:41-75) with the clock seed replaced

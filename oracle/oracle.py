@@ -42,6 +42,8 @@ def load(path=ORACLE_SO, build_if_missing=True):
         "ora_get_logdos": ([P, C.c_int32, C.POINTER(C.c_double)], C.c_int),
         "ora_get_counts": ([P, C.c_int32, C.POINTER(C.c_uint64)], C.c_int),
         "ora_get_visits": ([P, C.c_int32, C.POINTER(C.c_uint64)], C.c_int),
+        "ora_thermo_observables": ([C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int32,
+                                    C.c_int32, C.c_double, C.c_double, C.c_int32, C.POINTER(C.c_double)], C.c_int),
         "ora_get_observables": ([P, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_double)], C.c_int),
         "ora_get_spins": ([P, C.c_int32, C.POINTER(C.c_int8)], C.c_int),
         "ora_get_state": ([P, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_int32),
@@ -269,4 +271,17 @@ def thermodynamics(E, lng, N, tmin=0.01, tmax=50.01, nT=50000):
     lng = np.ascontiguousarray(lng, dtype=np.float64)
     out = np.empty((nT, 5))
     load().ora_thermodynamics(_dp(E), _dp(lng), E.size, int(N), tmin, tmax, nT, _dp(out))
+    return out
+
+
+def thermo_observables(energies, logdos, observables, t_min=0.01, t_max=50.01, num_t=50000):
+    """The oracle's canonical averages [T, ob] of per-bin observable means."""
+    lib = load()
+    E = np.ascontiguousarray(energies, dtype=np.float64)
+    lg = np.ascontiguousarray(logdos, dtype=np.float64)
+    ob = np.ascontiguousarray(observables, dtype=np.float64).reshape(len(E), -1)
+    out = np.empty((num_t, ob.shape[1]), dtype=np.float64)
+    dp = C.POINTER(C.c_double)
+    lib.ora_thermo_observables(E.ctypes.data_as(dp), lg.ctypes.data_as(dp), ob.ctypes.data_as(dp), len(E),
+                               ob.shape[1], t_min, t_max, num_t, out.ctypes.data_as(dp))
     return out

@@ -925,6 +925,42 @@ int ora_thermodynamics(const double* E, const double* lng, int32_t nb, int32_t N
     return REWL_OK;
 }
 
+/* The observable part of Thermodynamics::calculate_thermodynamics
+ * (thermodynamics.hpp:156-236): per temperature, sum over bins (from the top
+ * down) of O_b times the reduced Boltzmann weight, halved at the two end bins,
+ * divided by the trapezoid partition function. */
+int ora_thermo_observables(const double* E, const double* lng, const double* obs, int32_t nb, int32_t nobs,
+                           double tmin, double tmax, int32_t nT, double* out) {
+    const double dT = (tmax - tmin) / static_cast<double>(nT);
+    for (int32_t Ti = 1; Ti <= nT; ++Ti) {
+        const int32_t t = nT - Ti;
+        const double T = tmin + dT * static_cast<double>(t);
+        auto expo = [&](int32_t b) { return lng[b] - (E[b] / T); };
+        double mx = expo(nb - 1);
+        for (int32_t b = 2; b <= nb; ++b) { double v = expo(nb - b); mx = (mx < v ? v : mx); }
+        double part = 0.;
+        for (int32_t b = 1; b <= nb; ++b) {
+            const int32_t cb = nb - b;
+            const double e = exp(expo(cb) - mx);
+            part += (cb != 0 && cb != nb - 1) ? 2. * e : e;
+        }
+        part = 0.5 * part;
+        double* o = out + (size_t)nobs * t;
+        for (int32_t ob = 0; ob < nobs; ++ob) o[ob] = 0.;
+        for (int32_t b = 1; b <= nb; ++b) {
+            const int32_t cb = nb - b;
+            const double wgt = exp(expo(cb) - mx);
+            for (int32_t ob = 0; ob < nobs; ++ob) {
+                const double v = obs[(size_t)cb * nobs + ob];
+                if (cb == 0 || cb == nb - 1) o[ob] += 0.5 * (v) * wgt;
+                else o[ob] += (v) * wgt;
+            }
+        }
+        for (int32_t ob = 0; ob < nobs; ++ob) o[ob] /= (part);
+    }
+    return REWL_OK;
+}
+
 /* ------------------- CPU baseline ------------------------------------- */
 /* Sweeps every walker `sweeps` times with `threads` std::threads, walkers
  * partitioned round-robin; returns wall seconds and attempts. */

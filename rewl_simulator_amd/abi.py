@@ -156,6 +156,8 @@ SIGNATURES = {
     "rewl_get_flavor": ([_P, C.c_int32, C.POINTER(C.c_int32)], C.c_int),
     "rewl_get_rng_cursor": ([_P, C.c_int32, C.POINTER(C.c_uint64)], C.c_int),
     "rewl_get_observables": ([_P, C.c_int32, C.POINTER(C.c_double), C.c_int32, C.c_int32], C.c_int),
+    "rewl_thermo_observables": ([C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int32,
+                                 C.c_int32, C.c_double, C.c_double, C.c_int32, C.POINTER(C.c_double)], C.c_int),
     "rewl_random_field": ([C.c_uint64, C.c_int32, C.c_double, C.c_int32, C.POINTER(C.c_double)], C.c_int),
     "rewl_set_logdos": ([_P, C.c_int32, C.POINTER(C.c_double), C.c_int32], C.c_int),
     "rewl_set_window_lnf": ([_P, C.c_int32, C.c_double], C.c_int),

@@ -464,4 +464,37 @@ __global__ void k_thermo(const double* E, const double* lng, int nb, int N, doub
     o[4] = (double)N * (U2 - U * U) / (T * T);
 }
 
+// Canonical averages of per-bin observables (the observable part of
+// Thermodynamics::calculate_thermodynamics, thermodynamics.hpp:156-236):
+// <O>(T) = sum_b t_b O_b w_b / Z with w_b = exp(ln g_b - E_b/T - max), t_b = 1/2
+// at the two end bins and 1 inside, Z the trapezoid partition function.
+constexpr int kThermoMaxObs = 16;
+__global__ void k_thermo_obs(const double* E, const double* lng, const double* obs, int nb, int nobs, double tmin,
+                             double dT, int nT, double* out) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nT) return;
+    const double T = tmin + dT * (double)t;
+    double mx = lng[nb - 1] - E[nb - 1] / T;
+    for (int b = 2; b <= nb; ++b) {
+        const int cb = nb - b;
+        const double v = lng[cb] - E[cb] / T;
+        mx = (mx < v ? v : mx);
+    }
+    double part = 0.;
+    double acc[kThermoMaxObs];
+    for (int o = 0; o < kThermoMaxObs; ++o) acc[o] = 0.;
+    for (int b = 1; b <= nb; ++b) {
+        const int cb = nb - b;
+        const double w = exp((lng[cb] - E[cb] / T) - mx);
+        const bool end = (cb == 0 || cb == nb - 1);
+        part += end ? w : 2. * w;
+        for (int o = 0; o < nobs; ++o) {
+            const double ov = obs[(size_t)cb * nobs + o];
+            acc[o] += end ? 0.5 * ov * w : ov * w;
+        }
+    }
+    part = 0.5 * part;
+    for (int o = 0; o < nobs; ++o) out[(size_t)t * nobs + o] = acc[o] / part;
+}
+
 }  // namespace rewl

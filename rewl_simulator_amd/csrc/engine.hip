@@ -1167,6 +1167,36 @@ int rewl_thermodynamics(const double* E, const double* lng, int32_t nb, int32_t 
     return REWL_OK;
 }
 
+int rewl_thermo_observables(const double* E, const double* lng, const double* obs, int32_t nb, int32_t nobs,
+                            double tmin, double tmax, int32_t nT, double* out) {
+    if (!E || !lng || !obs || !out || nb < 2 || nT < 1 || nobs < 1 || nobs > kThermoMaxObs)
+        return fail(REWL_ERR_INVALID, "bad arguments");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(REWL_ERR_NO_DEVICE, "no HIP device");
+    double *dE = nullptr, *dl = nullptr, *dobs = nullptr, *dout = nullptr;
+    auto release = [&]() { (void)hipFree(dE); (void)hipFree(dl); (void)hipFree(dobs); (void)hipFree(dout); };
+    if (hipMalloc((void**)&dE, (size_t)nb * sizeof(double)) != hipSuccess ||
+        hipMalloc((void**)&dl, (size_t)nb * sizeof(double)) != hipSuccess ||
+        hipMalloc((void**)&dobs, (size_t)nb * nobs * sizeof(double)) != hipSuccess ||
+        hipMalloc((void**)&dout, (size_t)nT * nobs * sizeof(double)) != hipSuccess) {
+        release();
+        return fail(REWL_ERR_DEVICE, "hipMalloc failed");
+    }
+    hipError_t e = hipMemcpy(dE, E, (size_t)nb * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dl, lng, (size_t)nb * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dobs, obs, (size_t)nb * nobs * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        const double dT = (tmax - tmin) / (double)nT;
+        hipLaunchKernelGGL(k_thermo_obs, dim3((nT + 127) / 128), dim3(128), 0, nullptr, dE, dl, dobs, nb, nobs, tmin,
+                           dT, nT, dout);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpy(out, dout, (size_t)nT * nobs * sizeof(double), hipMemcpyDeviceToHost);
+    release();
+    if (e != hipSuccess) return fail(REWL_ERR_DEVICE, std::string("rewl_thermo_observables: ") + hipGetErrorString(e));
+    return REWL_OK;
+}
+
 int rewl_stitch_overlap(rewl_ctx* c, int32_t si, double states_per_site, double* energies, double* logdos,
                         int32_t capacity, int32_t* n_out) {
     if (!c || si < 0 || si >= (int32_t)c->sys.size() || !n_out) return fail(REWL_ERR_INVALID, "bad system");

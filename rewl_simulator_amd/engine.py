@@ -227,3 +227,16 @@ def device_count():
     n = C.c_int()
     lib.rewl_device_count(C.byref(n))
     return n.value
+
+
+def thermo_observables(energies, logdos, observables, t_min=0.01, t_max=50.01, num_t=50000):
+    """Canonical averages [T, ob] of per-bin observable means on the GPU."""
+    lib = abi.load_engine()
+    E = np.ascontiguousarray(energies, dtype=np.float64)
+    lg = np.ascontiguousarray(logdos, dtype=np.float64)
+    ob = np.ascontiguousarray(observables, dtype=np.float64).reshape(len(E), -1)
+    out = np.empty((num_t, ob.shape[1]), dtype=np.float64)
+    dp = C.POINTER(C.c_double)
+    abi.check(lib.rewl_thermo_observables(E.ctypes.data_as(dp), lg.ctypes.data_as(dp), ob.ctypes.data_as(dp),
+                                          len(E), ob.shape[1], t_min, t_max, num_t, out.ctypes.data_as(dp)))
+    return out

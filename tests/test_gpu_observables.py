@@ -56,3 +56,21 @@ def test_observables_off_refuses_query(gpu):
     with Engine(systems, configs.params()) as eng:
         with pytest.raises(abi.REWLError):
             eng.observables(0)
+
+
+def test_canonical_observables_match_oracle(gpu):
+    """The GPU canonical averages of per-bin observables against the oracle's
+    restatement of the reference loop, on a converged-looking curve (exact
+    L=16 Ising ln g) with smooth observables, 50 000 temperatures."""
+    from oracle import oracle as O
+    from rewl_simulator_amd import exact
+    from rewl_simulator_amd.engine import thermo_observables
+    E = -512.0 + 4.0 * np.arange(257)
+    lng = exact.binned_lng(16, -512.0, 4.0, 257)
+    ok = np.isfinite(lng)
+    E, lng = E[ok], lng[ok]
+    m = np.abs(E) / 512.0
+    obs = np.stack([m, m * m, m ** 4], axis=1)
+    g = thermo_observables(E, lng, obs)
+    o = O.thermo_observables(E, lng, obs)
+    np.testing.assert_allclose(g, o, rtol=1e-11, atol=1e-300)

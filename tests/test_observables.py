@@ -142,3 +142,26 @@ def test_ashkin_teller_microcanonical_averages_L3():
         checked += 1
     assert checked >= 15
     o.close()
+
+
+def test_canonical_observables_closed_forms():
+    """Two levels E = {0, 1}, g = {1, 1}, O = {0, 1}: <O>(T) = 1/(1 + e^{1/T});
+    a constant observable averages to itself at every T."""
+    E = np.array([0.0, 1.0])
+    lng = np.array([0.0, 0.0])
+    obs = np.array([[0.0, 3.25], [1.0, 3.25]])
+    out = O.thermo_observables(E, lng, obs, t_min=0.1, t_max=10.1, num_t=100)
+    T = 0.1 + 0.1 * np.arange(100)
+    np.testing.assert_allclose(out[:, 0], 1.0 / (1.0 + np.exp(1.0 / T)), rtol=1e-13)
+    np.testing.assert_allclose(out[:, 1], 3.25, rtol=1e-14)
+
+
+def test_canonical_observables_trapezoid_weights():
+    """Three levels: the end bins carry half weight (trapezoid rule)."""
+    E = np.array([-1.0, 0.0, 1.0])
+    lng = np.log(np.array([1.0, 2.0, 1.0]))
+    obs = np.array([[1.0], [0.0], [1.0]])
+    T = 2.0
+    out = O.thermo_observables(E, lng, obs, t_min=T, t_max=T + 1.0, num_t=1)
+    w = np.array([0.5, 1.0, 0.5]) * np.exp(lng - E / T)
+    assert out[0, 0] == pytest.approx((w[0] + w[2]) / w.sum(), rel=1e-14)
