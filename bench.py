@@ -47,6 +47,9 @@ def parse():
     ap.add_argument("--saturate", type=int, default=4, metavar="R",
                     help="also time R independent copies of the workload in one context (R x 2048 walkers, "
                          "enough to fill the chip); 0 to skip")
+    ap.add_argument("--workload", default="config2", choices=["config2", "config4"],
+                    help="config2: L=64 Ising REWL (the headline); config4: 256 batched random-strain "
+                         "Ashkin-Teller realisations in one context")
     ap.add_argument("--converge", type=float, default=0.0, metavar="LNF",
                     help="instead of the throughput steps, time the run from the first sweep until every "
                          "window's ln f is below LNF")
@@ -137,6 +140,60 @@ def exact_binned_lng(E_bins, bin_size):
     return out
 
 
+def batched(a):
+    """Config 4: 256 random-strain AT realisations (L in {16, 24, 32, 48},
+    8 windows x 4 walkers each, 8192 walkers) swept in one launch per step by
+    the field-model kernel; a step is sweeps_per_step sweeps + exchange +
+    check.  Attempts/s counts sigma and tau proposals (2N per sweep)."""
+    from rewl_simulator_amd import configs
+    from rewl_simulator_amd.engine import Engine
+    systems, params = configs.config4()
+    n = a.sweeps_per_step
+    params = dict(params, sweeps_per_check=n, sweeps_per_exchange=n, producers=a.producers,
+                  sweep_kernel=a.sweep_kernel, walkers_per_group=a.walkers_per_group)
+    eng = Engine(systems, params)
+    eng.adjust_to_range()
+
+    def step():
+        eng.sweeps(n)
+        eng.exchange()
+        eng.check()
+
+    for _ in range(a.warmup):
+        step()
+    eng.synchronize()
+    st0 = eng.stats()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        step()
+    eng.synchronize()
+    t1 = time.perf_counter()
+    st1 = eng.stats()
+    eng.close()
+    att = st1["attempts"] - st0["attempts"]
+    rate = att / (t1 - t0)
+    launches = st1["sweep_launches"] - st0["sweep_launches"]
+    avg_launch_s = (st1["kernel_ms_sweep"] - st0["kernel_ms_sweep"]) / 1e3 / max(launches, 1)
+    achieved = att / max(launches, 1) * 32.5 / avg_launch_s / 1e9
+    line = {
+        "metric": "spin-flip attempts/s (batched random-strain Ashkin-Teller)", "value": rate,
+        "unit": "spin-flip attempts/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": (t1 - t0) * 1e3 / a.steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64 energies and ln g", "data": "synthetic (Gaussian strain per realisation, Philox draws)",
+        "config": {"workload": "256 random-strain AT realisations, L in {16,24,32,48}, 8 windows x 4 walkers, "
+                               "512 bins, one launch per step", "walkers": 8192, "sweeps_per_step": n},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_sweep_field<AT, 3 producers>",
+                     "histogram_tier": "LDS", "lds_peak": LDS_PEAK_GBS, "lds_frac": achieved / LDS_PEAK_GBS,
+                     "avg_launch_ms": avg_launch_s * 1e3, "bytes_per_attempt": 32.5},
+    }
+    if not a.no_cpu:
+        threads = a.cpu_threads or min(16, os.cpu_count() or 1)
+        line["cpu_baseline"] = cpu_baseline(systems, params, a.cpu_seconds, threads)
+        line["cpu_baseline"]["gpu_over_cpu"] = rate / line["cpu_baseline"]["value"]
+    print(json.dumps(line), flush=True)
+
+
 def converge(eng, params, a):
     """Time to ln f < a.converge: wall time from the first sweep until the
     slowest window's ln f drops below the threshold (the reference's
@@ -207,6 +264,9 @@ def main():
         print(f"bench.py measures one GPU only; --gpus {a.gpus} is not supported by this build "
               f"(every box here has one GPU).", file=sys.stderr)
         sys.exit(2)
+    if a.workload == "config4":
+        batched(a)
+        return
     from rewl_simulator_amd import configs
     from rewl_simulator_amd.engine import Engine
 

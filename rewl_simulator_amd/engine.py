@@ -175,6 +175,29 @@ class Engine:
         abi.check(self.lib.rewl_get_observables(self.ctx, int(g), out.ctypes.data_as(C.POINTER(C.c_double)), n, nobs))
         return out
 
+    def system_observables(self, system=0):
+        """Per-bin observable means of one system on its global bin grid:
+        every walker's sums pooled over the windows and replicas that hold the
+        bin, divided by their pooled update counts (NaN where no walker has
+        been).  The reference averages the windows' means by overlap count
+        (concatenate_rewl_tables.hpp); pooling weights them by samples."""
+        s = self.systems[system]
+        nobs = abi.NUM_OBS_ISING if s["model"] == abi.ISING2D else abi.NUM_OBS_AT
+        nb = int(round((s["e_max"] - s["e_min"]) / s["bin_size"]))
+        tot = np.zeros((nb, nobs))
+        cnt = np.zeros(nb)
+        for w in self.windows:
+            if w["system"] != system:
+                continue
+            g0 = int(round((w["e_min"] - s["e_min"]) / s["bin_size"]))
+            for r in range(s["replicas_per_window"]):
+                g = w["first_walker"] + r
+                n = self.counts(g).astype(np.float64) + self.visits(g)
+                tot[g0:g0 + w["num_bins"]] += self.observables(g)
+                cnt[g0:g0 + w["num_bins"]] += n
+        with np.errstate(invalid="ignore", divide="ignore"):
+            return s["e_min"] + s["bin_size"] * np.arange(nb), tot / cnt[:, None]
+
     def rng_cursor(self, g):
         c = C.c_uint64()
         abi.check(self.lib.rewl_get_rng_cursor(self.ctx, int(g), C.byref(c)))

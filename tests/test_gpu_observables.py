@@ -74,3 +74,40 @@ def test_canonical_observables_match_oracle(gpu):
     g = thermo_observables(E, lng, obs)
     o = O.thermo_observables(E, lng, obs)
     np.testing.assert_allclose(g, o, rtol=1e-11, atol=1e-300)
+
+
+def test_canonical_magnetisation_L4_end_to_end(gpu):
+    """Observables on, a WL run of the 4x4 Ising model, pooled per-bin means,
+    stitched ln g and the canonical averages on the GPU: <|m|>(T) and <m^2>(T)
+    against the exact sums over all 2^16 states."""
+    from rewl_simulator_amd.engine import thermo_observables
+    L, N = 4, 16
+    s = configs.ising_full_range(L, 4.0, 2, 2)
+    p = configs.params(observables=1, flatness_mode=abi.FLAT_MIN_OVER_MEAN, flatness=0.8, schedule=abi.SCHED_HALVE,
+                       lnf_final=1e-5, sweeps_per_check=500, sweeps_per_exchange=500, average_mode=1)
+    with Engine([s], p) as eng:
+        eng.adjust_to_range()
+        assert eng.run(20000)
+        eng.finalize()
+        E, lng = eng.stitched(0, states_per_site=2.0)
+        Eg, means = eng.system_observables(0)
+    idx = np.searchsorted(Eg, E)
+    obs = means[idx]
+    ok = np.isfinite(obs[:, 0])
+    T = 1.0 + 0.5 * np.arange(9)
+    out = thermo_observables(E[ok], lng[ok], obs[ok], t_min=1.0, t_max=5.5, num_t=9)
+    # exact per-level g and microcanonical means from all 2^16 states, then
+    # the same trapezoid canonical average (end levels at half weight)
+    bits = (np.arange(1 << N)[:, None] >> np.arange(N)) & 1
+    sp = (1 - 2 * bits).reshape(-1, L, L)
+    Ex = -((sp * np.roll(sp, 1, 1)).sum(axis=(1, 2)) + (sp * np.roll(sp, 1, 2)).sum(axis=(1, 2)))
+    m = np.abs(sp.sum(axis=(1, 2))) / N
+    lev = np.unique(Ex)
+    g = np.array([(Ex == e).sum() for e in lev], dtype=np.float64)
+    m1 = np.array([m[Ex == e].mean() for e in lev])
+    m2 = np.array([(m[Ex == e] ** 2).mean() for e in lev])
+    half = np.ones(len(lev)); half[0] = half[-1] = 0.5
+    for i, t in enumerate(T):
+        w = half * g * np.exp(-(lev - lev.min()) / t)
+        np.testing.assert_allclose(out[i, 0], (w * m1).sum() / w.sum(), atol=0.02, err_msg=f"T = {t}")
+        np.testing.assert_allclose(out[i, 1], (w * m2).sum() / w.sum(), atol=0.02, err_msg=f"T = {t}")
