@@ -111,3 +111,28 @@ def test_canonical_magnetisation_L4_end_to_end(gpu):
         w = half * g * np.exp(-(lev - lev.min()) / t)
         np.testing.assert_allclose(out[i, 0], (w * m1).sum() / w.sum(), atol=0.02, err_msg=f"T = {t}")
         np.testing.assert_allclose(out[i, 1], (w * m2).sum() / w.sum(), atol=0.02, err_msg=f"T = {t}")
+
+
+def test_result_files_from_a_run(gpu, tmp_path):
+    """A finished run with observables writes the reference's three files;
+    they read back with the reference scripts' loader."""
+    from rewl_simulator_amd import output
+    s = configs.ising_full_range(8, 4.0, 2, 2)
+    p = configs.params(observables=1, flatness_mode=abi.FLAT_MIN_OVER_MEAN, flatness=0.8, schedule=abi.SCHED_HALVE,
+                       lnf_final=1e-3, sweeps_per_check=200, sweeps_per_exchange=200, average_mode=1)
+    with Engine([s], p) as eng:
+        eng.adjust_to_range()
+        assert eng.run(20000)
+        eng.finalize()
+        files = output.write_results(eng, p, str(tmp_path), t_min=0.5, t_max=5.5, num_t=50)
+    names = [f.split("/")[-1] for f in files]
+    assert names == ["microcanonical_observables-REWL_L-8_J-1.000000_h-0.000000.dat.txt",
+                     "self_averaged_observables-REWL_L-8_J-1.000000_h-0.000000.dat.txt",
+                     "nonlinear_observables-REWL_L-8_J-1.000000_h-0.000000.dat.txt"]
+    micro = np.loadtxt(files[0], delimiter=",", comments="#")
+    selfav = np.loadtxt(files[1], delimiter=",", comments="#")
+    nl = np.loadtxt(files[2], delimiter=",", comments="#")
+    assert micro.shape[1] == 2 + 4 and selfav.shape == (50, 1 + 5 + 4) and nl.shape == (50, 3)
+    # <|m|> falls from ~1 at T = 0.5 toward small values at T = 5.5
+    assert selfav[0, 6] > 0.95 and selfav[-1, 6] < 0.6
+    assert np.all(np.isfinite(nl))
