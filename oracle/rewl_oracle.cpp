@@ -25,9 +25,17 @@
  *           Philox(ctr={0,0,1,w}); exchange draws are sequential 64-bit words
  *           Philox(ctr={lo(c),hi(c),2,w}) with a per-walker cursor c.
  *
- * Parity pins (tests/test_oracle.py): mt19937_64 standard KAT, Philox
- * Random123 KATs, neighbour-table fixture of the reference's own tester,
- * closed-form energies, brute-force L=4 density of states, exact Beale DoS.
+ * With rewl_params.observables it also keeps the reference's per-bin
+ * observable running means and counts per bin (update_observables,
+ * update_observable_average) and restates the observable part of the
+ * canonical thermodynamics.
+ *
+ * Parity pins (tests/test_oracle.py, tests/test_observables.py):
+ * mt19937_64 standard KAT, Philox Random123 KATs, neighbour-table fixture of
+ * the reference's own tester, closed-form energies, brute-force L=4 density
+ * of states, exact Beale DoS, count conservation at the range test's slack,
+ * microcanonical observables against brute force over all states (L=4
+ * Ising, L=3 Ashkin-Teller), two- and three-level canonical closed forms.
  */
 #include <algorithm>
 #include <atomic>

@@ -282,6 +282,8 @@ def main():
     ap.add_argument("--lnf-final", type=float, default=1e-6)
     ap.add_argument("--ref-lnf", type=float, default=0.0)
     ap.add_argument("--lnf-warm", type=float, default=1e-3, help="ln f the warm-started windows begin at")
+    ap.add_argument("--mlp-in", default="", help="load this trained network (mlp.npz) instead of generating and training")
+    ap.add_argument("--no-eval", action="store_true", help="generate and train only")
     ap.add_argument("--steps", type=int, default=4000)
     ap.add_argument("--budget-L", type=int, default=0, help="also a fixed-budget comparison at this L")
     ap.add_argument("--budget-checks", type=int, default=200)
@@ -298,18 +300,23 @@ def main():
     sigmas = tuple(float(v) for v in a.sigmas.split(","))
     train_specs = training_specs(a.curves, sizes=sizes, sigmas=sigmas)
     held = heldout_specs(a.heldout, L=a.heldout_L)
-    t0 = time.perf_counter()
-    cl = generate(train_specs, lnf_final=a.lnf_final, log=log)
-    gen_s = time.perf_counter() - t0
-    mlp, hist = train(cl, steps=a.steps, log=log)
-    train_ms = mlp.last_ms()
-    mlp.save(os.path.join(a.out, "mlp.npz"))
+    if a.mlp_in:
+        mlp = MLP()
+        mlp.load(a.mlp_in)
+        info = dict(mlp_in=a.mlp_in)
+    else:
+        t0 = time.perf_counter()
+        cl = generate(train_specs, lnf_final=a.lnf_final, log=log)
+        gen_s = time.perf_counter() - t0
+        mlp, hist = train(cl, steps=a.steps, log=log)
+        info = dict(generate_seconds=gen_s, train_ms=mlp.last_ms(), curves=len(cl),
+                    train_points=int(sum(c["visited"].sum() for c in cl)), train_sizes=list(sizes))
+        mlp.save(os.path.join(a.out, "mlp.npz"))
+        if a.no_eval:
+            log(json.dumps(info))
+            return
     res = evaluate(mlp, held, lnf_final=a.lnf_final, lnf_warm=a.lnf_warm, ref_lnf=a.ref_lnf, log=log)
-    res["generate_seconds"] = gen_s
-    res["train_ms"] = train_ms
-    res["curves"] = len(cl)
-    res["train_points"] = int(sum(c["visited"].sum() for c in cl))
-    res["train_sizes"] = list(sizes)
+    res.update(info)
     res["heldout"] = dict(count=len(held), L=held[0][0], sigma=held[0][1])
     if a.budget_L:
         res["budget"] = budget_run(mlp, heldout_specs(a.heldout, L=a.budget_L), a.budget_checks, log=log)
