@@ -15,6 +15,7 @@
 // Nothing here computes on the CPU: every step is a call into librewl_hip.so.
 #pragma once
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <stdexcept>
@@ -114,7 +115,17 @@ struct REWL_Parameters {
     uint64_t seed = 1;
     int adjust_mode = 0;                         // 1: alternate AT planes (full spectrum)
     int average_mode = 0;                        // 1: equal-weight replica average
+    bool observables = false;                    // per-bin observables (update_observables)
 };
+
+// generate_random_field (random_fields.hpp:41-75) with an explicit seed in
+// place of the clock: the field the reference draws for that seed.
+inline std::vector<double> random_field(uint64_t seed, double strength, int num_sites, bool gaussian = true) {
+    std::vector<double> h((size_t)num_sites);
+    check(rewl_random_field(seed, gaussian ? REWL_DISORDER_GAUSSIAN : REWL_DISORDER_UNIFORM, strength, num_sites,
+                            h.data()));
+    return h;
+}
 
 template <class Hamiltonian_t>
 class REWL_simulation {
@@ -152,6 +163,7 @@ class REWL_simulation {
         p.producers = 0;
         p.adjust_mode = rp.adjust_mode;
         p.average_mode = rp.average_mode;
+        p.observables = rp.observables ? 1 : 0;
         check(rewl_create(&d, 1, &p, &ctx_));
         check(rewl_adjust_to_range(ctx_));
     }
@@ -196,6 +208,50 @@ class REWL_simulation {
         results(E, lg);
         check(rewl_thermodynamics(E.data(), lg.data(), (int32_t)E.size(), sys_.N(), tmin, tmax, nT, out.data()));
         return out;
+    }
+
+    // Per-bin observable means on the system's bin grid (observables on):
+    // every walker's sums pooled over the windows and replicas holding the
+    // bin, over their pooled update counts; means[b * nobs + o], NaN where no
+    // walker has been.  The reference averages window means by overlap count.
+    void observables(std::vector<double>& energies, std::vector<double>& means, int& nobs) const {
+        int32_t nw = 0;
+        check(rewl_num_windows(ctx_, &nw));
+        rewl_window_info w0{};
+        check(rewl_get_window_info(ctx_, 0, &w0));
+        nobs = sys_.states_per_site() == 2.0 ? REWL_NUM_OBS_ISING : REWL_NUM_OBS_AT;
+        int nb = 0;
+        std::vector<rewl_window_info> wins((size_t)nw);
+        for (int32_t k = 0; k < nw; ++k) {
+            check(rewl_get_window_info(ctx_, k, &wins[(size_t)k]));
+            const int g0 = (int)std::llround((wins[(size_t)k].e_min - w0.e_min) / w0.bin_size);
+            nb = std::max(nb, g0 + wins[(size_t)k].num_bins);
+        }
+        std::vector<double> sum((size_t)nb * nobs, 0.0), cnt((size_t)nb, 0.0);
+        for (int32_t k = 0; k < nw; ++k) {
+            const rewl_window_info& w = wins[(size_t)k];
+            const int g0 = (int)std::llround((w.e_min - w0.e_min) / w0.bin_size);
+            std::vector<double> s((size_t)w.num_bins * nobs);
+            std::vector<uint32_t> c((size_t)w.num_bins);
+            std::vector<uint64_t> v((size_t)w.num_bins);
+            for (int32_t r = 0; r < rp_.replicas_per_window; ++r) {
+                const int32_t g = w.first_walker + r;
+                check(rewl_get_observables(ctx_, g, s.data(), w.num_bins, nobs));
+                check(rewl_get_counts(ctx_, g, c.data(), w.num_bins));
+                check(rewl_get_visits(ctx_, g, v.data(), w.num_bins));
+                for (int b = 0; b < w.num_bins; ++b) {
+                    cnt[(size_t)(g0 + b)] += (double)c[(size_t)b] + (double)v[(size_t)b];
+                    for (int o = 0; o < nobs; ++o) sum[(size_t)(g0 + b) * nobs + o] += s[(size_t)b * nobs + o];
+                }
+            }
+        }
+        energies.resize((size_t)nb);
+        means.assign((size_t)nb * nobs, std::nan(""));
+        for (int b = 0; b < nb; ++b) {
+            energies[(size_t)b] = w0.e_min + w0.bin_size * b;
+            if (cnt[(size_t)b] > 0)
+                for (int o = 0; o < nobs; ++o) means[(size_t)b * nobs + o] = sum[(size_t)b * nobs + o] / cnt[(size_t)b];
+        }
     }
 
     rewl_stats stats() const {

@@ -4,7 +4,11 @@
 // Ashkin-Teller, Ising otherwise, as model_hamiltonians.hpp chooses by macro)
 // and prints the normalised microcanonical ln g/N per energy bin.
 //
-//   rewl_run L windows replicas final_increment [average_mode]
+//   rewl_run L windows replicas final_increment [average_mode] [observables]
+//
+// With observables = 1 it also prints the per-bin means of the model's
+// observables (E/N, then one column per observable).
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 
@@ -27,6 +31,7 @@ int main(int argc, char** argv) {
     rp.replicas_per_window = argc > 3 ? std::atoi(argv[3]) : 2;
     rp.final_increment = argc > 4 ? std::atof(argv[4]) : 1e-4;
     rp.average_mode = argc > 5 ? std::atoi(argv[5]) : 0;
+    rp.observables = argc > 6 && std::atoi(argv[6]) != 0;
     try {
         rewl_host::REWL_simulation<Hamiltonian_t<double>> sim(sp, rp);
         const bool done = sim.simulate(1000000);
@@ -38,6 +43,18 @@ int main(int argc, char** argv) {
                     (unsigned long long)st.exchanges_accepted);
         std::printf("# E/N, ln g/N\n");
         for (size_t i = 0; i < E.size(); ++i) std::printf("%.15e,%.15e\n", E[i] / sp.N(), lg[i] / sp.N());
+        if (rp.observables) {
+            std::vector<double> Eo, means;
+            int nobs = 0;
+            sim.observables(Eo, means, nobs);
+            std::printf("# observables: E/N, then %d per-bin means\n", nobs);
+            for (size_t i = 0; i < Eo.size(); ++i) {
+                if (std::isnan(means[i * nobs])) continue;
+                std::printf("%.15e", Eo[i] / sp.N());
+                for (int o = 0; o < nobs; ++o) std::printf(",%.15e", means[i * nobs + o]);
+                std::printf("\n");
+            }
+        }
         return done ? 0 : 3;
     } catch (const rewl_host::Error& e) {
         std::fprintf(stderr, "%s\n", e.what());
