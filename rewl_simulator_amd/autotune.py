@@ -1,17 +1,19 @@
-"""Launch-shape autotuner for the sweep kernel.
+"""Launch-shape autotuner for the sweep kernels.
 
 For a workload (list of systems + params) it times short sweeps of a scratch
 context for each candidate launch shape and returns the fastest:
 
-  * walkers per wavefront group: the LDS-fitting maximum (0), 32 or 16 --
-    fewer lanes per group means less LDS per workgroup, so more workgroups per
-    CU for large batches, at the cost of idle lanes;
-  * producer wavefronts per workgroup (integer/Philox path only): none (-1),
-    1 or 2 -- threshold generation on other SIMDs versus in the walker wave.
+  * integer energies, Philox draws: the row-unrolled kernel's variants
+    (sweep_kernel 0 = slot speculation, 2 = none, 4 = read-ahead) with 7 or 3
+    producer wavefronts, and the row kernel (sweep_kernel 1);
+  * field models, Philox draws: the field kernel (producers 0) or the generic
+    kernel (producers -1), with the LDS-fitting group width or 16 walkers;
+  * replay mode: the generic kernel at the LDS-fitting group width or 32.
 
 Results are cached per workload signature (in memory, and in a JSON file when
 a path is given).  The choice changes only speed: every launch shape runs
-the same chains (tests/test_gpu_parity.py::test_fast_path_producers_bit_exact).
+the same chains (tests/test_gpu_parity.py checks each kernel against the
+oracle bit for bit).
 """
 import hashlib
 import json
@@ -33,13 +35,22 @@ def signature(systems, params):
     return h
 
 
+TUNED_L = (16, 24, 32, 48, 64)
+
+
 def candidates(systems, params):
     intpath = all(s.get("field") is None and s.get("h", 0.0) == 0.0 for s in systems)
-    fast = intpath and params.get("rng_mode", abi.RNG_PHILOX) == abi.RNG_PHILOX
+    philox = params.get("rng_mode", abi.RNG_PHILOX) == abi.RNG_PHILOX
+    if not philox:
+        return [dict(walkers_per_group=w, producers=0, sweep_kernel=0) for w in (0, 32)]
+    if not intpath:
+        return [dict(walkers_per_group=w, producers=p, sweep_kernel=0) for p in (0, -1) for w in (0, 16)]
     out = []
-    for wpg in (0, 32, 16):
-        for prod in ((-1, 1, 2) if fast else (0,)):
-            out.append(dict(walkers_per_group=wpg, producers=prod))
+    if all(s["L"] in TUNED_L for s in systems):
+        for sk in (0, 2, 4):
+            for prod in (0, 3):
+                out.append(dict(walkers_per_group=0, producers=prod, sweep_kernel=sk))
+    out.append(dict(walkers_per_group=0, producers=0, sweep_kernel=1))
     return out
 
 

@@ -325,3 +325,18 @@ def test_acceptance_bounds_device_equal_host(gpu):
     abi.check(lib.rewl_selfcheck(1, C.byref(bad)))
     print(f"bounds differing from the host's: {bad.value:.0f} of 2^24", flush=True)
     assert bad.value == 0
+
+
+def test_autotuner_picks_a_shape_that_runs(gpu):
+    """The autotuner times every candidate launch shape of a small integer
+    workload and returns the fastest; that shape runs the oracle's chain."""
+    from rewl_simulator_amd import autotune
+    systems = [configs.ising_full_range(16, 8.0, 2, 4)]
+    p = configs.params(sweeps_per_check=20, sweeps_per_exchange=20)
+    best, rates = autotune.autotune(systems, p, sweeps=8)
+    assert best is not None and len(rates) >= 4
+    assert all(r > 0 for r in rates.values())
+    eng, ora = make_pair(systems, dict(p, **best))
+    eng.adjust_to_range(); ora.adjust_to_range()
+    eng.sweeps(10); ora.sweeps(10)
+    assert_same(eng, ora, f"autotuned shape {best}", cursors=False)
