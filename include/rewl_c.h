@@ -61,6 +61,13 @@ extern "C" {
  * (ising2d_observables.hpp:20-27, ashkin_teller2d_observables.hpp) */
 #define REWL_NUM_OBS_ISING      3
 #define REWL_NUM_OBS_AT        12
+/* observables = 2 adds the Fourier correlator G(qmin) after each group of
+ * three, the reference's CORRELATION_LENGTHS layout (Ising |m|, m^2, m^4, G;
+ * AT sigma, tau, phi, Baxter each |x|, x^2, x^4, G), sampled at the 1st,
+ * 1001st, ... update of a bin (fourier_correlator.hpp, ising2d_hamiltonian.hpp:
+ * 247-252) */
+#define REWL_NUM_OBS_ISING_CORR 4
+#define REWL_NUM_OBS_AT_CORR   16
 
 /* ---- enumerations ----------------------------------------------------- */
 enum rewl_model {

@@ -93,6 +93,7 @@ class Oracle:
             raise RuntimeError(self.lib.ora_last_error().decode())
         self.ctx = ctx
         self.systems = systems
+        self.params = dict(params)
         self.num_walkers = self.lib.ora_num_walkers(self.ctx)
         self.abi = abi
 
@@ -171,7 +172,8 @@ class Oracle:
         """(running means [bin, ob], counts_per_bin [bin]) of walker g."""
         n = self.layout(g)[3]
         s = self.systems[self.layout(g)[0]]
-        nobs = 3 if s["model"] == self.abi.ISING2D else 12
+        corr = self.params.get("observables", 0) == 2
+        nobs = (4 if corr else 3) if s["model"] == self.abi.ISING2D else (16 if corr else 12)
         means = np.empty((n, nobs), dtype=np.float64)
         counts = np.empty(n, dtype=np.float64)
         self._chk(self.lib.ora_get_observables(self.ctx, int(g), means.ctypes.data_as(C.POINTER(C.c_double)),

@@ -164,6 +164,7 @@ struct Walker {
     uint64_t exch_cursor = 0;
     uint64_t replay_draws = 0;             /* engine outputs consumed (replay) */
     std::vector<double> obs;               /* running means [bin][ob] (observables on) */
+    bool obs_corr = false;                 /* observables 2: with G(qmin)           */
     std::vector<double> obs_counts;        /* counts_per_bin, as the reference keeps it */
 };
 
@@ -313,31 +314,88 @@ inline void obs_average(Walker& w, size_t bin, int nobs, int ob, double value) {
  * ising2d_hamiltonian.hpp:234-256 (|m|, m^2, m^4), Ashkin-Teller
  * ashkin_teller2d_hamiltonian.hpp:267-318 (sigma, tau, phi, Baxter: |x|, x^2,
  * x^4 with phi = sqrt(sigma^2 + tau^2)); then counts_per_bin += 1. */
+/* Fourier_Correlator::compute_correlator (fourier_correlator.hpp:65-118) over
+ * the walker's FP64 spins: field = spin of `type` (Ising: the spin; AT: sigma
+ * or tau), or sigma * tau when mixed; kernel cos/sin(qmin idx), qmin = 2 pi/L
+ * (fourier_correlator.hpp:14-42). */
+double compute_correlator(const System& S, const Walker& w, int type, bool mixed) {
+    const int L = S.L;
+    const int nt = S.d.model == REWL_ISING2D ? 1 : 2;
+    const double qmin = 2. * acos(-1.) / (double)L;
+    double cr_x = 0., ci_x = 0., cr_y = 0., ci_y = 0.;
+    for (int idx = 0; idx != L; ++idx) {
+        double tcx = 0., tkr = 0., tki = 0.;
+        for (int jdx = 0; jdx != L; ++jdx) {
+            double fv = w.s[(size_t)(idx * L + jdx) * nt + type];
+            fv *= ((double)(!mixed) + (double)mixed * w.s[(size_t)(idx * L + jdx) * nt + (nt - 1)]);
+            tcx += fv;
+            cr_y += fv * cos(qmin * (double)jdx);
+            ci_y += fv * sin(qmin * (double)jdx);
+            tkr += (double)(idx == jdx) * cos(qmin * (double)jdx);
+            tki += (double)(idx == jdx) * sin(qmin * (double)jdx);
+        }
+        cr_x += tcx * tkr;
+        ci_x += tcx * tki;
+    }
+    cr_x = cr_x * cr_x + ci_x * ci_x;
+    cr_y = cr_y * cr_y + ci_y * ci_y;
+    return 0.5 * (cr_x + cr_y);
+}
+
+/* update_qmin_correlator (ising2d_observables.hpp:152-159): running mean over
+ * the n = counts_per_bin / 1000 earlier samples. */
+inline void corr_average(Walker& w, size_t bin, int nobs, int ob, double value, double n) {
+    double& avg = w.obs[bin * (size_t)nobs + (size_t)ob];
+    avg = (value + n * avg) / (n + 1);
+}
+
+/* update_observables at the walker's current bin: Ising
+ * ising2d_hamiltonian.hpp:234-256, Ashkin-Teller
+ * ashkin_teller2d_hamiltonian.hpp:267-318; with correlators (observables 2)
+ * the reference's CORRELATION_LENGTHS layout, G(qmin) sampled when
+ * counts_per_bin % 1000 == 0 (the AT tau correlator repeats the sigma one, as
+ * at ashkin_teller2d_hamiltonian.hpp:298-299); then counts_per_bin += 1. */
 inline void update_observables(const System& S, Walker& w) {
     if (w.obs.empty()) return;
     const size_t b = w.hi(w.E);
     const double divide_N = 1. / (double)S.N;
+    const bool corr = w.obs_corr;
+    const double counts = w.obs_counts[b];
+    const bool sample = corr && ((size_t)counts % 1000 == 0);
+    const double n = (double)((size_t)counts / 1000);
     if (S.d.model == REWL_ISING2D) {
+        const int nobs = corr ? REWL_NUM_OBS_ISING_CORR : REWL_NUM_OBS_ISING;
         const double m = divide_N * std::fabs(w.M);
-        obs_average(w, b, REWL_NUM_OBS_ISING, 0, m);
-        obs_average(w, b, REWL_NUM_OBS_ISING, 1, m * m);
-        obs_average(w, b, REWL_NUM_OBS_ISING, 2, m * m * m * m);
+        obs_average(w, b, nobs, 0, m);
+        obs_average(w, b, nobs, 1, m * m);
+        obs_average(w, b, nobs, 2, m * m * m * m);
+        if (sample) corr_average(w, b, nobs, 3, divide_N * divide_N * compute_correlator(S, w, 0, false), n);
     } else {
         const double sv = divide_N * std::fabs(w.Ms), tv = divide_N * std::fabs(w.Mt), bv = divide_N * std::fabs(w.Bax);
         const double phi2 = sv * sv + tv * tv;
-        const int n = REWL_NUM_OBS_AT;
-        obs_average(w, b, n, 0, sv);
-        obs_average(w, b, n, 1, sv * sv);
-        obs_average(w, b, n, 2, sv * sv * sv * sv);
-        obs_average(w, b, n, 3, tv);
-        obs_average(w, b, n, 4, tv * tv);
-        obs_average(w, b, n, 5, tv * tv * tv * tv);
-        obs_average(w, b, n, 6, std::sqrt(phi2));
-        obs_average(w, b, n, 7, phi2);
-        obs_average(w, b, n, 8, phi2 * phi2);
-        obs_average(w, b, n, 9, bv);
-        obs_average(w, b, n, 10, bv * bv);
-        obs_average(w, b, n, 11, bv * bv * bv * bv);
+        const int nobs = corr ? REWL_NUM_OBS_AT_CORR : REWL_NUM_OBS_AT;
+        const int k = corr ? 4 : 3;   // stride of the (|x|, x^2, x^4[, G]) groups
+        obs_average(w, b, nobs, 0, sv);
+        obs_average(w, b, nobs, 1, sv * sv);
+        obs_average(w, b, nobs, 2, sv * sv * sv * sv);
+        obs_average(w, b, nobs, k + 0, tv);
+        obs_average(w, b, nobs, k + 1, tv * tv);
+        obs_average(w, b, nobs, k + 2, tv * tv * tv * tv);
+        obs_average(w, b, nobs, 2 * k + 0, std::sqrt(phi2));
+        obs_average(w, b, nobs, 2 * k + 1, phi2);
+        obs_average(w, b, nobs, 2 * k + 2, phi2 * phi2);
+        obs_average(w, b, nobs, 3 * k + 0, bv);
+        obs_average(w, b, nobs, 3 * k + 1, bv * bv);
+        obs_average(w, b, nobs, 3 * k + 2, bv * bv * bv * bv);
+        if (sample) {
+            const double g_s = divide_N * divide_N * compute_correlator(S, w, 0, false);
+            const double g_t = g_s;
+            const double g_b = divide_N * divide_N * compute_correlator(S, w, 0, true);
+            corr_average(w, b, nobs, 3, g_s, n);
+            corr_average(w, b, nobs, 7, g_t, n);
+            corr_average(w, b, nobs, 11, g_s + g_t, n);
+            corr_average(w, b, nobs, 15, g_b, n);
+        }
     }
     w.obs_counts[b] += 1.;
 }
@@ -592,7 +650,10 @@ int ora_create(const rewl_system_desc* systems, int32_t nsys, const rewl_params*
                 w.cnt.assign(S.windows[k].nbins, 0);
                 w.visits.assign(S.windows[k].nbins, 0);
                 if (c->p.observables) {
-                    const int nobs = S.d.model == REWL_ISING2D ? REWL_NUM_OBS_ISING : REWL_NUM_OBS_AT;
+                    const bool corr = c->p.observables == 2;
+                    const int nobs = S.d.model == REWL_ISING2D ? (corr ? REWL_NUM_OBS_ISING_CORR : REWL_NUM_OBS_ISING)
+                                                                : (corr ? REWL_NUM_OBS_AT_CORR : REWL_NUM_OBS_AT);
+                    w.obs_corr = corr;
                     w.obs.assign(S.windows[k].nbins * (size_t)nobs, 0.0);
                     w.obs_counts.assign(S.windows[k].nbins, 0.0);
                 }
