@@ -165,3 +165,80 @@ def test_canonical_observables_trapezoid_weights():
     out = O.thermo_observables(E, lng, obs, t_min=T, t_max=T + 1.0, num_t=1)
     w = np.array([0.5, 1.0, 0.5]) * np.exp(lng - E / T)
     assert out[0, 0] == pytest.approx((w[0] + w[2]) / w.sum(), rel=1e-14)
+
+
+def _G_qmin(field, L):
+    """0.5 (|sum_y R_y e^{i q y}|^2 + |sum_x C_x e^{i q x}|^2) / N^2 with row
+    sums R and column sums C of a +-1 field, q = 2 pi / L (the reference's
+    Fourier correlator, fourier_correlator.hpp:65-118, divided by N^2)."""
+    q = 2 * np.pi / L
+    c, s = np.cos(q * np.arange(L)), np.sin(q * np.arange(L))
+    R, C = field.sum(axis=2), field.sum(axis=1)
+    return 0.5 * ((R @ c) ** 2 + (R @ s) ** 2 + (C @ c) ** 2 + (C @ s) ** 2) / (L * L) ** 2
+
+
+def test_ising_correlator_microcanonical_L4():
+    """observables = 2: G(qmin) sampled at every 1000th update of a bin; its
+    per-bin mean against the brute-force microcanonical average over all 2^16
+    states (about 100 samples per bin)."""
+    L, N = 4, 16
+    bits = (np.arange(1 << N)[:, None] >> np.arange(N)) & 1
+    sp = (1 - 2 * bits).reshape(-1, L, L).astype(np.float64)
+    Ex = -((sp * np.roll(sp, 1, 1)).sum(axis=(1, 2)) + (sp * np.roll(sp, 1, 2)).sum(axis=(1, 2)))
+    G = _G_qmin(sp, L)
+    exact = {int(e): G[Ex == e].mean() for e in np.unique(Ex)}
+    s = configs.ising_full_range(L, 4.0, 1, 1)
+    o = O.Oracle([s], configs.params(lnf_initial=1e-3, observables=2, exchange=0,
+                                     flatness_mode=abi.FLAT_MIN_OVER_MEAN, flatness=0.8, schedule=abi.SCHED_HALVE))
+    o.adjust_to_range()
+    o.sweeps(100000)
+    means, counts = o.observables(0)
+    assert means.shape[1] == 4
+    e_min, _, bs, nb = o.window_geometry(0, 0)
+    checked = 0
+    for b in range(nb):
+        E = int(round(e_min + bs * b))
+        if E in exact and counts[b] > 50000:
+            assert means[b, 3] == pytest.approx(exact[E], abs=0.012), E
+            checked += 1
+    assert checked >= 12
+    # the all-up / all-down level has no Fourier component at qmin
+    assert abs(means[0, 3]) < 1e-20
+    o.close()
+
+
+def test_ashkin_teller_correlators_L3():
+    """AT correlators: sigma against brute force, tau equal to sigma (the
+    reference computes both from sigma, ashkin_teller2d_hamiltonian.hpp:298-299),
+    phi their sum, Baxter from sigma tau."""
+    L, N = 3, 9
+    idx = np.arange(1 << (2 * N), dtype=np.int64)
+    bits = (idx[:, None] >> np.arange(2 * N)) & 1
+    sg = (1 - 2 * bits[:, :N]).reshape(-1, L, L).astype(np.float64)
+    ta = (1 - 2 * bits[:, N:]).reshape(-1, L, L).astype(np.float64)
+    E = np.zeros(len(idx))
+    for ax in (1, 2):
+        s2, t2 = np.roll(sg, 1, ax), np.roll(ta, 1, ax)
+        E -= (sg * s2 + ta * t2 + 0.5 * sg * s2 * ta * t2).sum(axis=(1, 2))
+    Gs, Gb = _G_qmin(sg, L), _G_qmin(sg * ta, L)
+    ex_s = {int(round(e)): Gs[E == e].mean() for e in np.unique(E)}
+    ex_b = {int(round(e)): Gb[E == e].mean() for e in np.unique(E)}
+    s = configs.at_system(L, -5.0 * N, 3.0 * N + 1.0, 1.0, 1, 1)
+    o = O.Oracle([s], configs.params(lnf_initial=1e-3, observables=2, exchange=0, adjust_mode=1,
+                                     flatness_mode=abi.FLAT_MIN_OVER_MEAN, flatness=0.8, schedule=abi.SCHED_HALVE))
+    o.adjust_to_range()
+    o.sweeps(300000)
+    means, counts = o.observables(0)
+    assert means.shape[1] == 16
+    np.testing.assert_array_equal(means[:, 7], means[:, 3])
+    np.testing.assert_allclose(means[:, 11], 2 * means[:, 3], rtol=1e-14)
+    e_min, _, bs, nb = o.window_geometry(0, 0)
+    checked = 0
+    for b in range(nb):
+        Eb = int(round(e_min + bs * b))
+        if Eb in ex_s and counts[b] > 30000:
+            assert means[b, 3] == pytest.approx(ex_s[Eb], abs=0.03), Eb
+            assert means[b, 15] == pytest.approx(ex_b[Eb], abs=0.03), Eb
+            checked += 1
+    assert checked >= 10
+    o.close()
