@@ -160,7 +160,9 @@ typedef struct rewl_params {
                                     ising2d_hamiltonian.hpp:234-256) or the 12 AT
                                     observables (ashkin_teller2d_hamiltonian.hpp:
                                     267-318); sweeps then run the generic kernel.
-                                    0: off                                          */
+                                    2: also the Fourier correlators G(qmin) at every
+                                    1000th update of a bin (REWL_NUM_OBS_*_CORR
+                                    layout).  0: off                                */
     int32_t reserved[2];
 } rewl_params;
 
@@ -235,8 +237,10 @@ int rewl_get_rng_cursor(rewl_ctx* ctx, int32_t walker, uint64_t* cursor);
 /* Observable sums of one walker (observables on): sums[b * nobs + o] = sum of
  * observable o over every update of the walker in bin b.  The reference keeps
  * running means (update_observable_average, ising2d_observables.hpp:134-145)
- * and counts_per_bin; here the mean is sums / (visits + counts) of that bin.
- * nobs is REWL_NUM_OBS_ISING or REWL_NUM_OBS_AT for the walker's model. */
+ * and counts_per_bin; here the mean is sums / (visits + counts) of that bin,
+ * and for a correlator slot (observables = 2) sums / ceil((visits + counts) /
+ * 1000), the number of samples.  nobs is REWL_NUM_OBS_ISING / _AT, or the
+ * _CORR counts with observables = 2, for the walker's model. */
 int rewl_get_observables(rewl_ctx* ctx, int32_t walker, double* sums, int32_t nbins, int32_t nobs);
 
 /* ---- state import (warm start, tests) ------------------------------------- */

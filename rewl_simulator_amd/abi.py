@@ -127,6 +127,8 @@ class MlpTrainParams(C.Structure):
 
 MLP_NUM_PARAMS = 33793
 NUM_OBS_ISING = 3
+NUM_OBS_ISING_CORR = 4
+NUM_OBS_AT_CORR = 16
 DISORDER_UNIFORM = 0
 DISORDER_GAUSSIAN = 1
 NUM_OBS_AT = 12

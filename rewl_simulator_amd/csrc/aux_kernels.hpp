@@ -180,6 +180,7 @@ constexpr int kMaxReplicas = 256;
 __global__ void __launch_bounds__(kLanes) k_exchange(KArgs a, const int32_t* pairs, int npairs, int replay) {
     __shared__ int shuffled[kMaxReplicas];
     __shared__ int accept[kMaxReplicas];
+    __shared__ double s_cs[128];   // cos/sin(qmin k), k < L <= 64, for the correlators
     const int pi = blockIdx.x;
     if (pi >= npairs) return;
     const int lw = pairs[pi], uw = lw + 1;
@@ -187,6 +188,13 @@ __global__ void __launch_bounds__(kLanes) k_exchange(KArgs a, const int32_t* pai
     const DevSystem& S = a.sys[WL.sys];
     const int R = WL.R;
     const int tid = threadIdx.x;
+    if (a.obs_corr) {
+        const double qmin = 2. * acos(-1.) / (double)S.L;
+        for (int k = tid; k < S.L; k += blockDim.x) {
+            s_cs[k] = cos(qmin * (double)k);
+            s_cs[S.L + k] = sin(qmin * (double)k);
+        }
+    }
     for (int k = tid; k < R; k += blockDim.x) shuffled[k] = R + k;
     __syncthreads();
     if (tid == 0) {
@@ -242,11 +250,19 @@ __global__ void __launch_bounds__(kLanes) k_exchange(KArgs a, const int32_t* pai
         a.cnt[ib] += 1u;
         a.lng[ib] += fu;
         if (a.obs) {
-            int m[3];
-            lattice_magnetisations(a.lat + a.w_lat[ga], S.L, S.model, m);
-            update_observables(a.obs + ia * a.nobs, S.model, S.N, m);
-            lattice_magnetisations(a.lat + a.w_lat[gb], S.L, S.model, m);
-            update_observables(a.obs + ib * a.nobs, S.model, S.N, m);
+            const int64_t hi[2] = {ia, ib};
+            const int gw[2] = {ga, gb};
+            for (int j = 0; j < 2; ++j) {
+                int m[3];
+                const uint64_t* lt = a.lat + a.w_lat[gw[j]];
+                lattice_magnetisations(lt, S.L, S.model, m);
+                update_observables(a.obs + hi[j] * a.nobs, S.model, S.N, m, a.obs_corr != 0);
+                if (a.obs_corr) {
+                    const uint32_t t = a.obs_tot[hi[j]];
+                    a.obs_tot[hi[j]] = t + 1u;
+                    if (t % 1000u == 0u) add_correlators(a.obs + hi[j] * a.nobs, lt, S.L, S.model, -1, nullptr, s_cs);
+                }
+            }
         }
     }
 }

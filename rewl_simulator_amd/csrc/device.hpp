@@ -73,7 +73,9 @@ struct KArgs {
     unsigned long long* exch_accepted;
     const void* bound_tab;     // acceptance-bound table (accept_bound.hpp), kBoundTabN entries
     double* obs;               // observable sums [(hist index) * nobs + ob], or null
-    int32_t nobs;              // observables per bin slot (3 Ising, 12 with any AT system)
+    uint32_t* obs_tot;         // with correlators: updates so far per (bin, walker), hist layout
+    int32_t nobs;              // observables per bin slot (3 / 4 Ising, 12 / 16 with any AT system)
+    int32_t obs_corr;          // 1: the layout holds correlator slots
     int32_t num_walkers, num_windows, num_groups;
 };
 
@@ -160,7 +162,9 @@ __device__ __forceinline__ void lattice_magnetisations(const uint64_t* lat, int 
 __device__ __forceinline__ void obs_add(double* p, double v) {
     __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-__device__ __forceinline__ void update_observables(double* obs, int model, int N, const int m[3]) {
+// With corr the layout leaves a correlator slot after each group of three
+// (the reference's CORRELATION_LENGTHS enum order).
+__device__ __forceinline__ void update_observables(double* obs, int model, int N, const int m[3], bool corr = false) {
     const double divide_N = 1. / (double)N;
     if (model == 0) {
         const double v = divide_N * fabs((double)m[0]);
@@ -168,22 +172,80 @@ __device__ __forceinline__ void update_observables(double* obs, int model, int N
         obs_add(obs + 1, v * v);
         obs_add(obs + 2, v * v * v * v);
     } else {
+        const int k = corr ? 4 : 3;
         const double sv = divide_N * fabs((double)m[0]), tv = divide_N * fabs((double)m[1]);
         const double bv = divide_N * fabs((double)m[2]);
         const double phi2 = sv * sv + tv * tv;
         obs_add(obs + 0, sv);
         obs_add(obs + 1, sv * sv);
         obs_add(obs + 2, sv * sv * sv * sv);
-        obs_add(obs + 3, tv);
-        obs_add(obs + 4, tv * tv);
-        obs_add(obs + 5, tv * tv * tv * tv);
-        obs_add(obs + 6, sqrt(phi2));
-        obs_add(obs + 7, phi2);
-        obs_add(obs + 8, phi2 * phi2);
-        obs_add(obs + 9, bv);
-        obs_add(obs + 10, bv * bv);
-        obs_add(obs + 11, bv * bv * bv * bv);
+        obs_add(obs + k + 0, tv);
+        obs_add(obs + k + 1, tv * tv);
+        obs_add(obs + k + 2, tv * tv * tv * tv);
+        obs_add(obs + 2 * k + 0, sqrt(phi2));
+        obs_add(obs + 2 * k + 1, phi2);
+        obs_add(obs + 2 * k + 2, phi2 * phi2);
+        obs_add(obs + 3 * k + 0, bv);
+        obs_add(obs + 3 * k + 1, bv * bv);
+        obs_add(obs + 3 * k + 2, bv * bv * bv * bv);
     }
+}
+
+// G(qmin) of one +-1 field given as L bit rows (bit 1 = -1): the reference's
+// Fourier_Correlator::compute_correlator (fourier_correlator.hpp:65-118),
+// 0.5 (|sum_y R_y e^{i q y}|^2 + |sum_x C_x e^{i q x}|^2), from row sums R
+// (popcounts) and column sums C (bit-sliced vertical counters), divided by
+// N^2 as update_observables does.  cs holds cos(q k), then sin(q k).
+template <typename RowFn>
+__device__ double fourier_qmin(int L, RowFn row, const double* cs) {
+    uint64_t c[7] = {0ull, 0ull, 0ull, 0ull, 0ull, 0ull, 0ull};   // count of set bits per column, bit-sliced
+    double rr = 0., ri = 0.;
+    for (int y = 0; y < L; ++y) {
+        const uint64_t w = row(y);
+        const double Ry = (double)(L - 2 * (int)__popcll(w));   // __popcll is unsigned
+        rr += Ry * cs[y];
+        ri += Ry * cs[L + y];
+        uint64_t carry = w;
+#pragma unroll
+        for (int k = 0; k < 7; ++k) {
+            const uint64_t t = c[k] & carry;
+            c[k] ^= carry;
+            carry = t;
+        }
+    }
+    double cr = 0., ci = 0.;
+    for (int x = 0; x < L; ++x) {
+        int n = 0;
+#pragma unroll
+        for (int k = 0; k < 7; ++k) n |= (int)((c[k] >> x) & 1ull) << k;
+        const double Cx = (double)(L - 2 * n);
+        cr += Cx * cs[x];
+        ci += Cx * cs[L + x];
+    }
+    const double N = (double)L * (double)L;
+    const double divide_N = 1. / N;
+    return divide_N * divide_N * (0.5 * ((rr * rr + ri * ri) + (cr * cr + ci * ci)));
+}
+
+// update_qmin_correlator's samples (ising2d_hamiltonian.hpp:247-252,
+// ashkin_teller2d_hamiltonian.hpp:296-316) added to the correlator slots of
+// one bin: Ising slot 3; AT sigma 3, tau 7 (computed from sigma, as the
+// reference does), phi 11 = sigma + tau, Baxter 15 (field sigma tau).  Row y
+// of the lattice is taken from `cur` (the sweep's register copy), the others
+// from HBM; y < 0 reads every row from HBM.
+__device__ __noinline__ void add_correlators(double* obs, const uint64_t* lat, int L, int model, int y,
+                                             const uint64_t* cur, const double* cs) {
+    auto plane = [&](int q, int r) -> uint64_t { return r == y ? cur[q] : lat[(int64_t)(q * L + r) * kLanes]; };
+    const double gs = fourier_qmin(L, [&](int r) { return plane(0, r); }, cs);
+    if (model == 0) {
+        obs_add(obs + 3, gs);
+        return;
+    }
+    const double gb = fourier_qmin(L, [&](int r) { return plane(0, r) ^ plane(1, r); }, cs);
+    obs_add(obs + 3, gs);
+    obs_add(obs + 7, gs);
+    obs_add(obs + 11, gs + gs);
+    obs_add(obs + 15, gb);
 }
 
 }  // namespace rewl

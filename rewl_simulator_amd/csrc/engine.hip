@@ -159,6 +159,7 @@ struct rewl_ctx {
     DevBuf<unsigned long long> d_exch;
     DevBuf<BoundEntry> d_bound_tab;
     DevBuf<double> d_obs;       // observable sums (observables on)
+    DevBuf<uint32_t> d_obs_tot; // updates so far per (bin, walker) (observables with correlators)
     int nobs = 0;
 
     // replay streams
@@ -194,7 +195,7 @@ void destroy_ctx(rewl_ctx* c) {
     c->d_w_hist.release(); c->d_w_lat.release(); c->d_win_iter.release(); c->d_win_sweeps.release();
     c->d_w_E.release(); c->d_win_lnf.release(); c->d_fields.release();
     c->d_w_cursor.release(); c->d_lat.release(); c->d_visits.release(); c->d_ring.release(); c->d_ring_filled.release();
-    c->d_exclude.release(); c->d_lng.release(); c->d_cnt.release(); c->d_exch.release(); c->d_bound_tab.release(); c->d_obs.release();
+    c->d_exclude.release(); c->d_lng.release(); c->d_cnt.release(); c->d_exch.release(); c->d_bound_tab.release(); c->d_obs.release(); c->d_obs_tot.release();
     c->d_sys.release(); c->d_win.release(); c->d_grp.release();
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -355,23 +356,29 @@ int launch_sweep_class(rewl_ctx* c, int cls, int nsweeps) {
     const int ng = (int)c->class_groups[cls].size();
     if (ng == 0) return REWL_OK;
     const bool obs = c->p.observables != 0;
+    const bool corr = c->p.observables == 2;
     if (INTP && c->p.rng_mode == REWL_RNG_PHILOX && !obs) return launch_sweep_fast<MODEL>(c, cls, nsweeps);
     if (!INTP && c->class_field[cls]) return launch_sweep_field<MODEL>(c, cls, nsweeps);
     const bool packed = c->class_packed[cls];
     const int nb = c->class_maxbins[cls];
-    const size_t lds = (size_t)nb * kLanes * sizeof(double) +
-                       (packed ? (size_t)((nb + 1) / 2) : (size_t)nb) * kLanes * sizeof(uint32_t);
+    // with correlators: ln g, counts and update counts at the group's stride, then the cos/sin table
+    const size_t lds = corr ? (((size_t)nb * c->class_maxlanes[cls] * 16 + 15) & ~(size_t)15) +
+                                  16 * (size_t)c->class_maxL[cls]
+                            : (size_t)nb * kLanes * sizeof(double) +
+                                  (packed ? (size_t)((nb + 1) / 2) : (size_t)nb) * kLanes * sizeof(uint32_t);
     const bool replay = c->p.rng_mode == REWL_RNG_REPLAY;
     const dim3 grid((unsigned)ng), block(kLanes);
     const int32_t* gl = c->d_class_groups[cls].p;
     const int fe = c->flush_every;
     void (*kern)(KArgs, const int32_t*, int, int, uint64_t, int);
-    if (obs) {
-        if (replay) kern = packed ? k_sweep<MODEL, INTP, true, true, true> : k_sweep<MODEL, INTP, true, false, true>;
-        else kern = packed ? k_sweep<MODEL, INTP, false, true, true> : k_sweep<MODEL, INTP, false, false, true>;
+    if (corr) {
+        kern = replay ? k_sweep<MODEL, INTP, true, false, 2> : k_sweep<MODEL, INTP, false, false, 2>;
+    } else if (obs) {
+        if (replay) kern = packed ? k_sweep<MODEL, INTP, true, true, 1> : k_sweep<MODEL, INTP, true, false, 1>;
+        else kern = packed ? k_sweep<MODEL, INTP, false, true, 1> : k_sweep<MODEL, INTP, false, false, 1>;
     } else {
-        if (replay) kern = packed ? k_sweep<MODEL, INTP, true, true, false> : k_sweep<MODEL, INTP, true, false, false>;
-        else kern = packed ? k_sweep<MODEL, INTP, false, true, false> : k_sweep<MODEL, INTP, false, false, false>;
+        if (replay) kern = packed ? k_sweep<MODEL, INTP, true, true, 0> : k_sweep<MODEL, INTP, true, false, 0>;
+        else kern = packed ? k_sweep<MODEL, INTP, false, true, 0> : k_sweep<MODEL, INTP, false, false, 0>;
     }
     HIP_TRY(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     hipLaunchKernelGGL(kern, grid, block, lds, c->stream, c->args, gl, ng, nsweeps, c->sweep_index, fe);
@@ -427,7 +434,7 @@ int rewl_create(const rewl_system_desc* systems, int32_t num_systems, const rewl
     const rewl_params& P = *params;
     if (P.sweeps_per_check <= 0 || P.sweeps_per_exchange <= 0 || P.lnf_initial <= 0 ||
         (P.flatness_mode < 0 || P.flatness_mode > 3) || (P.schedule < 0 || P.schedule > 1) ||
-        (P.rng_mode < 0 || P.rng_mode > 1) || (P.adjust_mode < 0 || P.adjust_mode > 1) || (P.average_mode < 0 || P.average_mode > 1) || (P.sweep_kernel < 0 || P.sweep_kernel > 4) || P.walkers_per_group < 0 || P.walkers_per_group > kLanes)
+        (P.rng_mode < 0 || P.rng_mode > 1) || (P.adjust_mode < 0 || P.adjust_mode > 1) || (P.average_mode < 0 || P.average_mode > 1) || (P.sweep_kernel < 0 || P.sweep_kernel > 4) || (P.observables < 0 || P.observables > 2) || P.walkers_per_group < 0 || P.walkers_per_group > kLanes)
         return fail(REWL_ERR_INVALID, "invalid rewl_params");
 
     rewl_ctx* c = new rewl_ctx();
@@ -541,6 +548,10 @@ int rewl_create(const rewl_system_desc* systems, int32_t num_systems, const rewl
         else if (field_fast) {
             const long long room = (long long)c->lds_max - 16 - 8LL * S.N;
             wpg = room <= 0 ? 0 : std::min<long long>(kLanes, room / (12LL * maxbins + 16LL * S.d.L));
+        } else if (P.observables == 2) {
+            // ln g, counts and update counts per (bin, walker), plus the cos/sin table
+            const long long room = (long long)c->lds_max - 16 - 16LL * S.d.L;
+            wpg = room <= 0 ? 0 : std::min<long long>(kLanes, room / (16LL * maxbins));
         }
         if (P.walkers_per_group > 0) wpg = std::min(wpg, (int)P.walkers_per_group);
         if (wpg < 1) return bail(fail(REWL_ERR_INVALID, "window too wide for LDS: " + std::to_string(maxbins) + " bins"));
@@ -608,6 +619,13 @@ int rewl_create(const rewl_system_desc* systems, int32_t num_systems, const rewl
         if (c->class_field[cls]) {
             if (field_lds(c, cls) > (size_t)c->lds_max)
                 return bail(fail(REWL_ERR_INVALID, "field systems of this batch do not fit LDS together"));
+            continue;
+        }
+        if (P.observables == 2) {
+            c->class_packed[cls] = false;
+            if (((((size_t)nb * c->class_maxlanes[cls] * 16) + 15) & ~(size_t)15) + 16 * (size_t)c->class_maxL[cls] >
+                (size_t)c->lds_max)
+                return bail(fail(REWL_ERR_INVALID, "window too wide for LDS with correlators"));
             continue;
         }
         if ((cls & 1) && P.rng_mode == REWL_RNG_PHILOX && !P.observables) {
@@ -719,11 +737,19 @@ int rewl_create(const rewl_system_desc* systems, int32_t num_systems, const rewl
     if (P.observables) {
         bool any_at = false;
         for (const auto& S : c->sys) any_at = any_at || S.d.model == REWL_ASHKIN_TELLER2D;
-        c->nobs = any_at ? REWL_NUM_OBS_AT : REWL_NUM_OBS_ISING;
+        const bool corr = P.observables == 2;
+        c->nobs = any_at ? (corr ? REWL_NUM_OBS_AT_CORR : REWL_NUM_OBS_AT)
+                         : (corr ? REWL_NUM_OBS_ISING_CORR : REWL_NUM_OBS_ISING);
         const size_t n = c->hist_total * (size_t)c->nobs;
         if (c->d_obs.alloc(n) != hipSuccess) return bail(fail(REWL_ERR_DEVICE, "hipMalloc failed for observables"));
         if (hipMemsetAsync(c->d_obs.p, 0, n * 8, c->stream) != hipSuccess)
             return bail(fail(REWL_ERR_DEVICE, "hipMemset failed"));
+        if (corr) {
+            if (c->d_obs_tot.alloc(c->hist_total) != hipSuccess)
+                return bail(fail(REWL_ERR_DEVICE, "hipMalloc failed for observables"));
+            if (hipMemsetAsync(c->d_obs_tot.p, 0, c->hist_total * 4, c->stream) != hipSuccess)
+                return bail(fail(REWL_ERR_DEVICE, "hipMemset failed"));
+        }
     }
 
     // ---- replay streams ---------------------------------------------------------
@@ -758,7 +784,7 @@ int rewl_create(const rewl_system_desc* systems, int32_t num_systems, const rewl
     a.win_lnf = c->d_win_lnf.p; a.win_iter = c->d_win_iter.p; a.win_sweeps = c->d_win_sweeps.p;
     a.win_oot = c->d_win_oot.p; a.win_done = c->d_win_done.p; a.exclude = c->d_exclude.p;
     a.lat = c->d_lat.p; a.lng = c->d_lng.p; a.cnt = c->d_cnt.p; a.visits = c->d_visits.p;
-    a.obs = c->d_obs.p; a.nobs = c->nobs;
+    a.obs = c->d_obs.p; a.nobs = c->nobs; a.obs_tot = c->d_obs_tot.p; a.obs_corr = P.observables == 2 ? 1 : 0;
     a.ring = c->d_ring.p; a.ring_filled = c->d_ring_filled.p; a.ring_cap = c->ring_cap;
     a.err = c->d_err.p; a.exch_accepted = c->d_exch.p; a.bound_tab = c->d_bound_tab.p;
     a.num_walkers = W; a.num_windows = (int)nwin; a.num_groups = (int)c->groups.size();
@@ -982,7 +1008,9 @@ int rewl_get_observables(rewl_ctx* c, int32_t g, double* sums, int32_t n, int32_
     if (!walker_ok(c, g) || !sums) return fail(REWL_ERR_INVALID, "bad walker");
     if (!c->d_obs.p) return fail(REWL_ERR_INVALID, "observables are off (rewl_params.observables)");
     const HostSystem& S = c->sys[c->win[c->w_window[g]].sys];
-    const int want = S.d.model == REWL_ISING2D ? REWL_NUM_OBS_ISING : REWL_NUM_OBS_AT;
+    const bool corr = c->p.observables == 2;
+    const int want = S.d.model == REWL_ISING2D ? (corr ? REWL_NUM_OBS_ISING_CORR : REWL_NUM_OBS_ISING)
+                                               : (corr ? REWL_NUM_OBS_AT_CORR : REWL_NUM_OBS_AT);
     if (nobs != want) return fail(REWL_ERR_INVALID, "nobs does not match the walker's model");
     const int nb = c->win[c->w_window[g]].nbins;
     if (n < nb) return fail(REWL_ERR_INVALID, "output buffer too small");

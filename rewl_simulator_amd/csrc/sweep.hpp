@@ -18,25 +18,27 @@
 
 namespace rewl {
 
+// LDS arrays are [bin][ls] with ls the LDS lane stride (64, or the group's
+// walker count when observables with correlators need more room per bin).
 template <bool PACKED>
-__device__ __forceinline__ void count_inc(uint32_t* s_cnt, int bin, int lane) {
+__device__ __forceinline__ void count_inc(uint32_t* s_cnt, int bin, int lane, int ls) {
     if (PACKED) {
-        atomicAdd(&s_cnt[(bin >> 1) * kLanes + lane], 1u << ((bin & 1) << 4));
+        atomicAdd(&s_cnt[(bin >> 1) * ls + lane], 1u << ((bin & 1) << 4));
     } else {
-        atomicAdd(&s_cnt[bin * kLanes + lane], 1u);
+        atomicAdd(&s_cnt[bin * ls + lane], 1u);
     }
 }
 
 template <bool PACKED>
-__device__ __forceinline__ void flush_counts(uint32_t* s_cnt, uint32_t* g_cnt, int64_t hb, int nbins, int lane) {
+__device__ __forceinline__ void flush_counts(uint32_t* s_cnt, uint32_t* g_cnt, int64_t hb, int nbins, int lane, int ls) {
     if (PACKED) {
         for (int b = 0; b < nbins; ++b) {
-            const uint32_t v = (s_cnt[(b >> 1) * kLanes + lane] >> ((b & 1) << 4)) & 0xFFFFu;
+            const uint32_t v = (s_cnt[(b >> 1) * ls + lane] >> ((b & 1) << 4)) & 0xFFFFu;
             g_cnt[hb + (int64_t)b * kLanes] += v;
         }
-        for (int b = 0; b < (nbins + 1) / 2; ++b) s_cnt[b * kLanes + lane] = 0u;
+        for (int b = 0; b < (nbins + 1) / 2; ++b) s_cnt[b * ls + lane] = 0u;
     } else {
-        for (int b = 0; b < nbins; ++b) g_cnt[hb + (int64_t)b * kLanes] = s_cnt[b * kLanes + lane];
+        for (int b = 0; b < nbins; ++b) g_cnt[hb + (int64_t)b * kLanes] = s_cnt[b * ls + lane];
     }
 }
 
@@ -63,14 +65,14 @@ __device__ __forceinline__ double draw_uniform(const uint64_t* ring, uint64_t ca
 // the proposal was accepted; updates cb/lc/E and the LDS histogram column.
 template <bool REPLAY, bool PACKED>
 __device__ __forceinline__ bool wl_accept_int(int En, int& E, int& cb, double& lc, double lnf, int wmin, int plo,
-                                              int pspan, int sh, double* s_lng, uint32_t* s_cnt, int lane,
+                                              int pspan, int sh, double* s_lng, uint32_t* s_cnt, int lane, int ls,
                                               const uint64_t* ring, uint64_t cap, uint64_t filled, uint64_t& cursor,
                                               int32_t* err, const u32x4& rnd, int p) {
     const bool in = (unsigned)(phi_int(En) - plo) < (unsigned)pspan;
     // (size_t)((E - min) / bs) truncates toward zero: E = min - 1, which the
     // +0.5 range test lets through when min == 0, lands in bin 0
     const int nb = in ? max((En - wmin) >> sh, 0) : cb;
-    const double ln = s_lng[nb * kLanes + lane];
+    const double ln = s_lng[nb * ls + lane];
     const float d = (float)(ln - lc);
     bool acc = in && (d <= 0.f);
     if (in && !(d <= 0.f)) {
@@ -79,8 +81,8 @@ __device__ __forceinline__ bool wl_accept_int(int En, int& E, int& cb, double& l
     }
     if (acc) { E = En; cb = nb; lc = ln; }
     lc += lnf;
-    s_lng[cb * kLanes + lane] = lc;
-    count_inc<PACKED>(s_cnt, cb, lane);
+    s_lng[cb * ls + lane] = lc;
+    count_inc<PACKED>(s_cnt, cb, lane, ls);
     return acc;
 }
 
@@ -92,12 +94,12 @@ __device__ __forceinline__ bool wl_accept_int(int En, int& E, int& cb, double& l
 template <bool REPLAY, bool PACKED>
 __device__ __forceinline__ bool wl_accept_fp(double En, double& E, int& cb, double& lc, double lnf, double emin,
                                              double bs, int plo, int phi, int nbmax, double* s_lng, uint32_t* s_cnt, int lane,
-                                             const uint64_t* ring, uint64_t cap, uint64_t filled, uint64_t& cursor,
+                                             int ls, const uint64_t* ring, uint64_t cap, uint64_t filled, uint64_t& cursor,
                                              int32_t* err, const u32x4& rnd, int p) {
     const int pe = trunc_half(En);
     const bool in = pe >= plo && pe < phi;
     const int nb = in ? min(max((int)((En - emin) / bs), 0), nbmax) : cb;
-    const double ln = s_lng[nb * kLanes + lane];
+    const double ln = s_lng[nb * ls + lane];
     const float d = (float)(ln - lc);
     bool acc = in && (d <= 0.f);
     if (in && !(d <= 0.f)) {
@@ -106,8 +108,8 @@ __device__ __forceinline__ bool wl_accept_fp(double En, double& E, int& cb, doub
     }
     if (acc) { E = En; cb = nb; lc = ln; }
     lc += lnf;
-    s_lng[cb * kLanes + lane] = lc;
-    count_inc<PACKED>(s_cnt, cb, lane);
+    s_lng[cb * ls + lane] = lc;
+    count_inc<PACKED>(s_cnt, cb, lane, ls);
     return acc;
 }
 
@@ -116,23 +118,30 @@ __device__ __forceinline__ double pm(uint32_t b) { return b ? -1.0 : 1.0; }
 
 
 // MODEL: 0 Ising2d, 1 Ashkin-Teller2d.  INTP: integer energies.  OBS: per-bin
-// observable sums (the walker tracks its magnetisations).
-template <int MODEL, bool INTP, bool REPLAY, bool PACKED, bool OBS>
+// observable sums (the walker tracks its magnetisations); OBS = 2 also samples
+// the Fourier correlators at every 1000th update of a bin, from the lattice as
+// it stands (rows above the current one already written back, the current one
+// in registers), with each bin's update count in LDS (s_tot).
+template <int MODEL, bool INTP, bool REPLAY, bool PACKED, int OBS>
 __device__ void sweep_walker(const KArgs& a, const DevGroup& G, const DevSystem& S, int lane, int nsweeps,
-                             uint64_t sweep_base, int flush_every, double* s_lng, uint32_t* s_cnt) {
+                             uint64_t sweep_base, int flush_every, double* s_lng, uint32_t* s_cnt, uint32_t* s_tot,
+                             const double* s_cs) {
     const int g = G.first_walker + lane;
     const int wi = a.w_window[g];
     const DevWindow Wn = a.win[wi];
     const double lnf = a.win_lnf[wi];
     const int64_t hb = G.hist_base + lane;
     const int nbins = G.nbins;
+    const int ls = OBS == 2 ? G.nwalkers : kLanes;
 
-    for (int b = 0; b < nbins; ++b) s_lng[b * kLanes + lane] = a.lng[hb + (int64_t)b * kLanes];
+    for (int b = 0; b < nbins; ++b) s_lng[b * ls + lane] = a.lng[hb + (int64_t)b * kLanes];
     if (PACKED) {
-        for (int b = 0; b < (nbins + 1) / 2; ++b) s_cnt[b * kLanes + lane] = 0u;
+        for (int b = 0; b < (nbins + 1) / 2; ++b) s_cnt[b * ls + lane] = 0u;
     } else {
-        for (int b = 0; b < nbins; ++b) s_cnt[b * kLanes + lane] = a.cnt[hb + (int64_t)b * kLanes];
+        for (int b = 0; b < nbins; ++b) s_cnt[b * ls + lane] = a.cnt[hb + (int64_t)b * kLanes];
     }
+    if (OBS == 2)
+        for (int b = 0; b < nbins; ++b) s_tot[b * ls + lane] = a.obs_tot[hb + (int64_t)b * kLanes];
 
     const int L = S.L;
     uint64_t* lat = a.lat + G.lat_base + lane;
@@ -153,12 +162,22 @@ __device__ void sweep_walker(const KArgs& a, const DevGroup& G, const DevSystem&
     } else {
         cb = min(max((int)((Ed - Wn.emin) / Wn.bs), 0), Wn.nbins - 1);
     }
-    double lc = s_lng[cb * kLanes + lane];
+    double lc = s_lng[cb * ls + lane];
     const int pspan = Wn.phi_hi - Wn.phi_lo;
     int mag[3] = {0, 0, 0};
     double* obs = OBS ? a.obs + hb * a.nobs : nullptr;
     const int64_t obs_bin = (int64_t)kLanes * a.nobs;
     if (OBS) lattice_magnetisations(lat, S.L, MODEL, mag);
+    // the observables of one update at bin b: the magnetisation moments, and
+    // with OBS = 2 the correlators when the bin's count before it is 0 mod 1000
+    auto observe = [&](int b, int y, const uint64_t* rows) {
+        update_observables(obs + b * obs_bin, MODEL, S.N, mag, OBS == 2);
+        if (OBS == 2) {
+            const uint32_t t = s_tot[b * ls + lane];
+            s_tot[b * ls + lane] = t + 1u;
+            if (t % 1000u == 0u) add_correlators(obs + b * obs_bin, lat, L, MODEL, y, rows, s_cs);
+        }
+    };
     constexpr int NP = MODEL == 0 ? 1 : 2;   // bit planes
     constexpr int NPASS = MODEL == 0 ? 1 : 2;
 
@@ -192,11 +211,11 @@ __device__ void sweep_walker(const KArgs& a, const DevGroup& G, const DevSystem&
                                                 (si ^ bit(dn[0], x));
                             const int En = Ei + S.twoJ * (4 - 2 * (int)na);
                             if (wl_accept_int<REPLAY, PACKED>(En, Ei, cb, lc, lnf, Wn.imin, Wn.phi_lo, pspan, Wn.shift,
-                                                             s_lng, s_cnt, lane, ring, cap, filled, cursor, a.err, rnd, p)) {
+                                                             s_lng, s_cnt, lane, ls, ring, cap, filled, cursor, a.err, rnd, p)) {
                                 cu[0] ^= (1ull << x);
                                 if (OBS) mag[0] += si ? 2 : -2;
                             }
-                            if (OBS) update_observables(obs + cb * obs_bin, MODEL, S.N, mag);
+                            if (OBS) observe(cb, y, cu);
                         } else {
                             const int site = y * L + x;
                             const double sv = pm(si);
@@ -208,11 +227,11 @@ __device__ void sweep_walker(const KArgs& a, const DevGroup& G, const DevSystem&
                             const double t = -sv;
                             const double En = Ed + -1. * t * f - -1. * sv * f;
                             if (wl_accept_fp<REPLAY, PACKED>(En, Ed, cb, lc, lnf, Wn.emin, Wn.bs, Wn.phi_lo, Wn.phi_hi,
-                                                            Wn.nbins - 1, s_lng, s_cnt, lane, ring, cap, filled, cursor, a.err, rnd, p)) {
+                                                            Wn.nbins - 1, s_lng, s_cnt, lane, ls, ring, cap, filled, cursor, a.err, rnd, p)) {
                                 cu[0] ^= (1ull << x);
                                 if (OBS) mag[0] += si ? 2 : -2;
                             }
-                            if (OBS) update_observables(obs + cb * obs_bin, MODEL, S.N, mag);
+                            if (OBS) observe(cb, y, cu);
                         }
                     } else {
                         const uint32_t sb = bit(cu[0], x), tb = bit(cu[1], x);
@@ -225,14 +244,14 @@ __device__ void sweep_walker(const KArgs& a, const DevGroup& G, const DevSystem&
                             const int qq = __popc(ms ^ mt);
                             const int En = Ei + S.twoJ * (4 - 2 * pf) + S.twoK * (4 - 2 * qq);
                             if (wl_accept_int<REPLAY, PACKED>(En, Ei, cb, lc, lnf, Wn.imin, Wn.phi_lo, pspan, Wn.shift,
-                                                             s_lng, s_cnt, lane, ring, cap, filled, cursor, a.err, rnd, p)) {
+                                                             s_lng, s_cnt, lane, ls, ring, cap, filled, cursor, a.err, rnd, p)) {
                                 if (fl) cu[1] ^= (1ull << x); else cu[0] ^= (1ull << x);
                                 if (OBS) {
                                     mag[fl] += (fl ? tb : sb) ? 2 : -2;
                                     mag[2] += (sb ^ tb) ? 2 : -2;
                                 }
                             }
-                            if (OBS) update_observables(obs + cb * obs_bin, MODEL, S.N, mag);
+                            if (OBS) observe(cb, y, cu);
                         } else {
                             const int site = y * L + x;
                             const double sg = pm(sb), ta = pm(tb);
@@ -255,14 +274,14 @@ __device__ void sweep_walker(const KArgs& a, const DevGroup& G, const DevSystem&
                             }
                             const double En = Ed + -en_new - -en_old;
                             if (wl_accept_fp<REPLAY, PACKED>(En, Ed, cb, lc, lnf, Wn.emin, Wn.bs, Wn.phi_lo, Wn.phi_hi,
-                                                            Wn.nbins - 1, s_lng, s_cnt, lane, ring, cap, filled, cursor, a.err, rnd, p)) {
+                                                            Wn.nbins - 1, s_lng, s_cnt, lane, ls, ring, cap, filled, cursor, a.err, rnd, p)) {
                                 if (fl) cu[1] ^= (1ull << x); else cu[0] ^= (1ull << x);
                                 if (OBS) {
                                     mag[fl] += (fl ? tb : sb) ? 2 : -2;
                                     mag[2] += (sb ^ tb) ? 2 : -2;
                                 }
                             }
-                            if (OBS) update_observables(obs + cb * obs_bin, MODEL, S.N, mag);
+                            if (OBS) observe(cb, y, cu);
                         }
                     }
                 }
@@ -277,21 +296,25 @@ __device__ void sweep_walker(const KArgs& a, const DevGroup& G, const DevSystem&
             }
             if (MODEL == 1) fl ^= 1;   // flavour switches after site N-1
         }
-        if (PACKED && ((s + 1) % flush_every) == 0) flush_counts<true>(s_cnt, a.cnt, hb, nbins, lane);
+        if (PACKED && ((s + 1) % flush_every) == 0) flush_counts<true>(s_cnt, a.cnt, hb, nbins, lane, ls);
     }
 
     if (PACKED) {
-        if (nsweeps % flush_every) flush_counts<true>(s_cnt, a.cnt, hb, nbins, lane);
+        if (nsweeps % flush_every) flush_counts<true>(s_cnt, a.cnt, hb, nbins, lane, ls);
     } else {
-        flush_counts<false>(s_cnt, a.cnt, hb, nbins, lane);
+        flush_counts<false>(s_cnt, a.cnt, hb, nbins, lane, ls);
     }
-    for (int b = 0; b < nbins; ++b) a.lng[hb + (int64_t)b * kLanes] = s_lng[b * kLanes + lane];
+    for (int b = 0; b < nbins; ++b) a.lng[hb + (int64_t)b * kLanes] = s_lng[b * ls + lane];
+    if (OBS == 2)
+        for (int b = 0; b < nbins; ++b) a.obs_tot[hb + (int64_t)b * kLanes] = s_tot[b * ls + lane];
     a.w_E[g] = INTP ? (double)Ei : Ed;
     a.w_flavor[g] = fl;
     if (REPLAY) a.w_cursor[g] = cursor;
 }
 
-template <int MODEL, bool INTP, bool REPLAY, bool PACKED, bool OBS>
+// LDS: ln g [nbins][ls] FP64, counts [nbins][ls] u32 (or packed pairs), and
+// with OBS = 2 update counts [nbins][ls] u32 and the cos/sin(qmin k) table.
+template <int MODEL, bool INTP, bool REPLAY, bool PACKED, int OBS>
 __global__ void __launch_bounds__(kLanes) k_sweep(KArgs a, const int32_t* groups, int ngroups, int nsweeps,
                                                   uint64_t sweep_base, int flush_every) {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -299,10 +322,24 @@ __global__ void __launch_bounds__(kLanes) k_sweep(KArgs a, const int32_t* groups
     for (int gi = blockIdx.x; gi < ngroups; gi += gridDim.x) {
         const DevGroup G = a.grp[groups[gi]];
         const DevSystem S = a.sys[G.sys];
+        const int ls = OBS == 2 ? G.nwalkers : kLanes;
         double* s_lng = reinterpret_cast<double*>(smem);
-        uint32_t* s_cnt = reinterpret_cast<uint32_t*>(s_lng + (size_t)G.nbins * kLanes);
+        uint32_t* s_cnt = reinterpret_cast<uint32_t*>(s_lng + (size_t)G.nbins * ls);
+        uint32_t* s_tot = s_cnt + (size_t)G.nbins * ls;
+        double* s_cs = reinterpret_cast<double*>(smem + ((((size_t)G.nbins * ls * 16) + 15) & ~(size_t)15));
+        if (OBS == 2) {
+            // Transform_Kernel (fourier_correlator.hpp:14-42): cos and sin of qmin k
+            const double qmin = 2. * acos(-1.) / (double)S.L;
+            for (int k = lane; k < S.L; k += blockDim.x) {
+                s_cs[k] = cos(qmin * (double)k);
+                s_cs[S.L + k] = sin(qmin * (double)k);
+            }
+            __syncthreads();
+        }
         if (lane < G.nwalkers)
-            sweep_walker<MODEL, INTP, REPLAY, PACKED, OBS>(a, G, S, lane, nsweeps, sweep_base, flush_every, s_lng, s_cnt);
+            sweep_walker<MODEL, INTP, REPLAY, PACKED, OBS>(a, G, S, lane, nsweeps, sweep_base, flush_every, s_lng, s_cnt,
+                                                           s_tot, s_cs);
+        if (OBS == 2) __syncthreads();   // the table is rebuilt for the next group
     }
 }
 

@@ -165,12 +165,25 @@ class Engine:
         abi.check(self.lib.rewl_get_flavor(self.ctx, int(g), C.byref(f)))
         return f.value
 
+    def num_obs(self, model):
+        corr = self.params.get("observables", 0) == 2
+        if model == abi.ISING2D:
+            return abi.NUM_OBS_ISING_CORR if corr else abi.NUM_OBS_ISING
+        return abi.NUM_OBS_AT_CORR if corr else abi.NUM_OBS_AT
+
+    def correlator_slots(self, model):
+        """Columns that hold Fourier-correlator samples (observables = 2)."""
+        if self.params.get("observables", 0) != 2:
+            return []
+        return [3] if model == abi.ISING2D else [3, 7, 11, 15]
+
     def observables(self, g):
         """Observable sums [bin, ob] of walker g (observables on); the mean of a
-        bin is the sum over visits(g) + counts(g) of that bin."""
+        bin is the sum over visits(g) + counts(g) of that bin, or for a
+        correlator slot over ceil((visits + counts) / 1000) samples."""
         n = self._nbins(g)
         s = self.systems[self.windows[self.walker_window[g]]["system"]]
-        nobs = abi.NUM_OBS_ISING if s["model"] == abi.ISING2D else abi.NUM_OBS_AT
+        nobs = self.num_obs(s["model"])
         out = np.empty((n, nobs), dtype=np.float64)
         abi.check(self.lib.rewl_get_observables(self.ctx, int(g), out.ctypes.data_as(C.POINTER(C.c_double)), n, nobs))
         return out
@@ -182,10 +195,12 @@ class Engine:
         been).  The reference averages the windows' means by overlap count
         (concatenate_rewl_tables.hpp); pooling weights them by samples."""
         s = self.systems[system]
-        nobs = abi.NUM_OBS_ISING if s["model"] == abi.ISING2D else abi.NUM_OBS_AT
+        nobs = self.num_obs(s["model"])
+        corr = self.correlator_slots(s["model"])
         nb = int(round((s["e_max"] - s["e_min"]) / s["bin_size"]))
         tot = np.zeros((nb, nobs))
         cnt = np.zeros(nb)
+        samples = np.zeros(nb)
         for w in self.windows:
             if w["system"] != system:
                 continue
@@ -195,8 +210,12 @@ class Engine:
                 n = self.counts(g).astype(np.float64) + self.visits(g)
                 tot[g0:g0 + w["num_bins"]] += self.observables(g)
                 cnt[g0:g0 + w["num_bins"]] += n
+                samples[g0:g0 + w["num_bins"]] += np.ceil(n / 1000.0)
         with np.errstate(invalid="ignore", divide="ignore"):
-            return s["e_min"] + s["bin_size"] * np.arange(nb), tot / cnt[:, None]
+            means = tot / cnt[:, None]
+            for c in corr:
+                means[:, c] = tot[:, c] / samples
+            return s["e_min"] + s["bin_size"] * np.arange(nb), means
 
     def rng_cursor(self, g):
         c = C.c_uint64()

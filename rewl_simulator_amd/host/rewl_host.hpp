@@ -115,7 +115,8 @@ struct REWL_Parameters {
     uint64_t seed = 1;
     int adjust_mode = 0;                         // 1: alternate AT planes (full spectrum)
     int average_mode = 0;                        // 1: equal-weight replica average
-    bool observables = false;                    // per-bin observables (update_observables)
+    int observables = 0;                         // 1: per-bin observables (update_observables),
+                                                 // 2: also the Fourier correlator G(qmin)
 };
 
 // generate_random_field (random_fields.hpp:41-75) with an explicit seed in
@@ -163,7 +164,7 @@ class REWL_simulation {
         p.producers = 0;
         p.adjust_mode = rp.adjust_mode;
         p.average_mode = rp.average_mode;
-        p.observables = rp.observables ? 1 : 0;
+        p.observables = rp.observables;
         check(rewl_create(&d, 1, &p, &ctx_));
         check(rewl_adjust_to_range(ctx_));
     }
@@ -213,13 +214,19 @@ class REWL_simulation {
     // Per-bin observable means on the system's bin grid (observables on):
     // every walker's sums pooled over the windows and replicas holding the
     // bin, over their pooled update counts; means[b * nobs + o], NaN where no
-    // walker has been.  The reference averages window means by overlap count.
+    // walker has been.  A correlator column (observables = 2: column 3 of each
+    // group of four) is sampled once per 1000 updates of a walker's bin, so
+    // its sums are divided by the pooled sample counts, ceil(updates / 1000)
+    // per walker.  The reference averages window means by overlap count.
     void observables(std::vector<double>& energies, std::vector<double>& means, int& nobs) const {
         int32_t nw = 0;
         check(rewl_num_windows(ctx_, &nw));
         rewl_window_info w0{};
         check(rewl_get_window_info(ctx_, 0, &w0));
-        nobs = sys_.states_per_site() == 2.0 ? REWL_NUM_OBS_ISING : REWL_NUM_OBS_AT;
+        const bool ising = sys_.states_per_site() == 2.0;
+        const bool corr = rp_.observables == 2;
+        nobs = ising ? (corr ? REWL_NUM_OBS_ISING_CORR : REWL_NUM_OBS_ISING)
+                     : (corr ? REWL_NUM_OBS_AT_CORR : REWL_NUM_OBS_AT);
         int nb = 0;
         std::vector<rewl_window_info> wins((size_t)nw);
         for (int32_t k = 0; k < nw; ++k) {
@@ -227,7 +234,7 @@ class REWL_simulation {
             const int g0 = (int)std::llround((wins[(size_t)k].e_min - w0.e_min) / w0.bin_size);
             nb = std::max(nb, g0 + wins[(size_t)k].num_bins);
         }
-        std::vector<double> sum((size_t)nb * nobs, 0.0), cnt((size_t)nb, 0.0);
+        std::vector<double> sum((size_t)nb * nobs, 0.0), cnt((size_t)nb, 0.0), smp((size_t)nb, 0.0);
         for (int32_t k = 0; k < nw; ++k) {
             const rewl_window_info& w = wins[(size_t)k];
             const int g0 = (int)std::llround((w.e_min - w0.e_min) / w0.bin_size);
@@ -240,7 +247,9 @@ class REWL_simulation {
                 check(rewl_get_counts(ctx_, g, c.data(), w.num_bins));
                 check(rewl_get_visits(ctx_, g, v.data(), w.num_bins));
                 for (int b = 0; b < w.num_bins; ++b) {
-                    cnt[(size_t)(g0 + b)] += (double)c[(size_t)b] + (double)v[(size_t)b];
+                    const double n = (double)c[(size_t)b] + (double)v[(size_t)b];
+                    cnt[(size_t)(g0 + b)] += n;
+                    smp[(size_t)(g0 + b)] += std::ceil(n / 1000.0);
                     for (int o = 0; o < nobs; ++o) sum[(size_t)(g0 + b) * nobs + o] += s[(size_t)b * nobs + o];
                 }
             }
@@ -250,7 +259,9 @@ class REWL_simulation {
         for (int b = 0; b < nb; ++b) {
             energies[(size_t)b] = w0.e_min + w0.bin_size * b;
             if (cnt[(size_t)b] > 0)
-                for (int o = 0; o < nobs; ++o) means[(size_t)b * nobs + o] = sum[(size_t)b * nobs + o] / cnt[(size_t)b];
+                for (int o = 0; o < nobs; ++o)
+                    means[(size_t)b * nobs + o] =
+                        sum[(size_t)b * nobs + o] / (corr && o % 4 == 3 ? smp[(size_t)b] : cnt[(size_t)b]);
         }
     }
 

@@ -7,7 +7,8 @@
 //   rewl_run L windows replicas final_increment [average_mode] [observables]
 //
 // With observables = 1 it also prints the per-bin means of the model's
-// observables (E/N, then one column per observable).
+// observables (E/N, then one column per observable); with observables = 2
+// the Fourier correlator G(qmin) columns as well.
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -31,7 +32,7 @@ int main(int argc, char** argv) {
     rp.replicas_per_window = argc > 3 ? std::atoi(argv[3]) : 2;
     rp.final_increment = argc > 4 ? std::atof(argv[4]) : 1e-4;
     rp.average_mode = argc > 5 ? std::atoi(argv[5]) : 0;
-    rp.observables = argc > 6 && std::atoi(argv[6]) != 0;
+    rp.observables = argc > 6 ? std::atoi(argv[6]) : 0;
     try {
         rewl_host::REWL_simulation<Hamiltonian_t<double>> sim(sp, rp);
         const bool done = sim.simulate(1000000);

@@ -2,8 +2,9 @@
 
 Three text files per system, in the reference's names, headers, column
 order, delimiter and number format (main.cpp:320-366 with main_headers.hpp's
-DELIMITER = ","; CORRELATION_LENGTHS off, since the correlator is not
-sampled here):
+DELIMITER = ","); with observables = 2 in the reference's default
+CORRELATION_LENGTHS layout (G(qmin) columns and correlation lengths), with
+observables = 1 in its layout without them:
 
   microcanonical_observables-<base>.txt   write_microcanonical_observables.hpp:16-65
       E/N, ln g/N, the per-bin observable means and counts per bin
@@ -33,6 +34,15 @@ AT_OBS = ["Sigma Mag", "Sigma Mag2", "Sigma Mag4", "Tau Mag", "Tau Mag2", "Tau M
 AT_NONLINEAR = ["Sigma Susceptibility", "Sigma Binder Cumulant", "Tau Susceptibility", "Tau Binder Cumulant",
                 "Baxter Susceptibility", "Baxter Binder Cumulant", "Phi Susceptibility", "Phi Binder Cumulant"]
 ENERGY_OBS = ["Free Energy", "Energy", "Energy2", "Entropy", "Specific Heat", "NUM OBS"]
+ISING_OBS_CORR = ["Magnetization", "Magnetization2", "Magnetization4", "G(qmin)", "Counts per Bin", "NUM OBS"]
+ISING_NONLINEAR_CORR = ["Susceptibility", "Binder Cumulant", "Correlation Length over L"]
+AT_OBS_CORR = ["Sigma Mag", "Sigma Mag2", "Sigma Mag4", "Sigma G(qmin)", "Tau Mag", "Tau Mag2", "Tau Mag4",
+               "Tau G(qmin)", "Phi", "Phi2", "Phi4", "Phi G(qmin)", "Baxter", "Baxter2", "Baxter4",
+               "Baxter G(qmin)", "Counts per Bin", "NUM OBS"]
+AT_NONLINEAR_CORR = ["Sigma Susceptibility", "Sigma Binder Cumulant", "Sigma Correlation Length over L",
+                     "Tau Susceptibility", "Tau Binder Cumulant", "Tau Correlation Length over L",
+                     "Baxter Susceptibility", "Baxter Binder Cumulant", "Baxter Correlation Length over L",
+                     "Phi Susceptibility", "Phi Binder Cumulant", "Phi Correlation Length over L"]
 
 
 def to_string(v):
@@ -132,24 +142,42 @@ def write_temperature_table(path, prefix, base, header, names, T, values):
     return fn
 
 
-def nonlinear(model, T, canon, N):
-    """Susceptibilities N (m2 - m^2)/T and Binder cumulants 1 - m4/(3 m2^2)
-    (two-component for phi: 2 (1 - m4/(2 m2^2))), in the reference's order."""
+def nonlinear(model, T, canon, N, corr=False):
+    """Susceptibilities N (m2 - m^2)/T, Binder cumulants 1 - m4/(3 m2^2)
+    (two-component for phi: 2 (1 - m4/(2 m2^2))) and, with correlators, the
+    correlation length over L, (1/(2 pi)) sqrt((G0 - Gq)/Gq) with G0 = m2
+    (order_parameter_cumulants.hpp), in the reference's order."""
+    L = int(round(np.sqrt(N)))
+
     def susc(m2, m):
         return N * (m2 - m * m) / T
 
     def binder(m4, m2):
         return 1. - (m4 / (m2 * m2)) / 3.
 
-    if model == abi.ISING2D:
-        m, m2, m4 = canon[:, 0], canon[:, 1], canon[:, 2]
-        return np.stack([susc(m2, m), binder(m4, m2)], axis=1)
+    def xi(g0, gq):
+        with np.errstate(invalid="ignore", divide="ignore"):
+            v = (L / (2. * np.arccos(-1.))) * np.sqrt((g0 - gq) / gq)
+        return np.where(gq == 0., np.finfo(np.float64).max, v) / L
+
+    k = 4 if corr else 3
     c = canon
-    phi_binder = 2. * (1. - 0.5 * (c[:, 8] / (c[:, 7] * c[:, 7])))
-    return np.stack([susc(c[:, 1], c[:, 0]), binder(c[:, 2], c[:, 1]),
-                     susc(c[:, 4], c[:, 3]), binder(c[:, 5], c[:, 4]),
-                     susc(c[:, 10], c[:, 9]), binder(c[:, 11], c[:, 10]),
-                     susc(c[:, 7], c[:, 6]), phi_binder], axis=1)
+    if model == abi.ISING2D:
+        out = [susc(c[:, 1], c[:, 0]), binder(c[:, 2], c[:, 1])]
+        if corr:
+            out.append(xi(c[:, 1], c[:, 3]))
+        return np.stack(out, axis=1)
+    sg, ta, ph, bx = 0, k, 2 * k, 3 * k
+    phi_binder = 2. * (1. - 0.5 * (c[:, ph + 2] / (c[:, ph + 1] * c[:, ph + 1])))
+    out = []
+    for base in (sg, ta, bx):
+        out += [susc(c[:, base + 1], c[:, base]), binder(c[:, base + 2], c[:, base + 1])]
+        if corr:
+            out.append(xi(c[:, base + 1], c[:, base + 3]))
+    out += [susc(c[:, ph + 1], c[:, ph]), phi_binder]
+    if corr:
+        out.append(xi(c[:, ph + 1], c[:, ph + 3]))
+    return np.stack(out, axis=1)
 
 
 def write_results(eng, params, path, system=0, t_min=0.01, t_max=50.01, num_t=50000):
@@ -162,7 +190,11 @@ def write_results(eng, params, path, system=0, t_min=0.01, t_max=50.01, num_t=50
     walkers = sum(1 for g in range(eng.num_walkers) if eng.windows[eng.walker_window[g]]["system"] == system)
     hdr = file_header(sys_hdr, rewl_header(walkers, s["replicas_per_window"], s.get("overlap", 0.75),
                                            params["sweeps_per_check"], params["lnf_final"], params["flatness"]))
-    names = ISING_OBS if model.startswith("Ising") else AT_OBS
+    corr = params.get("observables", 0) == 2
+    if model.startswith("Ising"):
+        names, nl_names = (ISING_OBS_CORR, ISING_NONLINEAR_CORR) if corr else (ISING_OBS, ISING_NONLINEAR)
+    else:
+        names, nl_names = (AT_OBS_CORR, AT_NONLINEAR_CORR) if corr else (AT_OBS, AT_NONLINEAR)
     sps = 2.0 if s["model"] == abi.ISING2D else 4.0
     E, lng = eng.stitched(system, states_per_site=sps)
     Eg, means = eng.system_observables(system)
@@ -186,7 +218,6 @@ def write_results(eng, params, path, system=0, t_min=0.01, t_max=50.01, num_t=50
     counts_col = np.full((num_t, 1), obs[:, -1].mean())
     files.append(write_temperature_table(path, "self_averaged_observables", base, hdr, ENERGY_OBS + names, T,
                                          np.concatenate([energy, canon, counts_col], axis=1)))
-    nl_names = ISING_NONLINEAR if model.startswith("Ising") else AT_NONLINEAR
     files.append(write_temperature_table(path, "nonlinear_observables", base, hdr, nl_names, T,
-                                         nonlinear(s["model"], T, canon, N)))
+                                         nonlinear(s["model"], T, canon, N, corr)))
     return files

@@ -113,12 +113,14 @@ def test_canonical_magnetisation_L4_end_to_end(gpu):
         np.testing.assert_allclose(out[i, 1], (w * m2).sum() / w.sum(), atol=0.02, err_msg=f"T = {t}")
 
 
-def test_result_files_from_a_run(gpu, tmp_path):
-    """A finished run with observables writes the reference's three files;
-    they read back with the reference scripts' loader."""
+@pytest.mark.parametrize("mode", [1, 2])
+def test_result_files_from_a_run(gpu, tmp_path, mode):
+    """A finished run with observables writes the reference's three files
+    (mode 2 in its CORRELATION_LENGTHS layout); they read back with the
+    reference scripts' loader."""
     from rewl_simulator_amd import output
     s = configs.ising_full_range(8, 4.0, 2, 2)
-    p = configs.params(observables=1, flatness_mode=abi.FLAT_MIN_OVER_MEAN, flatness=0.8, schedule=abi.SCHED_HALVE,
+    p = configs.params(observables=mode, flatness_mode=abi.FLAT_MIN_OVER_MEAN, flatness=0.8, schedule=abi.SCHED_HALVE,
                        lnf_final=1e-3, sweeps_per_check=200, sweeps_per_exchange=200, average_mode=1)
     with Engine([s], p) as eng:
         eng.adjust_to_range()
@@ -132,7 +134,51 @@ def test_result_files_from_a_run(gpu, tmp_path):
     micro = np.loadtxt(files[0], delimiter=",", comments="#")
     selfav = np.loadtxt(files[1], delimiter=",", comments="#")
     nl = np.loadtxt(files[2], delimiter=",", comments="#")
-    assert micro.shape[1] == 2 + 4 and selfav.shape == (50, 1 + 5 + 4) and nl.shape == (50, 3)
+    k = 4 if mode == 1 else 5
+    assert micro.shape[1] == 2 + k and selfav.shape == (50, 1 + 5 + k) and nl.shape == (50, 1 + k - 2)
     # <|m|> falls from ~1 at T = 0.5 toward small values at T = 5.5
     assert selfav[0, 6] > 0.95 and selfav[-1, 6] < 0.6
-    assert np.all(np.isfinite(nl))
+    assert np.all(np.isfinite(nl[:, :3]))
+    if mode == 2:
+        text = open(files[2]).read()
+        assert "#    4: Correlation Length over L\n" in text
+        assert "#    6: G(qmin)\n" in open(files[0]).read()
+        # the ground state has G(qmin) = 0; xi/L is largest in the ordered phase
+        assert abs(micro[np.argmin(micro[:, 0]), 5]) < 1e-12
+        T = nl[:, 0]
+        low, mid = np.argmin(np.abs(T - 1.0)), np.argmin(np.abs(T - 3.0))
+        assert np.isfinite(nl[low, 3]) and np.isfinite(nl[mid, 3]) and nl[low, 3] > nl[mid, 3] > 0
+
+
+@pytest.mark.parametrize("rng", [abi.RNG_REPLAY, abi.RNG_PHILOX])
+@pytest.mark.parametrize("model", [abi.ISING2D, abi.ASHKIN_TELLER2D])
+def test_correlators_match_oracle(gpu, model, rng):
+    """observables = 2: the Fourier correlators sampled at every 1000th update
+    of a bin, in sweeps and after exchanges, against the oracle's restatement
+    of the reference's correlator; the moments alongside, chains bit-exact."""
+    print(f"case: correlators model={model} rng={rng}", flush=True)
+    systems = _systems(model, field=True)
+    p = configs.params(rng_mode=rng, lnf_initial=0.05, observables=2, sweeps_per_exchange=50, adjust_mode=1)
+    eng, ora = Engine(systems, p), Oracle(systems, p)
+    eng.adjust_to_range(); ora.adjust_to_range()
+    for _ in range(6):
+        eng.sweeps(50); ora.sweeps(50)
+        eng.exchange(); ora.exchange()
+    eng.check(); ora.check()
+    assert_same(eng, ora, "correlators on", cursors=(rng == abi.RNG_REPLAY))
+    corr = eng.correlator_slots(model)
+    sampled = 0
+    for g in range(eng.num_walkers):
+        sums = eng.observables(g)
+        n = eng.counts(g).astype(np.float64) + eng.visits(g)
+        means, counts = ora.observables(g)
+        np.testing.assert_array_equal(n, counts)
+        seen = n > 0
+        k = np.ceil(n[seen] / 1000.0)
+        for c in range(sums.shape[1]):
+            div = k if c in corr else n[seen]
+            np.testing.assert_allclose(sums[seen, c] / div, means[seen, c], rtol=1e-9, atol=1e-13,
+                                       err_msg=f"walker {g}, column {c}")
+        sampled += int((n > 1000).sum())
+    assert sampled > 0   # some bins took more than one correlator sample
+    eng.close(); ora.close()

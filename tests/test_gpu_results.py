@@ -79,21 +79,26 @@ def test_host_program_matches_exact(gpu):
     assert err < 0.1
 
 
-def test_host_program_observables(gpu):
+@pytest.mark.parametrize("mode", [1, 2])
+def test_host_program_observables(gpu, mode):
     """The C++ host layer's pooled per-bin observables: the ground-state bin
-    (E0 = -2N, all spins equal) has |m| = m^2 = m^4 = 1 exactly, and every
-    mean lies in [0, 1]."""
+    (E0 = -2N, all spins equal) has |m| = m^2 = m^4 = 1 exactly and, with the
+    Fourier correlator (mode 2), G(qmin) = 0 (a uniform lattice has no
+    component at q = 2 pi / L); every mean lies in [0, 1]."""
     exe = os.path.join(ROOT, "rewl_simulator_amd", "host", "rewl_run")
-    out = subprocess.run([exe, "8", "2", "2", "1e-3", "1", "1"], capture_output=True, text=True, timeout=600)
+    out = subprocess.run([exe, "8", "2", "2", "1e-3", "1", str(mode)], capture_output=True, text=True, timeout=600)
     assert out.returncode == 0, out.stderr
     lines = out.stdout.splitlines()
     k = next(i for i, l in enumerate(lines) if l.startswith("# observables"))
     obs = np.array([[float(v) for v in l.split(",")] for l in lines[k + 1:] if l])
-    assert obs.shape[1] == 4 and len(obs) > 5
+    assert obs.shape[1] == (4 if mode == 1 else 5) and len(obs) > 5
     ground = obs[np.argmin(obs[:, 0])]
     assert ground[0] == -2.0
-    np.testing.assert_array_equal(ground[1:], [1.0, 1.0, 1.0])
-    assert np.all((obs[:, 1:] >= 0) & (obs[:, 1:] <= 1))
+    np.testing.assert_array_equal(ground[1:4], [1.0, 1.0, 1.0])
+    if mode == 2:
+        assert abs(ground[4]) < 1e-12
+        assert np.any(obs[:, 4] > 1e-4)
+    assert np.all((obs[:, 1:] >= -1e-12) & (obs[:, 1:] <= 1))
 
 
 def test_thermodynamics_kernel_matches_oracle(gpu):

@@ -74,3 +74,29 @@ def test_nonlinear_closed_forms():
     assert nl.shape == (2, 8)
     np.testing.assert_allclose(nl[:, 0], 16 * (0.3 - 0.25) / T)
     np.testing.assert_allclose(nl[:, 7], 2 * (1 - 0.5 * 0.6 / 0.49))
+
+
+def test_nonlinear_correlation_length():
+    """calculate_correlation_length (order_parameter_cumulants.hpp:53-59) over
+    L, in the CORRELATION_LENGTHS column order of ising2d_observables.hpp:186-190
+    and ashkin_teller2d_observables.hpp:285-297."""
+    T = np.array([1.0, 2.0])
+    L = 4
+    # G0 = 5 Gq: xi/L = sqrt(4)/(2 pi); Gq = 0 gives DBL_MAX / L.
+    canon = np.array([[0.5, 0.5, 0.3, 0.1], [0.5, 0.5, 0.3, 0.0]])
+    nl = output.nonlinear(abi.ISING2D, T, canon, L * L, corr=True)
+    assert nl.shape == (2, 3)
+    np.testing.assert_allclose(nl[0, 2], 2.0 / (2 * np.pi), rtol=1e-15)
+    assert nl[1, 2] == np.finfo(np.float64).max / L
+    np.testing.assert_allclose(nl[:, :2], output.nonlinear(abi.ISING2D, T, canon[:, :3], L * L))
+    # AT groups sigma, tau, phi, baxter of (m, m2, m4, G); output order sigma, tau, baxter, phi.
+    g = {"s": [0.5, 0.4, 0.3, 0.1], "t": [0.6, 0.5, 0.4, 0.05], "p": [0.7, 0.9, 1.1, 0.3], "b": [0.2, 0.2, 0.1, 0.04]}
+    c = np.tile(g["s"] + g["t"] + g["p"] + g["b"], (2, 1))
+    nl = output.nonlinear(abi.ASHKIN_TELLER2D, T, c, L * L, corr=True)
+    assert nl.shape == (2, 12)
+    ref = output.nonlinear(abi.ASHKIN_TELLER2D, T, c[:, [0, 1, 2, 4, 5, 6, 8, 9, 10, 12, 13, 14]], L * L)
+    np.testing.assert_allclose(nl[:, [0, 1, 3, 4, 6, 7, 9, 10]], ref)
+    for col, (m2, gq) in zip([2, 5, 8, 11], [(0.4, 0.1), (0.5, 0.05), (0.2, 0.04), (0.9, 0.3)]):
+        np.testing.assert_allclose(nl[:, col], np.sqrt((m2 - gq) / gq) / (2 * np.pi), rtol=1e-14)
+    assert len(output.AT_NONLINEAR_CORR) == 12 and output.AT_NONLINEAR_CORR[11] == "Phi Correlation Length over L"
+    assert len(output.AT_OBS_CORR) == 18 and output.AT_OBS_CORR[3] == "Sigma G(qmin)"
