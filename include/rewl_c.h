@@ -34,6 +34,9 @@
  *   rewl_thermodynamics    Thermodynamics::calculate_thermodynamics (thermodynamics.hpp:156-236)
  *   rewl_random_field      generate_random_field with an explicit seed (random_fields.hpp:41-75,
  *                          random_number_generators.hpp:33-57)
+ *   rewl_anneal            Simulated_Annealer::simulate_annealing (simulated_annealer.hpp:92-194)
+ *   rewl_get_densities     Ashkin_Teller2d_Obs::density_histograms before transfer_and_normalize
+ *                          (ashkin_teller2d_observables.hpp:113-138, ashkin_teller_densities.hpp:30-57)
  */
 #ifndef REWL_C_H
 #define REWL_C_H
@@ -68,6 +71,12 @@ extern "C" {
  * 247-252) */
 #define REWL_NUM_OBS_ISING_CORR 4
 #define REWL_NUM_OBS_AT_CORR   16
+/* Ashkin-Teller sigma-tau densities (rewl_params.densities): per energy bin a
+ * 32 x 32 count histogram of (sigma/N, tau/N) over [-1.05, 1.05]^2, cell
+ * (tau axis) * 32 + (sigma axis), axis = (int)((x + 1.05) / (2.1 / 32))
+ * (ashkin_teller_densities.hpp:11-34, ashkin_teller_densities_parameters.cxx) */
+#define REWL_DENSITY_AXIS_BINS 32
+#define REWL_DENSITY_CELLS     1024
 
 /* ---- enumerations ----------------------------------------------------- */
 enum rewl_model {
@@ -163,7 +172,11 @@ typedef struct rewl_params {
                                     2: also the Fourier correlators G(qmin) at every
                                     1000th update of a bin (REWL_NUM_OBS_*_CORR
                                     layout).  0: off                                */
-    int32_t reserved[2];
+    int32_t densities;           /* 1: Ashkin-Teller sigma-tau density histograms at
+                                    every update (the AT_DENSITIES build,
+                                    ashkin_teller2d_hamiltonian.hpp:291-293); needs
+                                    observables >= 1 and Ashkin-Teller systems only  */
+    int32_t reserved;
 } rewl_params;
 
 /* Per-window bookkeeping, one record per (system, window). */
@@ -242,6 +255,12 @@ int rewl_get_rng_cursor(rewl_ctx* ctx, int32_t walker, uint64_t* cursor);
  * 1000), the number of samples.  nobs is REWL_NUM_OBS_ISING / _AT, or the
  * _CORR counts with observables = 2, for the walker's model. */
 int rewl_get_observables(rewl_ctx* ctx, int32_t walker, double* sums, int32_t nbins, int32_t nobs);
+/* Density histogram counts of one walker (densities on): out[b * 1024 + c] =
+ * number of updates in bin b whose (sigma/N, tau/N) fell in cell c
+ * (update_densities, ashkin_teller_densities.hpp:30-34).  The reference then
+ * normalises each bin by its largest cell and averages the replicas
+ * (transfer_and_normalize, density_averaging.cpp:9-28); that is host work. */
+int rewl_get_densities(rewl_ctx* ctx, int32_t walker, uint32_t* out, int32_t nbins);
 
 /* ---- state import (warm start, tests) ------------------------------------- */
 int rewl_set_logdos(rewl_ctx* ctx, int32_t walker, const double* lng, int32_t n);
@@ -287,6 +306,43 @@ enum rewl_disorder { REWL_DISORDER_UNIFORM = 0, REWL_DISORDER_GAUSSIAN = 1 };
  * on (u, v) redrawn while u <= DBL_EPSILON, then the cosine or the sine branch
  * by a third draw < 0.5).  Host only: needs no GPU. */
 int rewl_random_field(uint64_t seed, int32_t distribution, double strength, int32_t num_sites, double* out);
+
+/* ---- simulated annealing ------------------------------------------------- */
+/* Simulated_Annealer (simulated_annealer.hpp:24-194, SIMULATED_ANNEALING
+ * build): a ground-state search run before the REWL simulation, for systems
+ * whose disorder makes the all-up state a poor start.  Defaults in brackets
+ * are the reference's SA_Parameters (simulated_annealing_parameters.cxx). */
+typedef struct rewl_anneal_params {
+    double  initial_temperature; /* [25]                                           */
+    double  final_temperature;   /* [0.25]; T_k = Ti (Tf / Ti)^(k / num_temperatures) */
+    int32_t num_temperatures;    /* [250]; temperature steps k = 0 .. num_temperatures */
+    int32_t block_size;          /* [1000] sweeps between equilibration checks     */
+    double  stdev_tolerance;     /* [0.01] allowed relative change of the block
+                                    energy variance                                 */
+    double  frozen_criterion;    /* [0] a block stdev equal to it ends the step    */
+    int32_t max_blocks;          /* blocks per temperature at most (no counterpart:
+                                    the reference loops until equilibrated); >= 1  */
+    int32_t chains;              /* independent chains per system (the reference
+                                    runs one)                                       */
+} rewl_anneal_params;
+/* Anneal `chains` independent chains of every system on the GPU, one lane per
+ * chain.  Each chain starts from random spins (randomize_dofs) and sweeps the
+ * lattice (sigma pass then tau pass for AT, as the flavour of change_state
+ * alternates) with Metropolis updates: accept if E' < E, else if
+ * u < exp(beta (E - E')) (sa_update, simulated_annealer.hpp:66-86).  After
+ * each block of block_size sweeps the block's mean and standard deviation
+ * decide whether the temperature step is done (simulated_annealer.hpp:
+ * 129-158, simulated_annealing_utilities.hpp).  Random numbers: Philox4x32-10
+ * keyed by the system's seed, counter (attempt / 4, domain 4, chain), word
+ * attempt mod 4, u = word 2^-32; the start spins from domain 5 with one word
+ * per degree of freedom (u < 0.5: +1).
+ * Outputs per (system, chain), index system * chains + chain: the lowest
+ * energy met (min_energy_found), the final energy, the sweeps done; and, if
+ * final_spins is not NULL, the final configuration in the reference's
+ * spin_array layout (num_DoF int8 +-1 per chain, systems packed in order),
+ * which main.cpp:56-76 hands to the REWL walkers as their start. */
+int rewl_anneal(const rewl_system_desc* systems, int32_t num_systems, const rewl_anneal_params* params,
+                double* min_energy, double* final_energy, uint64_t* sweeps, int8_t* final_spins);
 
 /* ---- device self-check ----------------------------------------------------- */
 /* No counterpart in the reference.  what = 0: largest absolute error of the

@@ -45,6 +45,7 @@ def load(path=ORACLE_SO, build_if_missing=True):
         "ora_thermo_observables": ([C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int32,
                                     C.c_int32, C.c_double, C.c_double, C.c_int32, C.POINTER(C.c_double)], C.c_int),
         "ora_get_observables": ([P, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_double)], C.c_int),
+        "ora_get_densities": ([P, C.c_int32, C.POINTER(C.c_uint32)], C.c_int),
         "ora_get_spins": ([P, C.c_int32, C.POINTER(C.c_int8)], C.c_int),
         "ora_get_state": ([P, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_int32),
                            C.POINTER(C.c_uint64)], C.c_int),
@@ -179,6 +180,13 @@ class Oracle:
         self._chk(self.lib.ora_get_observables(self.ctx, int(g), means.ctypes.data_as(C.POINTER(C.c_double)),
                                                counts.ctypes.data_as(C.POINTER(C.c_double))))
         return means, counts
+
+    def densities(self, g):
+        """AT sigma-tau density counts [bin, 1024] of walker g (densities on)."""
+        n = self.layout(g)[3]
+        out = np.empty((n, 1024), dtype=np.uint32)
+        self._chk(self.lib.ora_get_densities(self.ctx, int(g), out.ctypes.data_as(C.POINTER(C.c_uint32))))
+        return out
 
     def spins(self, g):
         s = self.systems[self.layout(g)[0]]

@@ -27,15 +27,22 @@
  *
  * With rewl_params.observables it also keeps the reference's per-bin
  * observable running means and counts per bin (update_observables,
- * update_observable_average) and restates the observable part of the
- * canonical thermodynamics.
+ * update_observable_average), the Fourier correlator with observables = 2,
+ * with rewl_params.densities the AT sigma-tau density counts
+ * (update_densities), and restates the observable part of the canonical
+ * thermodynamics.
  *
- * Parity pins (tests/test_oracle.py, tests/test_observables.py):
+ * Parity pins (tests/test_oracle.py, tests/test_observables.py,
+ * tests/test_densities.py):
  * mt19937_64 standard KAT, Philox Random123 KATs, neighbour-table fixture of
  * the reference's own tester, closed-form energies, brute-force L=4 density
  * of states, exact Beale DoS, count conservation at the range test's slack,
  * microcanonical observables against brute force over all states (L=4
- * Ising, L=3 Ashkin-Teller), two- and three-level canonical closed forms.
+ * Ising, L=3 Ashkin-Teller), G(qmin) against a brute-force structure factor,
+ * density cells against exact rational cell indices and the brute-force
+ * microcanonical (sigma, tau) distribution of the 3x3 AT model, a pinned
+ * ground state for the cell orientation, two- and three-level canonical
+ * closed forms.
  */
 #include <algorithm>
 #include <atomic>
@@ -166,6 +173,7 @@ struct Walker {
     std::vector<double> obs;               /* running means [bin][ob] (observables on) */
     bool obs_corr = false;                 /* observables 2: with G(qmin)           */
     std::vector<double> obs_counts;        /* counts_per_bin, as the reference keeps it */
+    std::vector<uint32_t> dens;            /* AT sigma-tau density counts [bin][1024] (densities on) */
 };
 
 struct Window {
@@ -349,6 +357,20 @@ inline void corr_average(Walker& w, size_t bin, int nobs, int ob, double value, 
     avg = (value + n * avg) / (n + 1);
 }
 
+/* update_densities (ashkin_teller_densities.hpp:11-34 with the constants of
+ * ashkin_teller_densities_parameters.cxx:15-30): one count in the cell
+ * axis(tau) * axis_bins + axis(sigma) of the bin's 32 x 32 histogram. */
+inline uint32_t density_axis_index(double observable) {
+    const double sigma_tau_max = 1.05;
+    const double sigma_tau_min = -sigma_tau_max;
+    const double density_binwidth = (sigma_tau_max - sigma_tau_min) / (double)REWL_DENSITY_AXIS_BINS;
+    return (uint32_t)((observable - sigma_tau_min) / density_binwidth);
+}
+inline void update_densities(Walker& w, size_t bin, double sigma, double tau) {
+    const size_t idx = (size_t)density_axis_index(tau) * REWL_DENSITY_AXIS_BINS + density_axis_index(sigma);
+    ++w.dens[bin * REWL_DENSITY_CELLS + idx];
+}
+
 /* update_observables at the walker's current bin: Ising
  * ising2d_hamiltonian.hpp:234-256, Ashkin-Teller
  * ashkin_teller2d_hamiltonian.hpp:267-318; with correlators (observables 2)
@@ -387,6 +409,7 @@ inline void update_observables(const System& S, Walker& w) {
         obs_average(w, b, nobs, 3 * k + 0, bv);
         obs_average(w, b, nobs, 3 * k + 1, bv * bv);
         obs_average(w, b, nobs, 3 * k + 2, bv * bv * bv * bv);
+        if (!w.dens.empty()) update_densities(w, b, divide_N * w.Ms, divide_N * w.Mt);
         if (sample) {
             const double g_s = divide_N * divide_N * compute_correlator(S, w, 0, false);
             const double g_t = g_s;
@@ -656,6 +679,13 @@ int ora_create(const rewl_system_desc* systems, int32_t nsys, const rewl_params*
                     w.obs_corr = corr;
                     w.obs.assign(S.windows[k].nbins * (size_t)nobs, 0.0);
                     w.obs_counts.assign(S.windows[k].nbins, 0.0);
+                    if (c->p.densities) {
+                        if (S.d.model != REWL_ASHKIN_TELLER2D) {
+                            delete c;
+                            return fail(REWL_ERR_INVALID, "densities are an Ashkin-Teller observable");
+                        }
+                        w.dens.assign(S.windows[k].nbins * (size_t)REWL_DENSITY_CELLS, 0u);
+                    }
                 }
                 w.gen.seed(S.d.walker_seeds ? S.d.walker_seeds[w.local] : (uint64_t)1);
                 recalculate_state(S, w);
@@ -792,6 +822,14 @@ int ora_get_observables(const ora_ctx* c, int32_t walker, double* means, double*
     if (w.obs.empty()) return fail(REWL_ERR_INVALID, "observables are off");
     std::copy(w.obs.begin(), w.obs.end(), means);
     std::copy(w.obs_counts.begin(), w.obs_counts.end(), counts);
+    return REWL_OK;
+}
+
+int ora_get_densities(const ora_ctx* c, int32_t walker, uint32_t* out) {
+    if (!c || walker < 0 || walker >= (int32_t)c->w.size() || !out) return fail(REWL_ERR_INVALID, "bad argument");
+    const Walker& w = c->w[walker];
+    if (w.dens.empty()) return fail(REWL_ERR_INVALID, "densities are off");
+    std::copy(w.dens.begin(), w.dens.end(), out);
     return REWL_OK;
 }
 

@@ -71,7 +71,8 @@ class Params(C.Structure):
         ("average_mode", C.c_int32),
         ("sweep_kernel", C.c_int32),
         ("observables", C.c_int32),
-        ("reserved", C.c_int32 * 2),
+        ("densities", C.c_int32),
+        ("reserved", C.c_int32),
     ]
 
 
@@ -132,6 +133,8 @@ NUM_OBS_AT_CORR = 16
 DISORDER_UNIFORM = 0
 DISORDER_GAUSSIAN = 1
 NUM_OBS_AT = 12
+DENSITY_AXIS_BINS = 32
+DENSITY_CELLS = 1024
 
 SIGNATURES = {
     "rewl_last_error": ([], C.c_char_p),
@@ -158,6 +161,7 @@ SIGNATURES = {
     "rewl_get_flavor": ([_P, C.c_int32, C.POINTER(C.c_int32)], C.c_int),
     "rewl_get_rng_cursor": ([_P, C.c_int32, C.POINTER(C.c_uint64)], C.c_int),
     "rewl_get_observables": ([_P, C.c_int32, C.POINTER(C.c_double), C.c_int32, C.c_int32], C.c_int),
+    "rewl_get_densities": ([_P, C.c_int32, C.POINTER(C.c_uint32), C.c_int32], C.c_int),
     "rewl_thermo_observables": ([C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int32,
                                  C.c_int32, C.c_double, C.c_double, C.c_int32, C.POINTER(C.c_double)], C.c_int),
     "rewl_random_field": ([C.c_uint64, C.c_int32, C.c_double, C.c_int32, C.POINTER(C.c_double)], C.c_int),

@@ -30,6 +30,7 @@ REFERENCE_PARAMS = dict(
     average_mode=0,
     sweep_kernel=0,
     observables=0,
+    densities=0,
 )
 
 DEFAULT_SEED = 20240501

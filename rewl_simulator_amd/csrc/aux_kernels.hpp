@@ -257,6 +257,7 @@ __global__ void __launch_bounds__(kLanes) k_exchange(KArgs a, const int32_t* pai
                 const uint64_t* lt = a.lat + a.w_lat[gw[j]];
                 lattice_magnetisations(lt, S.L, S.model, m);
                 update_observables(a.obs + hi[j] * a.nobs, S.model, S.N, m, a.obs_corr != 0);
+                if (S.model == 1 && a.dens) update_density(a.dens, hi[j], S.N, m);
                 if (a.obs_corr) {
                     const uint32_t t = a.obs_tot[hi[j]];
                     a.obs_tot[hi[j]] = t + 1u;

@@ -76,6 +76,7 @@ struct KArgs {
     uint32_t* obs_tot;         // with correlators: updates so far per (bin, walker), hist layout
     int32_t nobs;              // observables per bin slot (3 / 4 Ising, 12 / 16 with any AT system)
     int32_t obs_corr;          // 1: the layout holds correlator slots
+    uint32_t* dens;            // AT sigma-tau densities [(hist index) * 1024 + cell], or null
     int32_t num_walkers, num_windows, num_groups;
 };
 
@@ -189,6 +190,23 @@ __device__ __forceinline__ void update_observables(double* obs, int model, int N
         obs_add(obs + 3 * k + 1, bv * bv);
         obs_add(obs + 3 * k + 2, bv * bv * bv * bv);
     }
+}
+
+// update_densities (ashkin_teller_densities.hpp:17-34, called at
+// ashkin_teller2d_hamiltonian.hpp:291-293): the cell of (sigma/N, tau/N) in the
+// 32 x 32 grid over [-1.05, 1.05]^2, counted in the walker's histogram of
+// this bin.  The axis index truncates (x - min) / binwidth with the reference's
+// FP64 constants (min = -1.05, binwidth = 2.1 / 32, an exact quotient).
+__device__ __forceinline__ int density_axis(double x) {
+    constexpr double tmax = 1.05;
+    constexpr double tmin = -tmax;
+    constexpr double width = (tmax - tmin) / 32.;
+    return (int)(uint32_t)((x - tmin) / width);
+}
+__device__ __forceinline__ void update_density(uint32_t* dens, int64_t hidx, int N, const int m[3]) {
+    const double divide_N = 1. / (double)N;
+    const int cell = density_axis(divide_N * (double)m[1]) * 32 + density_axis(divide_N * (double)m[0]);
+    __hip_atomic_fetch_add(dens + hidx * 1024 + cell, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // G(qmin) of one +-1 field given as L bit rows (bit 1 = -1): the reference's

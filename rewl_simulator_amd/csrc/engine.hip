@@ -160,6 +160,7 @@ struct rewl_ctx {
     DevBuf<BoundEntry> d_bound_tab;
     DevBuf<double> d_obs;       // observable sums (observables on)
     DevBuf<uint32_t> d_obs_tot; // updates so far per (bin, walker) (observables with correlators)
+    DevBuf<uint32_t> d_dens;    // AT sigma-tau density counts [(hist index) * 1024 + cell] (densities on)
     int nobs = 0;
 
     // replay streams
@@ -195,7 +196,7 @@ void destroy_ctx(rewl_ctx* c) {
     c->d_w_hist.release(); c->d_w_lat.release(); c->d_win_iter.release(); c->d_win_sweeps.release();
     c->d_w_E.release(); c->d_win_lnf.release(); c->d_fields.release();
     c->d_w_cursor.release(); c->d_lat.release(); c->d_visits.release(); c->d_ring.release(); c->d_ring_filled.release();
-    c->d_exclude.release(); c->d_lng.release(); c->d_cnt.release(); c->d_exch.release(); c->d_bound_tab.release(); c->d_obs.release(); c->d_obs_tot.release();
+    c->d_exclude.release(); c->d_lng.release(); c->d_cnt.release(); c->d_exch.release(); c->d_bound_tab.release(); c->d_obs.release(); c->d_obs_tot.release(); c->d_dens.release();
     c->d_sys.release(); c->d_win.release(); c->d_grp.release();
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -434,8 +435,11 @@ int rewl_create(const rewl_system_desc* systems, int32_t num_systems, const rewl
     const rewl_params& P = *params;
     if (P.sweeps_per_check <= 0 || P.sweeps_per_exchange <= 0 || P.lnf_initial <= 0 ||
         (P.flatness_mode < 0 || P.flatness_mode > 3) || (P.schedule < 0 || P.schedule > 1) ||
-        (P.rng_mode < 0 || P.rng_mode > 1) || (P.adjust_mode < 0 || P.adjust_mode > 1) || (P.average_mode < 0 || P.average_mode > 1) || (P.sweep_kernel < 0 || P.sweep_kernel > 4) || (P.observables < 0 || P.observables > 2) || P.walkers_per_group < 0 || P.walkers_per_group > kLanes)
+        (P.rng_mode < 0 || P.rng_mode > 1) || (P.adjust_mode < 0 || P.adjust_mode > 1) || (P.average_mode < 0 || P.average_mode > 1) || (P.sweep_kernel < 0 || P.sweep_kernel > 4) || (P.observables < 0 || P.observables > 2) || P.walkers_per_group < 0 || P.walkers_per_group > kLanes ||
+        (P.densities < 0 || P.densities > 1))
         return fail(REWL_ERR_INVALID, "invalid rewl_params");
+    if (P.densities && !P.observables)
+        return fail(REWL_ERR_INVALID, "densities are recorded with the observables: set observables >= 1");
 
     rewl_ctx* c = new rewl_ctx();
     c->p = P;
@@ -735,8 +739,11 @@ int rewl_create(const rewl_system_desc* systems, int32_t num_systems, const rewl
         hipMemsetAsync(c->d_visits.p, 0, c->hist_total * 8, c->stream) != hipSuccess)
         return bail(fail(REWL_ERR_DEVICE, "hipMemset failed"));
     if (P.observables) {
-        bool any_at = false;
-        for (const auto& S : c->sys) any_at = any_at || S.d.model == REWL_ASHKIN_TELLER2D;
+        bool any_at = false, any_ising = false;
+        for (const auto& S : c->sys) {
+            any_at = any_at || S.d.model == REWL_ASHKIN_TELLER2D;
+            any_ising = any_ising || S.d.model == REWL_ISING2D;
+        }
         const bool corr = P.observables == 2;
         c->nobs = any_at ? (corr ? REWL_NUM_OBS_AT_CORR : REWL_NUM_OBS_AT)
                          : (corr ? REWL_NUM_OBS_ISING_CORR : REWL_NUM_OBS_ISING);
@@ -748,6 +755,14 @@ int rewl_create(const rewl_system_desc* systems, int32_t num_systems, const rewl
             if (c->d_obs_tot.alloc(c->hist_total) != hipSuccess)
                 return bail(fail(REWL_ERR_DEVICE, "hipMalloc failed for observables"));
             if (hipMemsetAsync(c->d_obs_tot.p, 0, c->hist_total * 4, c->stream) != hipSuccess)
+                return bail(fail(REWL_ERR_DEVICE, "hipMemset failed"));
+        }
+        if (P.densities) {
+            if (any_ising)
+                return bail(fail(REWL_ERR_INVALID, "densities are an Ashkin-Teller observable (every system must be AT)"));
+            const size_t nd = c->hist_total * (size_t)REWL_DENSITY_CELLS;
+            if (c->d_dens.alloc(nd) != hipSuccess) return bail(fail(REWL_ERR_DEVICE, "hipMalloc failed for densities"));
+            if (hipMemsetAsync(c->d_dens.p, 0, nd * 4, c->stream) != hipSuccess)
                 return bail(fail(REWL_ERR_DEVICE, "hipMemset failed"));
         }
     }
@@ -785,6 +800,7 @@ int rewl_create(const rewl_system_desc* systems, int32_t num_systems, const rewl
     a.win_oot = c->d_win_oot.p; a.win_done = c->d_win_done.p; a.exclude = c->d_exclude.p;
     a.lat = c->d_lat.p; a.lng = c->d_lng.p; a.cnt = c->d_cnt.p; a.visits = c->d_visits.p;
     a.obs = c->d_obs.p; a.nobs = c->nobs; a.obs_tot = c->d_obs_tot.p; a.obs_corr = P.observables == 2 ? 1 : 0;
+    a.dens = c->d_dens.p;
     a.ring = c->d_ring.p; a.ring_filled = c->d_ring_filled.p; a.ring_cap = c->ring_cap;
     a.err = c->d_err.p; a.exch_accepted = c->d_exch.p; a.bound_tab = c->d_bound_tab.p;
     a.num_walkers = W; a.num_windows = (int)nwin; a.num_groups = (int)c->groups.size();
@@ -1016,6 +1032,18 @@ int rewl_get_observables(rewl_ctx* c, int32_t g, double* sums, int32_t n, int32_
     if (n < nb) return fail(REWL_ERR_INVALID, "output buffer too small");
     HIP_TRY(hipMemcpy2DAsync(sums, (size_t)nobs * sizeof(double), c->d_obs.p + c->w_hist[g] * c->nobs,
                              (size_t)kLanes * c->nobs * sizeof(double), (size_t)nobs * sizeof(double), (size_t)nb,
+                             hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return REWL_OK;
+}
+
+int rewl_get_densities(rewl_ctx* c, int32_t g, uint32_t* out, int32_t n) {
+    if (!walker_ok(c, g) || !out) return fail(REWL_ERR_INVALID, "bad walker");
+    if (!c->d_dens.p) return fail(REWL_ERR_INVALID, "densities are off (rewl_params.densities)");
+    const int nb = c->win[c->w_window[g]].nbins;
+    if (n < nb) return fail(REWL_ERR_INVALID, "output buffer too small");
+    const size_t row = (size_t)REWL_DENSITY_CELLS * sizeof(uint32_t);
+    HIP_TRY(hipMemcpy2DAsync(out, row, c->d_dens.p + c->w_hist[g] * REWL_DENSITY_CELLS, row * kLanes, row, (size_t)nb,
                              hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     return REWL_OK;

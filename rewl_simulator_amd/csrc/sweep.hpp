@@ -172,6 +172,7 @@ __device__ void sweep_walker(const KArgs& a, const DevGroup& G, const DevSystem&
     // with OBS = 2 the correlators when the bin's count before it is 0 mod 1000
     auto observe = [&](int b, int y, const uint64_t* rows) {
         update_observables(obs + b * obs_bin, MODEL, S.N, mag, OBS == 2);
+        if (MODEL == 1 && a.dens) update_density(a.dens, hb + (int64_t)b * kLanes, S.N, mag);
         if (OBS == 2) {
             const uint32_t t = s_tot[b * ls + lane];
             s_tot[b * ls + lane] = t + 1u;

@@ -135,7 +135,6 @@ __device__ __forceinline__ void sweep_walker_tuned(const KArgs& a, const DevGrou
     constexpr int NW2 = (L + 15) / 16;   // words of 2-bit fields per row
     constexpr int NP = MODEL == 0 ? 1 : 2;
     constexpr int NPASS = MODEL == 0 ? 1 : 2;
-    constexpr int N = L * L;
     const bool active = lane < G.nwalkers;
     const int g = G.first_walker + (active ? lane : 0);
     const int wi = a.w_window[g];

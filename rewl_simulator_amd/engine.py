@@ -217,6 +217,14 @@ class Engine:
                 means[:, c] = tot[:, c] / samples
             return s["e_min"] + s["bin_size"] * np.arange(nb), means
 
+    def densities(self, g):
+        """Ashkin-Teller sigma-tau density counts [bin, 1024] of walker g
+        (densities on): cell (tau axis) * 32 + (sigma axis)."""
+        n = self._nbins(g)
+        out = np.empty((n, abi.DENSITY_CELLS), dtype=np.uint32)
+        abi.check(self.lib.rewl_get_densities(self.ctx, int(g), out.ctypes.data_as(C.POINTER(C.c_uint32)), n))
+        return out
+
     def rng_cursor(self, g):
         c = C.c_uint64()
         abi.check(self.lib.rewl_get_rng_cursor(self.ctx, int(g), C.byref(c)))

@@ -117,6 +117,7 @@ struct REWL_Parameters {
     int average_mode = 0;                        // 1: equal-weight replica average
     int observables = 0;                         // 1: per-bin observables (update_observables),
                                                  // 2: also the Fourier correlator G(qmin)
+    bool densities = false;                      // AT sigma-tau density plots (AT_DENSITIES)
 };
 
 // generate_random_field (random_fields.hpp:41-75) with an explicit seed in
@@ -165,6 +166,7 @@ class REWL_simulation {
         p.adjust_mode = rp.adjust_mode;
         p.average_mode = rp.average_mode;
         p.observables = rp.observables;
+        p.densities = rp.densities ? 1 : 0;
         check(rewl_create(&d, 1, &p, &ctx_));
         check(rewl_adjust_to_range(ctx_));
     }
@@ -262,6 +264,54 @@ class REWL_simulation {
                 for (int o = 0; o < nobs; ++o)
                     means[(size_t)b * nobs + o] =
                         sum[(size_t)b * nobs + o] / (corr && o % 4 == 3 ? smp[(size_t)b] : cnt[(size_t)b]);
+        }
+    }
+
+    // Density plots of the system (densities on), one 1024-cell plot per bin
+    // from the lowest window's first bin: each walker's bin divided by its
+    // largest cell (transfer_and_normalize, ashkin_teller_densities.hpp:36-57;
+    // 0/0 gives NaN there too), the replicas summed and scaled by 1/R
+    // (density_averaging.cpp:9-28), and overlapping windows averaged with
+    // equal weights in window order (concatenate_rewl_tables.hpp:261-263).
+    void density_plots(std::vector<std::vector<double>>& plots) const {
+        int32_t nw = 0;
+        check(rewl_num_windows(ctx_, &nw));
+        rewl_window_info w0{};
+        check(rewl_get_window_info(ctx_, 0, &w0));
+        const int R = rp_.replicas_per_window;
+        plots.clear();
+        std::vector<int> overlapping;
+        for (int32_t k = 0; k < nw; ++k) {
+            rewl_window_info w{};
+            check(rewl_get_window_info(ctx_, k, &w));
+            const size_t nb = (size_t)w.num_bins;
+            std::vector<uint32_t> c(nb * REWL_DENSITY_CELLS);
+            std::vector<double> avg(nb * REWL_DENSITY_CELLS, 0.0);
+            for (int r = 0; r < R; ++r) {
+                check(rewl_get_densities(ctx_, w.first_walker + r, c.data(), w.num_bins));
+                for (size_t b = 0; b < nb; ++b) {
+                    const uint32_t* cb = c.data() + b * REWL_DENSITY_CELLS;
+                    uint32_t mx = 0;
+                    for (int i = 0; i < REWL_DENSITY_CELLS; ++i) mx = std::max(mx, cb[i]);
+                    for (int i = 0; i < REWL_DENSITY_CELLS; ++i)
+                        avg[b * REWL_DENSITY_CELLS + i] += (double)cb[i] / (double)mx;
+                }
+            }
+            if (R > 1)
+                for (double& v : avg) v *= 1. / (double)R;
+            const size_t g0 = (size_t)std::llround((w.e_min - w0.e_min) / w0.bin_size);
+            for (size_t b = 0; b < nb; ++b) {
+                const double* src = avg.data() + b * REWL_DENSITY_CELLS;
+                if (g0 + b < plots.size()) {
+                    const double n = (double)overlapping[g0 + b];
+                    for (int i = 0; i < REWL_DENSITY_CELLS; ++i)
+                        plots[g0 + b][(size_t)i] = (1. / (1. + n)) * ((1. * n) * plots[g0 + b][(size_t)i] + src[i]);
+                    ++overlapping[g0 + b];
+                } else {
+                    plots.emplace_back(src, src + REWL_DENSITY_CELLS);
+                    overlapping.push_back(1);
+                }
+            }
         }
     }
 

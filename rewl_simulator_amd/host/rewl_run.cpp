@@ -4,11 +4,13 @@
 // Ashkin-Teller, Ising otherwise, as model_hamiltonians.hpp chooses by macro)
 // and prints the normalised microcanonical ln g/N per energy bin.
 //
-//   rewl_run L windows replicas final_increment [average_mode] [observables]
+//   rewl_run L windows replicas final_increment [average_mode] [observables] [densities]
 //
 // With observables = 1 it also prints the per-bin means of the model's
 // observables (E/N, then one column per observable); with observables = 2
-// the Fourier correlator G(qmin) columns as well.
+// the Fourier correlator G(qmin) columns as well.  With densities = 1
+// (Ashkin-Teller build, observables >= 1) it prints the sigma-tau density plot
+// of every bin, 1024 values a line (the reference's AT_DENSITIES output).
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -33,6 +35,7 @@ int main(int argc, char** argv) {
     rp.final_increment = argc > 4 ? std::atof(argv[4]) : 1e-4;
     rp.average_mode = argc > 5 ? std::atoi(argv[5]) : 0;
     rp.observables = argc > 6 ? std::atoi(argv[6]) : 0;
+    rp.densities = argc > 7 && std::atoi(argv[7]) != 0;
     try {
         rewl_host::REWL_simulation<Hamiltonian_t<double>> sim(sp, rp);
         const bool done = sim.simulate(1000000);
@@ -53,6 +56,15 @@ int main(int argc, char** argv) {
                 if (std::isnan(means[i * nobs])) continue;
                 std::printf("%.15e", Eo[i] / sp.N());
                 for (int o = 0; o < nobs; ++o) std::printf(",%.15e", means[i * nobs + o]);
+                std::printf("\n");
+            }
+        }
+        if (rp.densities) {
+            std::vector<std::vector<double>> plots;
+            sim.density_plots(plots);
+            std::printf("# density plots: %zu bins x %d cells\n", plots.size(), REWL_DENSITY_CELLS);
+            for (const auto& plot : plots) {
+                for (size_t i = 0; i < plot.size(); ++i) std::printf(i ? ",%.15e" : "%.15e", plot[i]);
                 std::printf("\n");
             }
         }

@@ -182,3 +182,83 @@ def test_correlators_match_oracle(gpu, model, rng):
         sampled += int((n > 1000).sum())
     assert sampled > 0   # some bins took more than one correlator sample
     eng.close(); ora.close()
+
+
+@pytest.mark.parametrize("rng", [abi.RNG_REPLAY, abi.RNG_PHILOX])
+@pytest.mark.parametrize("field", [False, True])
+@pytest.mark.parametrize("corr", [1, 2])
+def test_densities_match_oracle(gpu, corr, field, rng):
+    """densities = 1: the AT sigma-tau histograms of every update, in sweeps
+    and after exchanges, bit-exact against the oracle's restatement of
+    update_densities; every update lands in one cell."""
+    print(f"case: densities observables={corr} field={field} rng={rng}", flush=True)
+    systems = _systems(abi.ASHKIN_TELLER2D, field)
+    p = configs.params(rng_mode=rng, lnf_initial=0.05, observables=corr, densities=1, sweeps_per_exchange=20,
+                       adjust_mode=1)
+    eng, ora = Engine(systems, p), Oracle(systems, p)
+    eng.adjust_to_range(); ora.adjust_to_range()
+    for _ in range(4):
+        eng.sweeps(20); ora.sweeps(20)
+        eng.exchange(); ora.exchange()
+    eng.check(); ora.check()
+    assert_same(eng, ora, "densities on", cursors=(rng == abi.RNG_REPLAY))
+    for g in range(eng.num_walkers):
+        d = eng.densities(g)
+        np.testing.assert_array_equal(d, ora.densities(g), err_msg=f"walker {g}")
+        np.testing.assert_array_equal(d.sum(axis=1), eng.counts(g).astype(np.uint64) + eng.visits(g))
+    eng.close(); ora.close()
+
+
+def test_density_orientation(gpu):
+    """A walker held in the ground state sigma = +1, tau = -1 by a one-bin
+    window counts every update in cell 0 * 32 + 31 (sigma is the column)."""
+    L, N = 4, 16
+    s = configs.at_system(L, -5.0 * N, -5.0 * N + 4.0, 4.0, 1, 1)
+    p = configs.params(lnf_initial=0.5, observables=1, densities=1, exchange=0, adjust_mode=1)
+    with Engine([s], p) as eng:
+        eng.set_spins(0, np.tile(np.array([1, -1], np.int8), N))
+        eng.adjust_to_range()
+        eng.sweeps(3)
+        d = eng.densities(0)
+    assert d.shape == (1, 1024) and d[0, 31] == 3 * 2 * N and d[0].sum() == 3 * 2 * N
+
+
+def test_densities_refused_without_observables_or_for_ising(gpu):
+    with pytest.raises(abi.REWLError):
+        Engine(_systems(abi.ASHKIN_TELLER2D, False), configs.params(densities=1))
+    with pytest.raises(abi.REWLError):
+        Engine(_systems(abi.ISING2D, False), configs.params(observables=1, densities=1))
+    with Engine(_systems(abi.ASHKIN_TELLER2D, False), configs.params(observables=1)) as eng:
+        with pytest.raises(abi.REWLError):
+            eng.densities(0)
+
+
+def test_density_plots_from_a_run(gpu, tmp_path):
+    """The reference's density-plot files from an AT run: one file per bin of
+    the concatenated windows, every value in [0, 1] (each walker's bin is
+    normalised to a largest cell of 1 before the averages), the ground bin
+    only in the four corner cells."""
+    from rewl_simulator_amd import densities as D, output
+    L, N = 8, 64
+    s = configs.at_system(L, -5.0 * N, -2.0 * N, 4.0, 3, 2)
+    p = configs.params(observables=1, densities=1, flatness_mode=abi.FLAT_MIN_OVER_MEAN, flatness=0.8,
+                       schedule=abi.SCHED_HALVE, lnf_initial=0.1, sweeps_per_check=200, sweeps_per_exchange=200,
+                       adjust_mode=1)
+    with Engine([s], p) as eng:
+        eng.adjust_to_range()
+        for _ in range(10):
+            eng.sweeps(200); eng.exchange(); eng.check()
+        plots = D.system_densities(eng)
+        nb_sys = int(round((s["e_max"] - s["e_min"]) / s["bin_size"]))
+        first = eng.windows[0]
+    assert plots.shape == (nb_sys, 1024) and first["e_min"] == s["e_min"]
+    seen = ~np.isnan(plots).any(axis=1)
+    assert seen.sum() > 10
+    assert np.all((plots[seen] >= 0) & (plots[seen] <= 1)) and np.all(plots[seen].max(axis=1) > 0)
+    ground = plots[0]
+    if not np.isnan(ground).any():
+        assert np.all(ground[np.setdiff1d(np.arange(1024), [0, 31, 992, 1023])] == 0)
+    _, base, _ = output.system_strings(s)
+    files = D.write_density_plots(str(tmp_path / "Density_Plots"), plots, base)
+    assert len(files) == nb_sys
+    assert np.loadtxt(files[0], comments="#").shape == (32, 32)

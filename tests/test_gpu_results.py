@@ -101,6 +101,24 @@ def test_host_program_observables(gpu, mode):
     assert np.all((obs[:, 1:] >= -1e-12) & (obs[:, 1:] <= 1))
 
 
+def test_host_program_density_plots(gpu):
+    """The C++ host layer's density plots (Ashkin-Teller build, densities on):
+    one 1024-cell plot per bin, values in [0, 1] where every replica visited
+    the bin, the ground bin only in the four corner cells (sigma, tau = +-1)."""
+    exe = os.path.join(ROOT, "rewl_simulator_amd", "host", "rewl_run_at")
+    out = subprocess.run([exe, "6", "2", "2", "1e-3", "1", "1", "1"], capture_output=True, text=True, timeout=600)
+    assert out.returncode == 0, out.stderr
+    lines = out.stdout.splitlines()
+    k = next(i for i, l in enumerate(lines) if l.startswith("# density plots"))
+    plots = np.array([[float(v) for v in l.split(",")] for l in lines[k + 1:] if l])
+    nb = int(lines[k].split()[3])
+    assert plots.shape == (nb, 1024) and nb > 10
+    seen = ~np.isnan(plots).any(axis=1)
+    assert seen.sum() > 10
+    assert np.all((plots[seen] >= 0) & (plots[seen] <= 1)) and np.all(plots[seen].max(axis=1) > 0)
+    assert seen[0] and np.all(plots[0][np.setdiff1d(np.arange(1024), [0, 31, 992, 1023])] == 0)
+
+
 def test_thermodynamics_kernel_matches_oracle(gpu):
     """GPU FP64 trapezoid thermodynamics against the oracle's restatement of
     Thermodynamics::calculate_thermodynamics on the exact L=16 ln g."""
