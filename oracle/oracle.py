@@ -46,6 +46,8 @@ def load(path=ORACLE_SO, build_if_missing=True):
                                     C.c_int32, C.c_double, C.c_double, C.c_int32, C.POINTER(C.c_double)], C.c_int),
         "ora_get_observables": ([P, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_double)], C.c_int),
         "ora_get_densities": ([P, C.c_int32, C.POINTER(C.c_uint32)], C.c_int),
+        "ora_anneal": ([P, C.c_int32, P, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_uint64),
+                        C.POINTER(C.c_int8)], C.c_int),
         "ora_get_spins": ([P, C.c_int32, C.POINTER(C.c_int8)], C.c_int),
         "ora_get_state": ([P, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_int32),
                            C.POINTER(C.c_uint64)], C.c_int),
@@ -236,6 +238,17 @@ class Oracle:
         sec, att = C.c_double(), C.c_uint64()
         self._chk(self.lib.ora_bench_sweeps(self.ctx, int(sweeps), int(threads), C.byref(sec), C.byref(att)))
         return sec.value, att.value
+
+
+def anneal(systems, params, spins=True):
+    """The oracle's restatement of Simulated_Annealer::simulate_annealing, with
+    the engine's marshalling (rewl_simulator_amd.engine.anneal)."""
+    from rewl_simulator_amd.engine import _anneal_call
+    lib = load()
+    rc, out = _anneal_call(lib.ora_anneal, systems, params, spins)
+    if rc:
+        raise RuntimeError(lib.ora_last_error().decode())
+    return out
 
 
 def neighbors(L, site):

@@ -53,6 +53,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <limits>
 #include <random>
 #include <string>
 #include <thread>
@@ -630,6 +631,111 @@ void exchange_system(ora_ctx& c, System& S, bool even) {
 extern "C" {
 
 const char* ora_last_error(void) { return g_err.c_str(); }
+
+/* Simulated_Annealer::simulate_annealing (simulated_annealer.hpp:92-194) for
+ * `chains` independent chains of every system: randomize_dofs
+ * (ising2d_hamiltonian.hpp:136-150, ashkin_teller2d_hamiltonian.hpp:146-155)
+ * from Philox (ctr {j / 4, 0, 5, chain}, word j mod 4, u < 0.5 gives +1),
+ * then per temperature step k = 0..num_temperatures at
+ * T = Ti (Tf / Ti)^(k / num) (calculate_temperature, :18-22) blocks of
+ * block_size sweeps of sa_update (:66-86; Philox ctr {lo(a / 4), hi(a / 4),
+ * 4, chain}, word a mod 4 for attempt a, u = word 2^-32) until the block
+ * statistics settle (:129-158, simulated_annealing_utilities.hpp), or
+ * max_blocks blocks.  min_energy_found starts at +inf (uninitialised there). */
+inline bool sa_not_within_error(double value, double avg, double stdev) {
+    return !(avg - stdev <= value && value <= avg + stdev);
+}
+/* The reference's unqualified abs on a double resolves to ::abs(int) with the
+ * headers it includes: the difference is truncated to int; out of int's range
+ * the x86 conversion gives INT_MIN. */
+inline bool sa_not_within_tolerance(double new_v, double old_v, double tol) {
+    const double d = new_v - old_v;
+    const double a = std::fabs(d) < 2147483648.0 ? (double)std::abs((int)d) : -2147483648.0;
+    return a / old_v > tol;
+}
+inline void sa_array_statistics(const std::vector<double>& e, double* average, double* stdev) {
+    *average = 0.;
+    *stdev = 0.;
+    for (double v : e) *average += v;
+    *average /= (double)e.size();
+    for (double v : e) *stdev += (v - *average) * (v - *average);
+    *stdev = std::sqrt(*stdev / (double)e.size());
+}
+
+int ora_anneal(const rewl_system_desc* systems, int32_t nsys, const rewl_anneal_params* p, double* min_energy,
+               double* final_energy, uint64_t* sweeps, int8_t* final_spins) {
+    if (!systems || nsys < 1 || !p || p->num_temperatures < 1 || p->block_size < 1 || p->max_blocks < 1 || p->chains < 1)
+        return fail(REWL_ERR_INVALID, "bad argument");
+    int8_t* so = final_spins;
+    for (int si = 0; si < nsys; ++si) {
+        System S;
+        S.d = systems[si];
+        S.L = S.d.L; S.N = S.L * S.L;
+        S.ndof = S.d.model == REWL_ISING2D ? S.N : 2 * S.N;
+        if (S.d.field) S.field.assign(S.d.field, S.d.field + S.N);
+        S.has_field = S.d.field != nullptr || S.d.h != 0.0;
+        build_neighbors(S);
+        const uint32_t k0 = (uint32_t)S.d.seed, k1 = (uint32_t)(S.d.seed >> 32);
+        const int flavors = S.ndof / S.N;
+        for (int ch = 0; ch < p->chains; ++ch) {
+            Walker w;
+            w.s.assign((size_t)S.ndof, 1.0);
+            for (int j = 0; j < S.ndof; ++j) {
+                const P4 r = philox4x32_10(P4{{(uint32_t)j >> 2, 0u, 5u, (uint32_t)ch}}, k0, k1);
+                const double u = (double)r.v[j & 3] * 0x1p-32;
+                w.s[(size_t)j] = u < 0.5 ? 1. : -1.;
+            }
+            recalculate_state(S, w);
+            double min_found = std::numeric_limits<double>::infinity();
+            uint64_t a = 0, total = 0;
+            std::vector<double> equil((size_t)p->block_size);
+            for (int k = 0; k <= p->num_temperatures; ++k) {
+                const double T = p->initial_temperature *
+                                 std::pow(p->final_temperature / p->initial_temperature,
+                                          (double)k / (double)p->num_temperatures);
+                const double beta = 1. / T;
+                uint64_t sweep = 0;
+                double mean = DBL_MAX, sd = 1.;
+                bool incomplete = true;
+                int blocks = 0;
+                while (incomplete) {
+                    for (int b = 0; b < p->block_size; ++b) {
+                        for (int f = 0; f < flavors; ++f)
+                            for (int site = 0; site < S.N; ++site) {
+                                const Proposal t = change_state(S, w, site);
+                                double paccept = t.E < w.E ? 1. : -1.;
+                                if (paccept == -1.) paccept = std::exp(beta * (w.E - t.E));
+                                const P4 r = philox4x32_10(P4{{(uint32_t)(a >> 2), (uint32_t)(a >> 34), 4u, (uint32_t)ch}},
+                                                           k0, k1);
+                                const double u = (double)r.v[a & 3] * 0x1p-32;
+                                ++a;
+                                if (u < paccept) set_state(S, w, site, t);
+                                min_found = w.E < min_found ? w.E : min_found;
+                            }
+                        equil[(size_t)b] = w.E;
+                    }
+                    double avg = 0., sdn = 0.;
+                    sa_array_statistics(equil, &avg, &sdn);
+                    incomplete = sa_not_within_error(avg, mean, sd);
+                    incomplete = incomplete || sa_not_within_tolerance(sdn * sdn, sd * sd, p->stdev_tolerance);
+                    incomplete = (sdn != p->frozen_criterion) && incomplete;
+                    mean = avg;
+                    sd = sdn;
+                    sweep += (uint64_t)p->block_size;
+                    if (++blocks >= p->max_blocks) incomplete = false;
+                }
+                total += sweep;
+            }
+            const size_t o = (size_t)si * (size_t)p->chains + (size_t)ch;
+            min_energy[o] = min_found;
+            final_energy[o] = w.E;
+            sweeps[o] = total;
+            if (so)
+                for (int j = 0; j < S.ndof; ++j) *so++ = w.s[(size_t)j] > 0 ? 1 : -1;
+        }
+    }
+    return REWL_OK;
+}
 
 int ora_create(const rewl_system_desc* systems, int32_t nsys, const rewl_params* params, ora_ctx** out) {
     if (!systems || nsys <= 0 || !params || !out) return fail(REWL_ERR_INVALID, "null argument");

@@ -76,6 +76,19 @@ class Params(C.Structure):
     ]
 
 
+class AnnealParams(C.Structure):
+    _fields_ = [
+        ("initial_temperature", C.c_double),
+        ("final_temperature", C.c_double),
+        ("num_temperatures", C.c_int32),
+        ("block_size", C.c_int32),
+        ("stdev_tolerance", C.c_double),
+        ("frozen_criterion", C.c_double),
+        ("max_blocks", C.c_int32),
+        ("chains", C.c_int32),
+    ]
+
+
 class WindowInfo(C.Structure):
     _fields_ = [
         ("system", C.c_int32),
@@ -162,6 +175,8 @@ SIGNATURES = {
     "rewl_get_rng_cursor": ([_P, C.c_int32, C.POINTER(C.c_uint64)], C.c_int),
     "rewl_get_observables": ([_P, C.c_int32, C.POINTER(C.c_double), C.c_int32, C.c_int32], C.c_int),
     "rewl_get_densities": ([_P, C.c_int32, C.POINTER(C.c_uint32), C.c_int32], C.c_int),
+    "rewl_anneal": ([C.POINTER(SystemDesc), C.c_int32, C.POINTER(AnnealParams), C.POINTER(C.c_double),
+                     C.POINTER(C.c_double), C.POINTER(C.c_uint64), C.POINTER(C.c_int8)], C.c_int),
     "rewl_thermo_observables": ([C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int32,
                                  C.c_int32, C.c_double, C.c_double, C.c_int32, C.POINTER(C.c_double)], C.c_int),
     "rewl_random_field": ([C.c_uint64, C.c_int32, C.c_double, C.c_int32, C.POINTER(C.c_double)], C.c_int),

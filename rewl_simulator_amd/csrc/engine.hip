@@ -25,6 +25,7 @@
 #include "sweep_fast.hpp"
 #include "sweep_field.hpp"
 #include "sweep_tuned.hpp"
+#include "anneal.hpp"
 
 using namespace rewl;
 
@@ -410,6 +411,18 @@ int max_ndof(const rewl_ctx* c) {
     for (auto& s : c->sys) m = std::max(m, s.ndof);
     return m;
 }
+
+// Device buffers of one rewl_anneal call, released on every return path.
+struct AnnealBuffers {
+    DevBuf<DevSystem> sys;
+    DevBuf<double> fields, beta, ebuf, emin, efin;
+    DevBuf<uint64_t> lat;
+    DevBuf<unsigned long long> sweeps;
+    ~AnnealBuffers() {
+        sys.release(); fields.release(); beta.release(); ebuf.release(); emin.release(); efin.release();
+        lat.release(); sweeps.release();
+    }
+};
 
 }  // namespace
 
@@ -1313,6 +1326,93 @@ int rewl_stitch_overlap(rewl_ctx* c, int32_t si, double states_per_site, double*
     for (int i = 0; i < nb_total && i < capacity; ++i) {
         if (energies) energies[i] = S.d.e_min + S.d.bin_size * (double)i;
         if (logdos) logdos[i] = lng[(size_t)i];
+    }
+    return REWL_OK;
+}
+
+int rewl_anneal(const rewl_system_desc* systems, int32_t num_systems, const rewl_anneal_params* params,
+                double* min_energy, double* final_energy, uint64_t* sweeps, int8_t* final_spins) {
+    if (!systems || num_systems < 1 || !params || !min_energy || !final_energy || !sweeps)
+        return fail(REWL_ERR_INVALID, "bad argument");
+    const rewl_anneal_params& P = *params;
+    if (!(P.initial_temperature > 0) || !(P.final_temperature > 0) || P.num_temperatures < 1 || P.block_size < 1 ||
+        P.max_blocks < 1 || P.chains < 1 || !(P.stdev_tolerance >= 0))
+        return fail(REWL_ERR_INVALID, "invalid rewl_anneal_params");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+        return fail(REWL_ERR_NO_DEVICE, "no HIP device visible: the annealer runs only on the GPU");
+    const int gps = (P.chains + kLanes - 1) / kLanes;
+    const size_t ngroups = (size_t)num_systems * (size_t)gps;
+    const size_t ebuf_n = ngroups * (size_t)P.block_size * kLanes;
+    if (ebuf_n * sizeof(double) > ((size_t)16 << 30))
+        return fail(REWL_ERR_INVALID, "block energies of all chains exceed 16 GiB: fewer chains or a smaller block");
+    std::vector<DevSystem> ds((size_t)num_systems);
+    std::vector<double> fields;
+    std::vector<size_t> foff((size_t)num_systems, (size_t)-1);
+    for (int si = 0; si < num_systems; ++si) {
+        const rewl_system_desc& d = systems[si];
+        if (d.model != REWL_ISING2D && d.model != REWL_ASHKIN_TELLER2D) return fail(REWL_ERR_INVALID, "unknown model");
+        if (d.L < 2 || d.L > 64) return fail(REWL_ERR_INVALID, "L must lie in [2, 64]");
+        DevSystem& D = ds[(size_t)si];
+        D.model = d.model; D.L = d.L; D.N = d.L * d.L; D.ndof = d.model == REWL_ISING2D ? D.N : 2 * D.N;
+        D.has_field = d.field != nullptr || d.h != 0.0;
+        D.J = d.J; D.K = d.K; D.h = d.h;
+        D.k0 = (uint32_t)d.seed; D.k1 = (uint32_t)(d.seed >> 32);
+        if (d.field) {
+            foff[(size_t)si] = fields.size();
+            fields.insert(fields.end(), d.field, d.field + D.N);
+        }
+    }
+    // the temperature schedule on the host, as calculate_temperature computes it
+    std::vector<double> beta((size_t)P.num_temperatures + 1);
+    for (int k = 0; k <= P.num_temperatures; ++k) {
+        const double T = P.initial_temperature * pow(P.final_temperature / P.initial_temperature,
+                                                      (double)k / (double)P.num_temperatures);
+        beta[(size_t)k] = 1. / T;
+    }
+    AnnealBuffers B;
+    const hipStream_t st = nullptr;
+    int rc;
+    if ((rc = upload(B.fields, fields, st))) return rc;
+    for (int si = 0; si < num_systems; ++si)
+        if (foff[(size_t)si] != (size_t)-1) ds[(size_t)si].field = B.fields.p + foff[(size_t)si];
+    if ((rc = upload(B.sys, ds, st))) return rc;
+    if ((rc = upload(B.beta, beta, st))) return rc;
+    const size_t nout = (size_t)num_systems * (size_t)P.chains;
+    HIP_TRY(B.lat.alloc(ngroups * (size_t)kAnnealGroupWords));
+    HIP_TRY(B.ebuf.alloc(ebuf_n));
+    HIP_TRY(B.emin.alloc(nout));
+    HIP_TRY(B.efin.alloc(nout));
+    HIP_TRY(B.sweeps.alloc(nout));
+    AnnealArgs a{};
+    a.sys = B.sys.p; a.lat = B.lat.p; a.ebuf = B.ebuf.p; a.beta = B.beta.p;
+    a.num_temps = P.num_temperatures; a.block_size = P.block_size; a.max_blocks = P.max_blocks;
+    a.chains = P.chains; a.groups_per_sys = gps;
+    a.tol = P.stdev_tolerance; a.frozen = P.frozen_criterion;
+    a.min_energy = B.emin.p; a.final_energy = B.efin.p; a.sweeps = B.sweeps.p;
+    hipLaunchKernelGGL(k_anneal, dim3((unsigned)ngroups), dim3(kLanes), 0, st, a);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(st));
+    HIP_TRY(hipMemcpy(min_energy, B.emin.p, nout * sizeof(double), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(final_energy, B.efin.p, nout * sizeof(double), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(sweeps, B.sweeps.p, nout * sizeof(uint64_t), hipMemcpyDeviceToHost));
+    if (final_spins) {
+        std::vector<uint64_t> lat(ngroups * (size_t)kAnnealGroupWords);
+        HIP_TRY(hipMemcpy(lat.data(), B.lat.p, lat.size() * sizeof(uint64_t), hipMemcpyDeviceToHost));
+        int8_t* o = final_spins;
+        for (int si = 0; si < num_systems; ++si) {
+            const DevSystem& D = ds[(size_t)si];
+            const int planes = D.model == REWL_ISING2D ? 1 : 2;
+            for (int ch = 0; ch < P.chains; ++ch) {
+                const uint64_t* g = lat.data() + ((size_t)si * gps + (size_t)(ch / kLanes)) * kAnnealGroupWords;
+                const int lane = ch % kLanes;
+                for (int i = 0; i < D.N; ++i)
+                    for (int q = 0; q < planes; ++q) {
+                        const uint64_t w = g[(size_t)(q * D.L + i / D.L) * kLanes + (size_t)lane];
+                        *o++ = ((w >> (i % D.L)) & 1ull) ? -1 : 1;
+                    }
+            }
+        }
     }
     return REWL_OK;
 }

@@ -45,6 +45,48 @@ def _params(p):
     return out
 
 
+# SA_Parameters (simulated_annealing_parameters.cxx:9-20); max_blocks and
+# chains have no counterpart there.
+ANNEAL_DEFAULTS = dict(initial_temperature=25.0, final_temperature=0.25, num_temperatures=250, block_size=1000,
+                       stdev_tolerance=0.01, frozen_criterion=0.0, max_blocks=100, chains=1)
+
+
+def anneal_params(**kw):
+    p = dict(ANNEAL_DEFAULTS)
+    p.update(kw)
+    return p
+
+
+def _anneal_call(fn, systems, params, keep_spins):
+    """Shared marshalling of rewl_anneal and its oracle restatement."""
+    keep = []
+    descs = _desc_array([dict(s) for s in systems], keep)
+    ap = abi.AnnealParams()
+    for k, v in params.items():
+        setattr(ap, k, v)
+    n = len(systems) * int(params["chains"])
+    emin = np.empty(n, np.float64)
+    efin = np.empty(n, np.float64)
+    sw = np.empty(n, np.uint64)
+    ndof = sum(int(s["L"]) ** 2 * (1 if s["model"] == abi.ISING2D else 2) for s in systems) * int(params["chains"])
+    spins = np.empty(ndof, np.int8) if keep_spins else None
+    rc = fn(descs, len(systems), C.byref(ap), emin.ctypes.data_as(C.POINTER(C.c_double)),
+            efin.ctypes.data_as(C.POINTER(C.c_double)), sw.ctypes.data_as(C.POINTER(C.c_uint64)),
+            spins.ctypes.data_as(C.POINTER(C.c_int8)) if keep_spins else None)
+    return rc, dict(min_energy=emin, final_energy=efin, sweeps=sw, spins=spins)
+
+
+def anneal(systems, params, spins=True):
+    """Simulated annealing of params["chains"] chains per system on the GPU
+    (rewl_anneal; Simulated_Annealer::simulate_annealing).  Returns arrays
+    indexed system * chains + chain: min_energy, final_energy, sweeps, and the
+    final spins in the reference layout, chains of each system in order."""
+    lib = abi.load_engine()
+    rc, out = _anneal_call(lib.rewl_anneal, systems, params, spins)
+    abi.check(rc)
+    return out
+
+
 class Engine:
     """One rewl_ctx: every walker of the given systems on the current GPU."""
 

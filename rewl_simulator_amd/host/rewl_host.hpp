@@ -129,6 +129,60 @@ inline std::vector<double> random_field(uint64_t seed, double strength, int num_
     return h;
 }
 
+// SA_Parameters (simulated_annealing_parameters.cxx:9-20); max_blocks and
+// chains have no counterpart there (the reference loops until a temperature
+// step equilibrates, with one chain).
+struct SA_Parameters {
+    double initial_temperature = 25.;
+    double final_temperature = 0.25;
+    int num_iterations = 250;
+    int block_size = 1000;
+    double energy_stdev_tolerance = 0.01;
+    double frozen_criterion = 0.;
+    int max_blocks = 100;
+    int chains = 1;
+};
+
+// The SIMULATED_ANNEALING start of main.cpp:56-76: anneal on the GPU and
+// return the ground-state estimate (min_energy_found) for the bottom of the
+// energy range, and the final configuration that every walker then imports.
+// With several chains, the chain that met the lowest energy is taken.
+struct Annealed {
+    double ground_state_energy = 0.;
+    std::vector<int8_t> spins;
+};
+
+template <class Hamiltonian_t>
+Annealed simulated_annealing(const typename Hamiltonian_t::Parameters& sp, const SA_Parameters& sa, uint64_t seed) {
+    rewl_system_desc d{};
+    Hamiltonian_t::describe(sp, d);
+    d.e_min = sp.energy_min();
+    d.e_max = sp.energy_max();
+    d.bin_size = sp.energy_bin_size();
+    d.num_windows = 1;
+    d.replicas_per_window = 1;
+    d.seed = seed;
+    rewl_anneal_params p{};
+    p.initial_temperature = sa.initial_temperature;
+    p.final_temperature = sa.final_temperature;
+    p.num_temperatures = sa.num_iterations;
+    p.block_size = sa.block_size;
+    p.stdev_tolerance = sa.energy_stdev_tolerance;
+    p.frozen_criterion = sa.frozen_criterion;
+    p.max_blocks = sa.max_blocks;
+    p.chains = sa.chains;
+    const size_t C = (size_t)sa.chains, ndof = (size_t)sp.num_DoF();
+    std::vector<double> emin(C), efin(C);
+    std::vector<uint64_t> sweeps(C);
+    std::vector<int8_t> spins(C * ndof);
+    check(rewl_anneal(&d, 1, &p, emin.data(), efin.data(), sweeps.data(), spins.data()));
+    const size_t best = (size_t)(std::min_element(emin.begin(), emin.end()) - emin.begin());
+    Annealed out;
+    out.ground_state_energy = emin[best];
+    out.spins.assign(spins.begin() + (std::ptrdiff_t)(best * ndof), spins.begin() + (std::ptrdiff_t)((best + 1) * ndof));
+    return out;
+}
+
 template <class Hamiltonian_t>
 class REWL_simulation {
   public:
@@ -174,6 +228,15 @@ class REWL_simulation {
     // the reference's default range for the model
     REWL_simulation(const System_Parameters& sp, const REWL_Parameters& rp)
         : REWL_simulation(sp, rp, sp.energy_min(), sp.energy_max(), sp.energy_bin_size()) {}
+
+    // import_DoFs (main.cpp:202): every walker starts from `spins` (num_DoF
+    // values +-1 in the reference layout) and is brought into its window again.
+    void import_DoFs(const std::vector<int8_t>& spins) {
+        int32_t nw = 0;
+        check(rewl_num_walkers(ctx_, &nw));
+        for (int32_t g = 0; g < nw; ++g) check(rewl_set_spins(ctx_, g, spins.data(), (int32_t)spins.size()));
+        check(rewl_adjust_to_range(ctx_));
+    }
 
     REWL_simulation(const REWL_simulation&) = delete;
     REWL_simulation& operator=(const REWL_simulation&) = delete;

@@ -4,13 +4,17 @@
 // Ashkin-Teller, Ising otherwise, as model_hamiltonians.hpp chooses by macro)
 // and prints the normalised microcanonical ln g/N per energy bin.
 //
-//   rewl_run L windows replicas final_increment [average_mode] [observables] [densities]
+//   rewl_run L windows replicas final_increment [average_mode] [observables] [densities] [anneal]
 //
 // With observables = 1 it also prints the per-bin means of the model's
 // observables (E/N, then one column per observable); with observables = 2
 // the Fourier correlator G(qmin) columns as well.  With densities = 1
 // (Ashkin-Teller build, observables >= 1) it prints the sigma-tau density plot
 // of every bin, 1024 values a line (the reference's AT_DENSITIES output).
+// With anneal = 1 it first anneals 64 chains on the GPU (20 temperature steps,
+// blocks of 200 sweeps; the SIMULATED_ANNEALING start of main.cpp:56-76),
+// takes the lowest energy met as the bottom of the range and starts every
+// walker from that chain's final spins.
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -36,8 +40,21 @@ int main(int argc, char** argv) {
     rp.average_mode = argc > 5 ? std::atoi(argv[5]) : 0;
     rp.observables = argc > 6 ? std::atoi(argv[6]) : 0;
     rp.densities = argc > 7 && std::atoi(argv[7]) != 0;
+    const bool anneal = argc > 8 && std::atoi(argv[8]) != 0;
     try {
-        rewl_host::REWL_simulation<Hamiltonian_t<double>> sim(sp, rp);
+        double e_min = sp.energy_min();
+        rewl_host::Annealed start;
+        if (anneal) {
+            rewl_host::SA_Parameters sa;
+            sa.num_iterations = 20;
+            sa.block_size = 200;
+            sa.chains = 64;
+            start = rewl_host::simulated_annealing<Hamiltonian_t<double>>(sp, sa, rp.seed);
+            e_min = start.ground_state_energy;
+            std::printf("# annealed ground state %.15e\n", e_min);
+        }
+        rewl_host::REWL_simulation<Hamiltonian_t<double>> sim(sp, rp, e_min, sp.energy_max(), sp.energy_bin_size());
+        if (anneal) sim.import_DoFs(start.spins);
         const bool done = sim.simulate(1000000);
         std::vector<double> E, lg;
         sim.results(E, lg);

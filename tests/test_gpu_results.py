@@ -119,6 +119,23 @@ def test_host_program_density_plots(gpu):
     assert seen[0] and np.all(plots[0][np.setdiff1d(np.arange(1024), [0, 31, 992, 1023])] == 0)
 
 
+def test_host_program_anneal_start(gpu):
+    """The C++ host layer's SIMULATED_ANNEALING start: the annealer finds the
+    clean L = 8 Ising ground state (-2N = -128), the walkers import its spins,
+    and the run converges to the exact density of states as before."""
+    exe = os.path.join(ROOT, "rewl_simulator_amd", "host", "rewl_run")
+    out = subprocess.run([exe, "8", "2", "2", "1e-5", "1", "0", "0", "1"], capture_output=True, text=True, timeout=600)
+    assert out.returncode == 0, out.stderr
+    lines = out.stdout.splitlines()
+    gs = next(l for l in lines if l.startswith("# annealed ground state"))
+    assert float(gs.split()[-1]) == -128.0
+    rows = [l for l in lines if l and not l.startswith("#")]
+    data = np.array([[float(v) for v in r.split(",")] for r in rows])
+    err = _anchored_error(data[:, 0] * 64, data[:, 1] * 64, 8, 8.0)
+    print(f"host program L=8 after annealing: max |d ln g| = {err:.4f}", flush=True)
+    assert err < 0.1
+
+
 def test_thermodynamics_kernel_matches_oracle(gpu):
     """GPU FP64 trapezoid thermodynamics against the oracle's restatement of
     Thermodynamics::calculate_thermodynamics on the exact L=16 ln g."""
