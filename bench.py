@@ -47,9 +47,9 @@ def parse():
     ap.add_argument("--saturate", type=int, default=4, metavar="R",
                     help="also time R independent copies of the workload in one context (R x 2048 walkers, "
                          "enough to fill the chip); 0 to skip")
-    ap.add_argument("--workload", default="config2", choices=["config2", "config4"],
-                    help="config2: L=64 Ising REWL (the headline); config4: 256 batched random-strain "
-                         "Ashkin-Teller realisations in one context")
+    ap.add_argument("--workload", default="config2", choices=["config2", "config3", "config4"],
+                    help="config2: L=64 Ising REWL (the headline); config3: L=32 Ashkin-Teller REWL, 512 "
+                         "walkers; config4: 256 batched random-strain Ashkin-Teller realisations in one context")
     ap.add_argument("--converge", type=float, default=0.0, metavar="LNF",
                     help="instead of the throughput steps, time the run from the first sweep until every "
                          "window's ln f is below LNF")
@@ -140,14 +140,28 @@ def exact_binned_lng(E_bins, bin_size):
     return out
 
 
+AT_WORKLOADS = {
+    "config3": dict(metric="spin-flip attempts/s (Ashkin-Teller L=32)", walkers=512,
+                    data="synthetic (all-up start, Philox draws)", kernel="k_sweep_tuned<AT, L=32>",
+                    workload="2D Ashkin-Teller REWL, L=32, J=1, K=0.5, 1024 bins of width 8, 16 windows x 32 walkers"),
+    "config4": dict(metric="spin-flip attempts/s (batched random-strain Ashkin-Teller)", walkers=8192,
+                    data="synthetic (Gaussian strain per realisation, Philox draws)", kernel="k_sweep_field<AT, 3 producers>",
+                    workload="256 random-strain AT realisations, L in {16,24,32,48}, 8 windows x 4 walkers, "
+                             "512 bins, one launch per step"),
+}
+
+
 def batched(a):
-    """Config 4: 256 random-strain AT realisations (L in {16, 24, 32, 48},
-    8 windows x 4 walkers each, 8192 walkers) swept in one launch per step by
-    the field-model kernel; a step is sweeps_per_step sweeps + exchange +
-    check.  Attempts/s counts sigma and tau proposals (2N per sweep)."""
+    """The Ashkin-Teller workloads.  Config 3: one L=32 system, 16 windows x
+    32 walkers, row-unrolled integer kernel.  Config 4: 256 random-strain
+    realisations (L in {16, 24, 32, 48}, 8 windows x 4 walkers each, 8192
+    walkers) swept in one launch per step by the field-model kernel.  A step is
+    sweeps_per_step sweeps + exchange + check.  Attempts/s counts sigma and tau
+    proposals (2N per sweep)."""
     from rewl_simulator_amd import configs
     from rewl_simulator_amd.engine import Engine
-    systems, params = configs.config4()
+    w = AT_WORKLOADS[a.workload]
+    systems, params = configs.config3() if a.workload == "config3" else configs.config4()
     n = a.sweeps_per_step
     params = dict(params, sweeps_per_check=n, sweeps_per_exchange=n, producers=a.producers,
                   sweep_kernel=a.sweep_kernel, walkers_per_group=a.walkers_per_group)
@@ -176,14 +190,13 @@ def batched(a):
     avg_launch_s = (st1["kernel_ms_sweep"] - st0["kernel_ms_sweep"]) / 1e3 / max(launches, 1)
     achieved = att / max(launches, 1) * 32.5 / avg_launch_s / 1e9
     line = {
-        "metric": "spin-flip attempts/s (batched random-strain Ashkin-Teller)", "value": rate,
+        "metric": w["metric"], "value": rate,
         "unit": "spin-flip attempts/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
         "ms_per_step": (t1 - t0) * 1e3 / a.steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64 energies and ln g", "data": "synthetic (Gaussian strain per realisation, Philox draws)",
-        "config": {"workload": "256 random-strain AT realisations, L in {16,24,32,48}, 8 windows x 4 walkers, "
-                               "512 bins, one launch per step", "walkers": 8192, "sweeps_per_step": n},
+        "dtype": "f64 energies and ln g" if a.workload == "config4" else "int32 energies, f64 ln g", "data": w["data"],
+        "config": {"workload": w["workload"], "walkers": w["walkers"], "sweeps_per_step": n},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_sweep_field<AT, 3 producers>",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": w["kernel"],
                      "histogram_tier": "LDS", "lds_peak": LDS_PEAK_GBS, "lds_frac": achieved / LDS_PEAK_GBS,
                      "avg_launch_ms": avg_launch_s * 1e3, "bytes_per_attempt": 32.5},
     }
@@ -264,7 +277,7 @@ def main():
         print(f"bench.py measures one GPU only; --gpus {a.gpus} is not supported by this build "
               f"(every box here has one GPU).", file=sys.stderr)
         sys.exit(2)
-    if a.workload == "config4":
+    if a.workload in AT_WORKLOADS:
         batched(a)
         return
     from rewl_simulator_amd import configs

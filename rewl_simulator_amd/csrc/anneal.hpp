@@ -7,6 +7,7 @@
 #include <cfloat>
 
 #include "aux_kernels.hpp"
+#include "sweep.hpp"
 #include "sweep_fast.hpp"
 
 namespace rewl {
@@ -99,35 +100,48 @@ __device__ __forceinline__ bool sa_not_within_tolerance(double nv, double ov, do
     return a / ov > tol;
 }
 
+// One Metropolis attempt (sa_update, simulated_annealer.hpp:66-86): accept if
+// the proposal lowers the energy, else if u < exp(beta (E - E')).
+__device__ __forceinline__ bool metropolis(double E, double En, double beta, double u) {
+    return En < E || exp_less(u, beta * (E - En));
+}
+
+// The lattice rows of the current site are held in registers as the REWL
+// sweep holds them (the row above, the current row, the row below, row 0 as
+// rewritten, the row after next), so an attempt reads no memory; local
+// energies are summed in the reference's order (change_state).
+template <int MODEL>
 __global__ void __launch_bounds__(kLanes) k_anneal(AnnealArgs a) {
     const int gid = blockIdx.x, lane = threadIdx.x;
     const int si = gid / a.groups_per_sys;
     const int chain = (gid % a.groups_per_sys) * kLanes + lane;
     if (chain >= a.chains) return;   // no barrier follows
     const DevSystem S = a.sys[si];
-    const uint64_t* cl = a.lat + (int64_t)gid * kAnnealGroupWords;
-    uint64_t* lat = a.lat + (int64_t)gid * kAnnealGroupWords;
-    const int64_t base = lane;
-    const int planes = S.model == 0 ? 1 : 2;
+    if (S.model != MODEL) return;   // the other model's launch anneals this system
+    uint64_t* lat = a.lat + (int64_t)gid * kAnnealGroupWords + lane;
+    constexpr int NP = MODEL == 0 ? 1 : 2;
+    const int L = S.L;
 
     // randomize_dofs: spin_array[j] = u < 0.5 ? +1 : -1, j in the reference
     // layout (AT: 2 site + flavour); bit 1 = spin -1
     {
         u32x4 r{0u, 0u, 0u, 0u};
-        for (int y = 0; y < S.L; ++y) {
+        for (int y = 0; y < L; ++y) {
             uint64_t row[2] = {0ull, 0ull};
-            for (int x = 0; x < S.L; ++x)
-                for (int q = 0; q < planes; ++q) {
-                    const uint32_t j = (uint32_t)(planes * (y * S.L + x) + q);   // < 8192
+            for (int x = 0; x < L; ++x)
+#pragma unroll
+                for (int q = 0; q < NP; ++q) {
+                    const uint32_t j = (uint32_t)(NP * (y * L + x) + q);   // < 8192
                     if ((j & 3u) == 0u)
                         r = philox4x32_10(u32x4{j >> 2, 0u, DOMAIN_ANNEAL_INIT, (uint32_t)chain}, S.k0, S.k1);
                     // u = w 2^-32 < 0.5 gives +1
                     if (word_of(r, (int)(j & 3u)) >= 0x80000000u) row[q] |= 1ull << x;
                 }
-            for (int q = 0; q < planes; ++q) lat[base + (int64_t)(q * S.L + y) * kLanes] = row[q];
+#pragma unroll
+            for (int q = 0; q < NP; ++q) lat[(int64_t)(q * L + y) * kLanes] = row[q];
         }
     }
-    double E = recalc_energy(S, cl, base);
+    double E = recalc_energy(S, a.lat + (int64_t)gid * kAnnealGroupWords, lane);
     double emin = __builtin_inf();   // the reference leaves min_energy_found uninitialised
     unsigned long long total = 0ull;
     uint64_t ctr = 0ull;
@@ -142,22 +156,84 @@ __global__ void __launch_bounds__(kLanes) k_anneal(AnnealArgs a) {
         int blocks = 0;
         while (incomplete) {
             for (int b = 0; b < a.block_size; ++b) {
-                for (int pass = 0; pass < planes; ++pass)
-                    for (int site = 0; site < S.N; ++site) {
-                        if ((ctr & 3) == 0)
-                            rnd = philox4x32_10(u32x4{(uint32_t)(ctr >> 2), (uint32_t)(ctr >> 34), DOMAIN_ANNEAL,
-                                                      (uint32_t)chain}, S.k0, S.k1);
-                        const double u = (double)word_of(rnd, (int)(ctr & 3)) * 0x1p-32;
-                        ++ctr;
-                        const int plane = S.model == 0 ? 0 : fl;
-                        const double En = propose_fp(S, cl, base, site, fl, E);
-                        if (En < E || exp_less(u, beta * (E - En))) {
-                            const int x = site % S.L, y = site / S.L;
-                            lat[base + (int64_t)(plane * S.L + y) * kLanes] ^= 1ull << x;
-                            E = En;
-                        }
-                        emin = E < emin ? E : emin;
+                for (int pass = 0; pass < NP; ++pass) {
+                    uint64_t up[NP], cu[NP], dn[NP], r0[NP], nx[NP];
+#pragma unroll
+                    for (int q = 0; q < NP; ++q) {
+                        up[q] = lat[(int64_t)(q * L + L - 1) * kLanes];
+                        cu[q] = lat[(int64_t)(q * L) * kLanes];
+                        dn[q] = lat[(int64_t)(q * L + (L > 1 ? 1 : 0)) * kLanes];
+                        r0[q] = 0ull;
                     }
+                    for (int y = 0; y < L; ++y) {
+#pragma unroll
+                        for (int q = 0; q < NP; ++q) nx[q] = (y + 2 < L) ? lat[(int64_t)(q * L + y + 2) * kLanes] : 0ull;
+                        for (int x = 0; x < L; ++x) {
+                            if ((ctr & 3) == 0)
+                                rnd = philox4x32_10(u32x4{(uint32_t)(ctr >> 2), (uint32_t)(ctr >> 34), DOMAIN_ANNEAL,
+                                                          (uint32_t)chain}, S.k0, S.k1);
+                            const double u = (double)word_of(rnd, (int)(ctr & 3)) * 0x1p-32;
+                            ++ctr;
+                            const int site = y * L + x;
+                            const int xl = (x == 0) ? L - 1 : x - 1;
+                            const int xr = (x == L - 1) ? 0 : x + 1;
+                            double En;
+                            if constexpr (MODEL == 0) {
+                                const double sv = pm(bit(cu[0], x));
+                                double f = field_at(S, site);
+                                f += S.J * pm(bit(up[0], x));
+                                f += S.J * pm(bit(cu[0], xl));
+                                f += S.J * pm(bit(cu[0], xr));
+                                f += S.J * pm(bit(dn[0], x));
+                                const double tv = -sv;
+                                En = E + -1. * tv * f - -1. * sv * f;
+                                if (metropolis(E, En, beta, u)) {
+                                    cu[0] ^= 1ull << x;
+                                    E = En;
+                                }
+                            } else {
+                                const double sg = pm(bit(cu[0], x)), ta = pm(bit(cu[NP - 1], x));
+                                double ts = sg, tt = ta;
+                                ts += -2. * ts * (double)(fl == 0);
+                                tt += -2. * tt * (double)(fl == 1);
+                                const double ns[4] = {pm(bit(up[0], x)), pm(bit(cu[0], xl)), pm(bit(cu[0], xr)),
+                                                      pm(bit(dn[0], x))};
+                                const double nt[4] = {pm(bit(up[NP - 1], x)), pm(bit(cu[NP - 1], xl)),
+                                                      pm(bit(cu[NP - 1], xr)), pm(bit(dn[NP - 1], x))};
+                                double en_new = 0., en_old = 0.;
+                                if (S.has_field) {
+                                    const double h = field_at(S, site);
+                                    en_new += h * (ts * tt);
+                                    en_old += h * (sg * ta);
+                                }
+#pragma unroll
+                                for (int k = 0; k < 4; ++k) {
+                                    en_new += S.J * ts * ns[k];
+                                    en_new += S.J * tt * nt[k];
+                                    en_new += S.K * ts * tt * ns[k] * nt[k];
+                                    en_old += S.J * sg * ns[k];
+                                    en_old += S.J * ta * nt[k];
+                                    en_old += S.K * sg * ta * ns[k] * nt[k];
+                                }
+                                En = E + -en_new - -en_old;
+                                if (metropolis(E, En, beta, u)) {
+                                    cu[fl] ^= 1ull << x;
+                                    E = En;
+                                }
+                            }
+                            emin = E < emin ? E : emin;
+                        }
+#pragma unroll
+                        for (int q = 0; q < NP; ++q) {
+                            lat[(int64_t)(q * L + y) * kLanes] = cu[q];
+                            if (y == 0) r0[q] = cu[q];
+                            up[q] = cu[q];
+                            cu[q] = dn[q];
+                            dn[q] = (y + 2 < L) ? nx[q] : r0[q];
+                        }
+                    }
+                    if (MODEL == 1) fl ^= 1;   // the flavour switches after site N - 1
+                }
                 eb[(int64_t)b * kLanes] = E;
             }
             double avg, sdn;

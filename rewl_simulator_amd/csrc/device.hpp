@@ -192,6 +192,27 @@ __device__ __forceinline__ void update_observables(double* obs, int model, int N
     }
 }
 
+// The values update_observables adds, in group order: Ising |m|, m^2, m^4;
+// AT sigma, tau, phi, Baxter, each (|x|, x^2, x^4) (phi: sqrt(phi2), phi2,
+// phi2^2).  Value i goes to slot (i / 3) * k + i % 3, k = 3, or 4 with the
+// correlator slots.
+template <int MODEL>
+__device__ __forceinline__ void observable_values(int N, const int m[3], double* v) {
+    const double divide_N = 1. / (double)N;
+    if (MODEL == 0) {
+        const double x = divide_N * fabs((double)m[0]);
+        v[0] = x; v[1] = x * x; v[2] = x * x * x * x;
+    } else {
+        const double sv = divide_N * fabs((double)m[0]), tv = divide_N * fabs((double)m[1]);
+        const double bv = divide_N * fabs((double)m[2]);
+        const double phi2 = sv * sv + tv * tv;
+        v[0] = sv; v[1] = sv * sv; v[2] = sv * sv * sv * sv;
+        v[3] = tv; v[4] = tv * tv; v[5] = tv * tv * tv * tv;
+        v[6] = sqrt(phi2); v[7] = phi2; v[8] = phi2 * phi2;
+        v[9] = bv; v[10] = bv * bv; v[11] = bv * bv * bv * bv;
+    }
+}
+
 // update_densities (ashkin_teller_densities.hpp:17-34, called at
 // ashkin_teller2d_hamiltonian.hpp:291-293): the cell of (sigma/N, tau/N) in the
 // 32 x 32 grid over [-1.05, 1.05]^2, counted in the walker's histogram of

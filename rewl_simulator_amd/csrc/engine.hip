@@ -1390,8 +1390,16 @@ int rewl_anneal(const rewl_system_desc* systems, int32_t num_systems, const rewl
     a.chains = P.chains; a.groups_per_sys = gps;
     a.tol = P.stdev_tolerance; a.frozen = P.frozen_criterion;
     a.min_energy = B.emin.p; a.final_energy = B.efin.p; a.sweeps = B.sweeps.p;
-    hipLaunchKernelGGL(k_anneal, dim3((unsigned)ngroups), dim3(kLanes), 0, st, a);
-    HIP_TRY(hipGetLastError());
+    bool has_model[2] = {false, false};
+    for (const DevSystem& D : ds) has_model[D.model] = true;
+    if (has_model[REWL_ISING2D]) {
+        hipLaunchKernelGGL(k_anneal<REWL_ISING2D>, dim3((unsigned)ngroups), dim3(kLanes), 0, st, a);
+        HIP_TRY(hipGetLastError());
+    }
+    if (has_model[REWL_ASHKIN_TELLER2D]) {
+        hipLaunchKernelGGL(k_anneal<REWL_ASHKIN_TELLER2D>, dim3((unsigned)ngroups), dim3(kLanes), 0, st, a);
+        HIP_TRY(hipGetLastError());
+    }
     HIP_TRY(hipStreamSynchronize(st));
     HIP_TRY(hipMemcpy(min_energy, B.emin.p, nout * sizeof(double), hipMemcpyDeviceToHost));
     HIP_TRY(hipMemcpy(final_energy, B.efin.p, nout * sizeof(double), hipMemcpyDeviceToHost));

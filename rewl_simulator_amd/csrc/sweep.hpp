@@ -168,10 +168,34 @@ __device__ void sweep_walker(const KArgs& a, const DevGroup& G, const DevSystem&
     double* obs = OBS ? a.obs + hb * a.nobs : nullptr;
     const int64_t obs_bin = (int64_t)kLanes * a.nobs;
     if (OBS) lattice_magnetisations(lat, S.L, MODEL, mag);
+    // The moments of consecutive updates in one bin are summed in registers
+    // and added to the bin's sums when the walker changes bin (and at the end
+    // of the launch): one atomic per moment per run instead of per update.
+    constexpr int NV = MODEL == 0 ? 3 : 12;
+    constexpr int GK = OBS == 2 ? 4 : 3;   // slot stride of the (|x|, x^2, x^4[, G]) groups
+    double acc[NV];
+#pragma unroll
+    for (int i = 0; i < NV; ++i) acc[i] = 0.;
+    int accb = -1;
+    auto flush_acc = [&]() {
+        if (accb < 0) return;
+        double* p = obs + accb * obs_bin;
+#pragma unroll
+        for (int i = 0; i < NV; ++i) obs_add(p + (i / 3) * GK + i % 3, acc[i]);
+    };
     // the observables of one update at bin b: the magnetisation moments, and
     // with OBS = 2 the correlators when the bin's count before it is 0 mod 1000
     auto observe = [&](int b, int y, const uint64_t* rows) {
-        update_observables(obs + b * obs_bin, MODEL, S.N, mag, OBS == 2);
+        if (b != accb) {
+            flush_acc();
+            accb = b;
+#pragma unroll
+            for (int i = 0; i < NV; ++i) acc[i] = 0.;
+        }
+        double v[NV];
+        observable_values<MODEL>(S.N, mag, v);
+#pragma unroll
+        for (int i = 0; i < NV; ++i) acc[i] += v[i];
         if (MODEL == 1 && a.dens) update_density(a.dens, hb + (int64_t)b * kLanes, S.N, mag);
         if (OBS == 2) {
             const uint32_t t = s_tot[b * ls + lane];
@@ -305,6 +329,7 @@ __device__ void sweep_walker(const KArgs& a, const DevGroup& G, const DevSystem&
     } else {
         flush_counts<false>(s_cnt, a.cnt, hb, nbins, lane, ls);
     }
+    if (OBS) flush_acc();
     for (int b = 0; b < nbins; ++b) a.lng[hb + (int64_t)b * kLanes] = s_lng[b * ls + lane];
     if (OBS == 2)
         for (int b = 0; b < nbins; ++b) a.obs_tot[hb + (int64_t)b * kLanes] = s_tot[b * ls + lane];
