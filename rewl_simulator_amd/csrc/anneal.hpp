@@ -112,12 +112,18 @@ __device__ __forceinline__ bool metropolis(double E, double En, double beta, dou
 // energies are summed in the reference's order (change_state).
 template <int MODEL>
 __global__ void __launch_bounds__(kLanes) k_anneal(AnnealArgs a) {
+    extern __shared__ __align__(16) double s_field[];   // the system's per-site field, if any
     const int gid = blockIdx.x, lane = threadIdx.x;
     const int si = gid / a.groups_per_sys;
     const int chain = (gid % a.groups_per_sys) * kLanes + lane;
-    if (chain >= a.chains) return;   // no barrier follows
     const DevSystem S = a.sys[si];
-    if (S.model != MODEL) return;   // the other model's launch anneals this system
+    if (S.model != MODEL) return;   // the whole workgroup: the other model's launch anneals this system
+    // every chain of the workgroup reads the same field: staged in LDS once,
+    // so that an attempt does not wait on a global load
+    if (S.field)
+        for (int i = lane; i < S.N; i += kLanes) s_field[i] = S.field[i];
+    __syncthreads();
+    if (chain >= a.chains) return;   // no barrier follows
     uint64_t* lat = a.lat + (int64_t)gid * kAnnealGroupWords + lane;
     constexpr int NP = MODEL == 0 ? 1 : 2;
     const int L = S.L;
@@ -142,6 +148,7 @@ __global__ void __launch_bounds__(kLanes) k_anneal(AnnealArgs a) {
         }
     }
     double E = recalc_energy(S, a.lat + (int64_t)gid * kAnnealGroupWords, lane);
+    const double mJ = -S.J, mK = -S.K;
     double emin = __builtin_inf();   // the reference leaves min_energy_found uninitialised
     unsigned long long total = 0ull;
     uint64_t ctr = 0ull;
@@ -179,12 +186,13 @@ __global__ void __launch_bounds__(kLanes) k_anneal(AnnealArgs a) {
                             const int xr = (x == L - 1) ? 0 : x + 1;
                             double En;
                             if constexpr (MODEL == 0) {
+                                // J s_n is exactly +-J: a select, summed in the reference order
                                 const double sv = pm(bit(cu[0], x));
-                                double f = field_at(S, site);
-                                f += S.J * pm(bit(up[0], x));
-                                f += S.J * pm(bit(cu[0], xl));
-                                f += S.J * pm(bit(cu[0], xr));
-                                f += S.J * pm(bit(dn[0], x));
+                                double f = S.field ? s_field[site] : S.h;
+                                f += bit(up[0], x) ? mJ : S.J;
+                                f += bit(cu[0], xl) ? mJ : S.J;
+                                f += bit(cu[0], xr) ? mJ : S.J;
+                                f += bit(dn[0], x) ? mJ : S.J;
                                 const double tv = -sv;
                                 En = E + -1. * tv * f - -1. * sv * f;
                                 if (metropolis(E, En, beta, u)) {
@@ -192,28 +200,30 @@ __global__ void __launch_bounds__(kLanes) k_anneal(AnnealArgs a) {
                                     E = En;
                                 }
                             } else {
-                                const double sg = pm(bit(cu[0], x)), ta = pm(bit(cu[NP - 1], x));
-                                double ts = sg, tt = ta;
-                                ts += -2. * ts * (double)(fl == 0);
-                                tt += -2. * tt * (double)(fl == 1);
-                                const double ns[4] = {pm(bit(up[0], x)), pm(bit(cu[0], xl)), pm(bit(cu[0], xr)),
-                                                      pm(bit(dn[0], x))};
-                                const double nt[4] = {pm(bit(up[NP - 1], x)), pm(bit(cu[NP - 1], xl)),
-                                                      pm(bit(cu[NP - 1], xr)), pm(bit(dn[NP - 1], x))};
+                                // Every product of couplings and spins in the local energy
+                                // (J s s', J t t', K s s' t t', h s t) is exactly +-J, +-K or
+                                // +-h, so each term is a select on the parity of its spin
+                                // bits (bit 1 = -1); the terms are summed in the reference
+                                // order, so the sums round as there.
+                                const uint32_t sb = bit(cu[0], x), tb = bit(cu[NP - 1], x);
+                                const uint32_t tsb = sb ^ (uint32_t)(fl == 0), ttb = tb ^ (uint32_t)(fl == 1);
+                                const uint32_t nsb[4] = {bit(up[0], x), bit(cu[0], xl), bit(cu[0], xr), bit(dn[0], x)};
+                                const uint32_t ntb[4] = {bit(up[NP - 1], x), bit(cu[NP - 1], xl), bit(cu[NP - 1], xr),
+                                                         bit(dn[NP - 1], x)};
                                 double en_new = 0., en_old = 0.;
                                 if (S.has_field) {
-                                    const double h = field_at(S, site);
-                                    en_new += h * (ts * tt);
-                                    en_old += h * (sg * ta);
+                                    const double h = S.field ? s_field[site] : S.h;
+                                    en_new += (tsb ^ ttb) ? -h : h;
+                                    en_old += (sb ^ tb) ? -h : h;
                                 }
 #pragma unroll
                                 for (int k = 0; k < 4; ++k) {
-                                    en_new += S.J * ts * ns[k];
-                                    en_new += S.J * tt * nt[k];
-                                    en_new += S.K * ts * tt * ns[k] * nt[k];
-                                    en_old += S.J * sg * ns[k];
-                                    en_old += S.J * ta * nt[k];
-                                    en_old += S.K * sg * ta * ns[k] * nt[k];
+                                    en_new += (tsb ^ nsb[k]) ? mJ : S.J;
+                                    en_new += (ttb ^ ntb[k]) ? mJ : S.J;
+                                    en_new += (tsb ^ ttb ^ nsb[k] ^ ntb[k]) ? mK : S.K;
+                                    en_old += (sb ^ nsb[k]) ? mJ : S.J;
+                                    en_old += (tb ^ ntb[k]) ? mJ : S.J;
+                                    en_old += (sb ^ tb ^ nsb[k] ^ ntb[k]) ? mK : S.K;
                                 }
                                 En = E + -en_new - -en_old;
                                 if (metropolis(E, En, beta, u)) {

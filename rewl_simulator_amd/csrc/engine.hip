@@ -331,6 +331,8 @@ int launch_sweep_tuned(rewl_ctx* c, int li, int nsweeps) {
     void* kern;
     if (sk == 2) kern = simple ? (void*)k_sweep_tuned<MODEL, L, true, 0> : (void*)k_sweep_tuned<MODEL, L, false, 0>;
     else if (sk == 4) kern = simple ? (void*)k_sweep_tuned<MODEL, L, true, 2> : (void*)k_sweep_tuned<MODEL, L, false, 2>;
+    else if (c->p.observables)
+        kern = simple ? (void*)k_sweep_tuned<MODEL, L, true, 1, true> : (void*)k_sweep_tuned<MODEL, L, false, 1, true>;
     else kern = simple ? (void*)k_sweep_tuned<MODEL, L, true, 1> : (void*)k_sweep_tuned<MODEL, L, false, 1>;
     HIP_TRY(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     KArgs args = c->args;
@@ -545,10 +547,14 @@ int rewl_create(const rewl_system_desc* systems, int32_t num_systems, const rewl
         c->nprod = P.producers < 0 ? 0 : (P.producers == 0 ? 1 : std::min<int>(P.producers, 3));
         c->tuned_nprod = P.producers == 0 ? 7 : std::min<int>(std::max<int>(P.producers, 1), 7);
         const int li = tuned_index(S.d.L);
-        const bool tuned = fast && P.producers >= 0 && li >= 0 && P.sweep_kernel != 1;
+        // observables = 1 (without densities) also run on the row-unrolled kernel, in its default form
+        const bool tuned_obs = P.observables == 1 && !P.densities && P.sweep_kernel != 2 && P.sweep_kernel != 4;
+        const bool tuned = S.intpath && P.rng_mode == REWL_RNG_PHILOX && (!obs || tuned_obs) && P.producers >= 0 &&
+                           li >= 0 && P.sweep_kernel != 1;
         if (P.sweep_kernel >= 2 && !tuned)
             return bail(fail(REWL_ERR_INVALID, "the row-unrolled kernel needs the Philox integer path, producers >= 0 "
-                                               "and L in {16, 24, 32, 48, 64}"));
+                                               "and L in {16, 24, 32, 48, 64}; with observables, sweep_kernel 0 or 3 "
+                                               "and no densities"));
         int wpg = kLanes;
         // guard bins: |Delta E| <= 4 (2J) (+ 4 (2K)) on the integer path
         int guard = 0;
@@ -1391,13 +1397,17 @@ int rewl_anneal(const rewl_system_desc* systems, int32_t num_systems, const rewl
     a.tol = P.stdev_tolerance; a.frozen = P.frozen_criterion;
     a.min_energy = B.emin.p; a.final_energy = B.efin.p; a.sweeps = B.sweeps.p;
     bool has_model[2] = {false, false};
-    for (const DevSystem& D : ds) has_model[D.model] = true;
+    size_t lds = 0;   // the largest per-site field, staged in LDS by each workgroup
+    for (int si = 0; si < num_systems; ++si) {
+        has_model[ds[(size_t)si].model] = true;
+        if (systems[si].field) lds = std::max(lds, (size_t)ds[(size_t)si].N * sizeof(double));
+    }
     if (has_model[REWL_ISING2D]) {
-        hipLaunchKernelGGL(k_anneal<REWL_ISING2D>, dim3((unsigned)ngroups), dim3(kLanes), 0, st, a);
+        hipLaunchKernelGGL(k_anneal<REWL_ISING2D>, dim3((unsigned)ngroups), dim3(kLanes), lds, st, a);
         HIP_TRY(hipGetLastError());
     }
     if (has_model[REWL_ASHKIN_TELLER2D]) {
-        hipLaunchKernelGGL(k_anneal<REWL_ASHKIN_TELLER2D>, dim3((unsigned)ngroups), dim3(kLanes), 0, st, a);
+        hipLaunchKernelGGL(k_anneal<REWL_ASHKIN_TELLER2D>, dim3((unsigned)ngroups), dim3(kLanes), lds, st, a);
         HIP_TRY(hipGetLastError());
     }
     HIP_TRY(hipStreamSynchronize(st));

@@ -128,9 +128,14 @@ __device__ __forceinline__ void tuned_attempt(int on, const TunedWindow& tw, int
     a = acc ? 1 : 0;
 }
 
-template <int MODEL, int L, bool SIMPLE, int SPEC>
+// OBS (with SPEC 1 only): the observables of update_observables at every
+// attempt, from magnetisations kept in registers; the moments of consecutive
+// attempts in one bin are summed in registers and added to the bin's sums
+// when the walker leaves it.
+template <int MODEL, int L, bool SIMPLE, int SPEC, bool OBS = false>
 __device__ __forceinline__ void sweep_walker_tuned(const KArgs& a, const DevGroup& G, const DevSystem& S, int lane,
                                                    int nsweeps, int guard, uint32_t ring_lane) {
+    static_assert(!OBS || SPEC == 1, "observables are tracked by the SPEC 1 form");
     constexpr int NW = (L + 31) / 32;    // 32-bit words per row
     constexpr int NW2 = (L + 15) / 16;   // words of 2-bit fields per row
     constexpr int NP = MODEL == 0 ? 1 : 2;
@@ -177,6 +182,33 @@ __device__ __forceinline__ void sweep_walker_tuned(const KArgs& a, const DevGrou
 
     const int m4J = -2 * S.twoJ, m4K = -2 * S.twoK;
     int buf = 0;
+
+    constexpr int NV = MODEL == 0 ? 3 : 12;
+    int mag[3] = {0, 0, 0};
+    double accv[NV];
+#pragma unroll
+    for (int i = 0; i < NV; ++i) accv[i] = 0.;
+    uint32_t acck = 0xffffffffu;   // slot of the bin being summed (none yet)
+    auto flush_obs = [&]() {
+        if (acck == 0xffffffffu) return;
+        const int b = (int)((acck - zero_off) / (uint32_t)stride16);
+        double* p = a.obs + (hb + (int64_t)b * kLanes) * a.nobs;
+#pragma unroll
+        for (int i = 0; i < NV; ++i) obs_add(p + i, accv[i]);   // observables = 1: slot i
+    };
+    auto observe = [&]() {
+        if (cbo != acck) {
+            flush_obs();
+            acck = cbo;
+#pragma unroll
+            for (int i = 0; i < NV; ++i) accv[i] = 0.;
+        }
+        double v[NV];
+        observable_values<MODEL>(L * L, mag, v);
+#pragma unroll
+        for (int i = 0; i < NV; ++i) accv[i] += v[i];
+    };
+    if (OBS && active) lattice_magnetisations(a.lat + G.lat_base + lane, L, MODEL, mag);
 
     for (int s = 0; s < nsweeps; ++s) {
         for (int pass = 0; pass < NPASS; ++pass) {
@@ -343,6 +375,19 @@ __device__ __forceinline__ void sweep_walker_tuned(const KArgs& a, const DevGrou
                             frev[h] = (frev[h] << 1) | (uint32_t)acc;
                             on = ac ? on1 : on0;
                             nbo = ac ? nbo1 : nbo0;
+                            if constexpr (OBS) {
+                                // the flipped spin's bit in its plane (1 = -1 before the flip)
+                                const int pb = (int)((P >> x) & 1ull);
+                                const int d = ac ? 4 * pb - 2 : 0;
+                                if (MODEL == 0) {
+                                    mag[0] += d;
+                                } else {
+                                    mag[0] += fl ? 0 : d;
+                                    mag[1] += fl ? d : 0;
+                                    mag[2] += ac ? 4 * (int)(((P ^ Q) >> x) & 1ull) - 2 : 0;
+                                }
+                                observe();
+                            }
                         }
                     } else {
 #pragma unroll
@@ -389,6 +434,7 @@ __device__ __forceinline__ void sweep_walker_tuned(const KArgs& a, const DevGrou
     }
 
     if (active) {
+        if (OBS) flush_obs();
         for (int b = 0; b < nbins; ++b) {
             const uint32_t off = zero_off + (uint32_t)(b * stride16);
             a.lng[hb + (int64_t)b * kLanes] = lds_f64(off);
@@ -453,7 +499,7 @@ __device__ __forceinline__ void produce_bounds_tuned(const DevGroup& G, const De
 // slots of 16 bytes, then one ring row of tuned_ring_stride(L) bytes per
 // walker (two buffers of L FP64 bounds, padded by 16 bytes so that the
 // walkers' 16-byte reads fall on different banks).
-template <int MODEL, int L, bool SIMPLE, int SPEC>
+template <int MODEL, int L, bool SIMPLE, int SPEC, bool OBS = false>
 __global__ void __launch_bounds__(512) k_sweep_tuned(KArgs a, const int32_t* groups, int ngroups, int nsweeps,
                                                      uint64_t sweep_base, int guard) {
     const int tid = threadIdx.x;
@@ -468,7 +514,7 @@ __global__ void __launch_bounds__(512) k_sweep_tuned(KArgs a, const int32_t* gro
         const int lane = tid & (kLanes - 1);
         const uint32_t ring_lane = ring + (uint32_t)(lane * tuned_ring_stride(L));
         if (tid < kLanes)
-            sweep_walker_tuned<MODEL, L, SIMPLE, SPEC>(a, G, S, tid, nsweeps, guard, ring_lane);
+            sweep_walker_tuned<MODEL, L, SIMPLE, SPEC, OBS>(a, G, S, tid, nsweeps, guard, ring_lane);
         else
             produce_bounds_tuned<MODEL, L>(G, S, tid / kLanes - 1, (int)(blockDim.x / kLanes) - 1, lane, nsweeps,
                                            sweep_base, ring_lane, static_cast<const BoundEntry*>(a.bound_tab));

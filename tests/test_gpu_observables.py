@@ -262,3 +262,31 @@ def test_density_plots_from_a_run(gpu, tmp_path):
     files = D.write_density_plots(str(tmp_path / "Density_Plots"), plots, base)
     assert len(files) == nb_sys
     assert np.loadtxt(files[0], comments="#").shape == (32, 32)
+
+
+@pytest.mark.parametrize("model", [abi.ISING2D, abi.ASHKIN_TELLER2D])
+def test_observables_row_kernel_match_oracle(gpu, model):
+    """observables = 1 in Philox mode at L = 16 runs on the row-unrolled
+    kernel, which tracks the magnetisations in registers: chains bit-exact,
+    per-bin sums / counts against the oracle's running means (rtol 1e-9)."""
+    print(f"case: row-kernel observables model={model}", flush=True)
+    L, N = 16, 256
+    s = (configs.ising_system(L, -2.0 * N, 2.0 * N + 8.0, 8.0, 3, 3) if model == abi.ISING2D
+         else configs.at_system(L, -5.0 * N, 3.0 * N + 12.0, 12.0, 3, 3))
+    p = configs.params(rng_mode=abi.RNG_PHILOX, lnf_initial=0.05, observables=1, sweeps_per_exchange=20, adjust_mode=1)
+    eng, ora = Engine([s], p), Oracle([s], p)
+    eng.adjust_to_range(); ora.adjust_to_range()
+    for _ in range(3):
+        eng.sweeps(20); ora.sweeps(20)
+        eng.exchange(); ora.exchange()
+    eng.check(); ora.check()
+    assert_same(eng, ora, "row kernel, observables on", cursors=False)
+    for g in range(eng.num_walkers):
+        sums = eng.observables(g)
+        n = eng.counts(g).astype(np.float64) + eng.visits(g)
+        means, counts = ora.observables(g)
+        np.testing.assert_array_equal(n, counts, err_msg=f"counts per bin, walker {g}")
+        seen = n > 0
+        np.testing.assert_allclose(sums[seen] / n[seen, None], means[seen], rtol=1e-9, atol=1e-14,
+                                   err_msg=f"walker {g}")
+    eng.close(); ora.close()
