@@ -168,9 +168,10 @@ typedef struct rewl_params {
                                     walker's current bin: |m|, m^2, m^4 (Ising,
                                     ising2d_hamiltonian.hpp:234-256) or the 12 AT
                                     observables (ashkin_teller2d_hamiltonian.hpp:
-                                    267-318); in Philox mode with integer energies
-                                    and L in {16,24,32,48,64} the row-unrolled
-                                    kernel keeps them, otherwise the generic one.
+                                    267-318); in Philox mode the row-unrolled
+                                    kernel (integer energies, L in {16,24,32,48,64})
+                                    or the field-model kernel keeps them, otherwise
+                                    the generic one.
                                     2: also the Fourier correlators G(qmin) at every
                                     1000th update of a bin (REWL_NUM_OBS_*_CORR
                                     layout).  0: off                                */

@@ -302,7 +302,8 @@ template <int MODEL>
 int launch_sweep_field(rewl_ctx* c, int cls, int nsweeps) {
     const int ng = (int)c->class_groups[cls].size();
     const size_t lds = field_lds(c, cls);
-    const void* kern = (const void*)k_sweep_field<MODEL, kFieldProducers>;
+    const void* kern = c->p.observables ? (const void*)k_sweep_field<MODEL, kFieldProducers, true>
+                                        : (const void*)k_sweep_field<MODEL, kFieldProducers>;
     HIP_TRY(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     KArgs args = c->args;
     const int32_t* gl = c->d_class_groups[cls].p;
@@ -543,7 +544,7 @@ int rewl_create(const rewl_system_desc* systems, int32_t num_systems, const rewl
         const bool fast = S.intpath && P.rng_mode == REWL_RNG_PHILOX && !obs;
         // field models in Philox mode: k_sweep_field (sweep_kernel 1 or producers < 0 keep k_sweep)
         const bool field_fast = !S.intpath && P.rng_mode == REWL_RNG_PHILOX && P.producers >= 0 &&
-                                P.sweep_kernel != 1 && !obs;
+                                P.sweep_kernel != 1 && (!obs || (P.observables == 1 && !P.densities));
         c->nprod = P.producers < 0 ? 0 : (P.producers == 0 ? 1 : std::min<int>(P.producers, 3));
         c->tuned_nprod = P.producers == 0 ? 7 : std::min<int>(std::max<int>(P.producers, 1), 7);
         const int li = tuned_index(S.d.L);

@@ -80,7 +80,11 @@ __device__ void field_produce(const DevGroup& G, const DevSystem& S, int wave, i
     }
 }
 
-template <int MODEL>
+// OBS: the observables of update_observables (observables = 1) at every
+// attempt, from magnetisations kept in registers; the moments of consecutive
+// attempts in one bin are summed in registers and added to the bin's sums
+// when the walker leaves it.
+template <int MODEL, bool OBS = false>
 __device__ void sweep_walker_field(const KArgs& a, const DevGroup& G, const DevSystem& S, int lane, int nsweeps,
                                    double* s_lng, uint32_t* s_cnt, const double* s_field, const double* ring) {
     const bool active = lane < G.nwalkers;
@@ -110,6 +114,20 @@ __device__ void sweep_walker_field(const KArgs& a, const DevGroup& G, const DevS
     constexpr int NP = MODEL == 0 ? 1 : 2;
     constexpr int NPASS = MODEL == 0 ? 1 : 2;
     int buf = 0;
+
+    constexpr int NV = MODEL == 0 ? 3 : 12;
+    int mag[3] = {0, 0, 0};
+    double accv[NV];
+#pragma unroll
+    for (int i = 0; i < NV; ++i) accv[i] = 0.;
+    int accb = -1;   // bin being summed (none yet)
+    auto flush_obs = [&]() {
+        if (accb < 0) return;
+        double* p = a.obs + (hb + (int64_t)accb * kLanes) * a.nobs;
+#pragma unroll
+        for (int i = 0; i < NV; ++i) obs_add(p + i, accv[i]);   // observables = 1: slot i
+    };
+    if (OBS && active) lattice_magnetisations(lat, L, MODEL, mag);
 
     for (int s = 0; s < nsweeps; ++s) {
         for (int pass = 0; pass < NPASS; ++pass) {
@@ -178,6 +196,29 @@ __device__ void sweep_walker_field(const KArgs& a, const DevGroup& G, const DevS
                     lc += lnf;
                     s_lng[cb * stride + lane] = lc;
                     atomicAdd(&s_cnt[cb * stride + lane], 1u);
+                    if constexpr (OBS) {
+                        // the flipped spin's bit before the flip (1 = -1), in its plane
+                        const uint64_t pw = (MODEL == 1 && fl) ? cu[NP - 1] : cu[0];
+                        const int pb = (int)((pw >> x) & 1ull);
+                        const int d = acc ? 4 * pb - 2 : 0;
+                        if (MODEL == 0) {
+                            mag[0] += d;
+                        } else {
+                            mag[0] += fl ? 0 : d;
+                            mag[1] += fl ? d : 0;
+                            mag[2] += acc ? 4 * (int)(((cu[0] ^ cu[NP - 1]) >> x) & 1ull) - 2 : 0;
+                        }
+                        if (cb != accb) {
+                            flush_obs();
+                            accb = cb;
+#pragma unroll
+                            for (int i = 0; i < NV; ++i) accv[i] = 0.;
+                        }
+                        double v[NV];
+                        observable_values<MODEL>(S.N, mag, v);
+#pragma unroll
+                        for (int i = 0; i < NV; ++i) accv[i] += v[i];
+                    }
                     if (acc) {
                         if (MODEL == 1 && fl) cu[1] ^= (1ull << x);
                         else cu[0] ^= (1ull << x);
@@ -200,6 +241,7 @@ __device__ void sweep_walker_field(const KArgs& a, const DevGroup& G, const DevS
     }
 
     if (active) {
+        if (OBS) flush_obs();
         for (int b = 0; b < nbins; ++b) {
             a.lng[hb + (int64_t)b * kLanes] = s_lng[b * stride + lane];
             a.cnt[hb + (int64_t)b * kLanes] = s_cnt[b * stride + lane];
@@ -211,7 +253,7 @@ __device__ void sweep_walker_field(const KArgs& a, const DevGroup& G, const DevS
 
 // One workgroup = one walker wavefront + NPROD producer wavefronts; groups are
 // taken persistently in blockIdx order.
-template <int MODEL, int NPROD>
+template <int MODEL, int NPROD, bool OBS = false>
 __global__ void __launch_bounds__(64 * (1 + NPROD)) k_sweep_field(KArgs a, const int32_t* groups, int ngroups,
                                                                   int nsweeps, uint64_t sweep_base) {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -229,7 +271,7 @@ __global__ void __launch_bounds__(64 * (1 + NPROD)) k_sweep_field(KArgs a, const
         double* ring = s_field + S.N;
         for (int i = tid; i < S.N; i += blockDim.x) s_field[i] = field_at(S, i);
         if (tid < kLanes)
-            sweep_walker_field<MODEL>(a, G, S, tid, nsweeps, s_lng, s_cnt, s_field, ring);
+            sweep_walker_field<MODEL, OBS>(a, G, S, tid, nsweeps, s_lng, s_cnt, s_field, ring);
         else
             field_produce<MODEL>(G, S, tid / kLanes - 1, NPROD, tid & (kLanes - 1), nsweeps, sweep_base, ring, tab);
         __syncthreads();   // the ring, field and LDS histograms are reused by the next group
