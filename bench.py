@@ -50,6 +50,9 @@ def parse():
     ap.add_argument("--workload", default="config2", choices=["config2", "config3", "config4"],
                     help="config2: L=64 Ising REWL (the headline); config3: L=32 Ashkin-Teller REWL, 512 "
                          "walkers; config4: 256 batched random-strain Ashkin-Teller realisations in one context")
+    ap.add_argument("--observables", type=int, default=0, choices=[0, 1, 2],
+                    help="per-bin observables during the sweeps (1: moments, 2: with the Fourier correlator); "
+                         "the headline figure is measured with 0, as the workload's reference runs them")
     ap.add_argument("--converge", type=float, default=0.0, metavar="LNF",
                     help="instead of the throughput steps, time the run from the first sweep until every "
                          "window's ln f is below LNF")
@@ -164,7 +167,7 @@ def batched(a):
     systems, params = configs.config3() if a.workload == "config3" else configs.config4()
     n = a.sweeps_per_step
     params = dict(params, sweeps_per_check=n, sweeps_per_exchange=n, producers=a.producers,
-                  sweep_kernel=a.sweep_kernel, walkers_per_group=a.walkers_per_group)
+                  sweep_kernel=a.sweep_kernel, walkers_per_group=a.walkers_per_group, observables=a.observables)
     eng = Engine(systems, params)
     eng.adjust_to_range()
 
@@ -194,7 +197,8 @@ def batched(a):
         "unit": "spin-flip attempts/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
         "ms_per_step": (t1 - t0) * 1e3 / a.steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64 energies and ln g" if a.workload == "config4" else "int32 energies, f64 ln g", "data": w["data"],
-        "config": {"workload": w["workload"], "walkers": w["walkers"], "sweeps_per_step": n},
+        "config": {"workload": w["workload"], "walkers": w["walkers"], "sweeps_per_step": n,
+                   "observables": a.observables},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": w["kernel"],
                      "histogram_tier": "LDS", "lds_peak": LDS_PEAK_GBS, "lds_frac": achieved / LDS_PEAK_GBS,
@@ -286,7 +290,7 @@ def main():
     systems, params = configs.config2()
     params = dict(params, sweeps_per_check=a.sweeps_per_step, sweeps_per_exchange=a.sweeps_per_step,
                   walkers_per_group=a.walkers_per_group, producers=a.producers, sweep_kernel=a.sweep_kernel,
-                  average_mode=a.average_mode)
+                  average_mode=a.average_mode, observables=a.observables)
     if a.one_over_t:
         from rewl_simulator_amd import abi
         params["schedule"] = abi.SCHED_ONE_OVER_T
@@ -324,7 +328,7 @@ def main():
     att_per_launch = attempts / max(launches, 1)
     achieved = att_per_launch * BYTES_PER_ATTEMPT / avg_launch_s / 1e9
     lnf = min(w["lnf"] for w in (eng.window_info(k) for k in range(eng.num_windows)))
-    default_launch = a.sweep_kernel in (0, 3) and a.producers == 0 and a.walkers_per_group == 0
+    default_launch = a.sweep_kernel in (0, 3) and a.producers == 0 and a.walkers_per_group == 0 and not a.observables
     traffic = MEASURED_TRAFFIC.get(params["sweeps_per_check"]) if default_launch else None
 
     line = {
@@ -349,6 +353,7 @@ def main():
             "parallelism": "one lane per walker, 1 GPU",
             "producers": a.producers,
             "sweep_kernel": a.sweep_kernel,
+            "observables": a.observables,
         },
         "roofline": {
             "bound": "hbm",
