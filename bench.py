@@ -278,8 +278,8 @@ def main():
         calibrate_copy(a.calibrate_copy)
         return
     if a.gpus != 1:
-        print(f"bench.py measures one GPU only; --gpus {a.gpus} is not supported by this build "
-              f"(every box here has one GPU).", file=sys.stderr)
+        print(f"bench.py measures one GPU only; --gpus {a.gpus} is not supported: this build runs the "
+              f"engine on one GPU (sharding windows across GPUs is a design, not built).", file=sys.stderr)
         sys.exit(2)
     if a.workload in AT_WORKLOADS:
         batched(a)

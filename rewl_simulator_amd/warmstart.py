@@ -190,7 +190,7 @@ def anchored_max_error(a, b, mask):
 def evaluate(mlp, specs, lnf_final=1e-6, lnf_warm=1e-3, ref_lnf=1e-8, max_checks=40000, log=print):
     """Plain (flat ln g, ln f = 1) and warm-started (network ln g, ln f =
     lnf_warm) runs of the same held-out systems to ln f < lnf_final.  The
-    plain result judges the warm one (max |difference| over commonly visited
+    plain result is the yardstick of the warm one (max |difference| over commonly visited
     bins, anchored); with ref_lnf > 0 both are also compared to a plain run
     taken to ref_lnf."""
     systems = rfat_systems(specs)
@@ -198,7 +198,7 @@ def evaluate(mlp, specs, lnf_final=1e-6, lnf_warm=1e-3, ref_lnf=1e-8, max_checks
     ref = None
     if ref_lnf > 0:
         # the same disorder with independent walker streams: a reference that
-        # shares the main runs' streams would repeat their errors
+        # shares the compared runs' streams would repeat their errors
         ref_systems = [dict(s, seed=s["seed"] + 0x9E3779B9) for s in systems]
         with Engine(ref_systems, rfat_params(ref_lnf)) as eng:
             eng.adjust_to_range()
@@ -230,9 +230,9 @@ def evaluate(mlp, specs, lnf_final=1e-6, lnf_warm=1e-3, ref_lnf=1e-8, max_checks
     res["warm_vs_plain_max"] = float(np.max(dev))
     res["warm_vs_plain_median"] = float(np.median(dev))
     # the network's own error before any sampling, against the best curve at hand
-    judge = ref if ref is not None else out["plain"]
+    yardstick = ref if ref is not None else out["plain"]
     pe = [anchored_max_error(predict_lng(mlp, r["E"], L, sigma), r["lng"], r["visited"])
-          for r, (L, sigma, seed) in zip(judge, specs)]
+          for r, (L, sigma, seed) in zip(yardstick, specs)]
     res["prediction_error_max"] = float(np.max(pe))
     res["prediction_error_median"] = float(np.median(pe))
     res["speedup_wall"] = res["plain"]["seconds"] / (res["warm"]["seconds"] + res["warm"]["init_seconds"])
@@ -273,7 +273,7 @@ def heldout_specs(n, L=48, sigma=0.5, seed0=900000):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--out", default="gpurun_out/warmstart")
+    ap.add_argument("--out", default="warmstart_out")
     ap.add_argument("--curves", type=int, default=256)
     ap.add_argument("--sizes", default="12,16,20,24")
     ap.add_argument("--sigmas", default="0,0.25,0.5,0.75,1")

@@ -2,7 +2,7 @@
 against the reference rule it replaces (wang_landau.hpp:72-84):
 d = (float)x; accept if d <= 0, else if k 2^-32 < exp(-(double)d).  The
 header is compiled for the host with g++ (its table from glibc log1p, the rule
-judged by glibc exp) and probed at random draws, at every word near the
+checked against glibc exp) and probed at random draws, at every word near the
 table's interval ends, and at the floats and midpoints around every bound."""
 import os
 import subprocess
@@ -30,7 +30,7 @@ static bool reference(uint32_t k, double x) {
 }
 
 // true if exp(-d) lies so close to the rounding midpoint above u that a libm
-// exp within 0.502 ulp may round either way (long double exp as the judge)
+// exp within 0.502 ulp may round either way (long double exp as the yardstick)
 static bool ambiguous(uint32_t k, double x) {
     const float d = (float)x;
     if (d <= 0.f || k == 0u) return false;
