@@ -55,3 +55,17 @@ def test_anneal_rejects_bad_parameters(gpu):
     for bad in (dict(chains=0), dict(max_blocks=0), dict(block_size=0), dict(final_temperature=0.0)):
         with pytest.raises(abi.REWLError):
             anneal([s], anneal_params(**bad))
+
+
+def test_anneal_mixed_models_in_one_call(gpu):
+    """Ising and Ashkin-Teller systems in one call (one launch per model,
+    each skipping the other's workgroups), with and without fields: every
+    chain as the oracle has it."""
+    systems = [_system(abi.ASHKIN_TELLER2D, True, L=4), _system(abi.ISING2D, False, L=6),
+               _system(abi.ISING2D, True, L=5), _system(abi.ASHKIN_TELLER2D, False, L=3)]
+    p = anneal_params(initial_temperature=6.0, final_temperature=0.3, num_temperatures=6, block_size=20,
+                      max_blocks=4, chains=5)
+    g = anneal(systems, p)
+    o = O.anneal(systems, p)
+    for k in ("min_energy", "final_energy", "sweeps", "spins"):
+        np.testing.assert_array_equal(g[k], o[k], err_msg=k)
