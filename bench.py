@@ -57,6 +57,9 @@ def parse():
                     help="instead of the throughput steps, time the run from the first sweep until every "
                          "window's ln f is below LNF")
     ap.add_argument("--max-checks", type=int, default=200000)
+    ap.add_argument("--half-range", action="store_true",
+                    help="with --workload config3 --converge: the reference's own Ashkin-Teller range [-5N, 0) "
+                         "instead of the full spectrum")
     ap.add_argument("--average-mode", type=int, default=0,
                     help="replica averaging: 0 count-weighted (the reference), 1 equal weights")
     ap.add_argument("--dump", default="", metavar="PATH",
@@ -217,11 +220,17 @@ def converge_at(a):
     the threshold.  No exact density of states is known for it; the stitched,
     normalised ln g is checked at the ground bin, which holds only the four
     ground states (the lowest excitation costs 8J + 4K = 12 > the bin width
-    8), so ln g there must be ln 4."""
+    8), so ln g there must be ln 4 (with the full range, where the
+    normalisation is exact), and at the step to bin 1, which holds only the
+    8N single-flip excitations: ln g(1) - ln g(0) = ln 2N for either range."""
     import numpy as np
     from rewl_simulator_amd import configs
     from rewl_simulator_amd.engine import Engine
     systems, params = configs.config3(lnf_final=a.converge)
+    if a.half_range:
+        N = 32 * 32
+        systems = [dict(systems[0], e_max=0.0)]
+        assert systems[0]["e_min"] == -5.0 * N
     params = dict(params, average_mode=a.average_mode, observables=a.observables)
     eng = Engine(systems, params)
     eng.adjust_to_range()
@@ -255,7 +264,10 @@ def converge_at(a):
         "sweeps": st1["sweeps"] - st0["sweeps"], "attempts": st1["attempts"] - st0["attempts"],
         "attempts_per_s": (st1["attempts"] - st0["attempts"]) / (t1 - t0),
         "ground_bin_lng": float(lng[0]), "ground_bin_error": float(abs(lng[0] - np.log(4.0))),
-        "config": {"workload": AT_WORKLOADS["config3"]["workload"], "schedule": "halving",
+        # bin 1 holds only the single-flip excitations E0 + 8J + 4K: 4 ground states x 2N flips
+        "lowest_gap_error": float(abs((lng[1] - lng[0]) - np.log(2.0 * 32 * 32))),
+        "config": {"workload": AT_WORKLOADS["config3"]["workload"] + (", range [-5N, 0)" if a.half_range else ""),
+                   "schedule": "halving",
                    "average_mode": params["average_mode"], "observables": a.observables},
     }), flush=True)
 
