@@ -136,7 +136,7 @@ struct HistIndex {
     }
 };
 
-/* ---------------- glazier EQUAL_WINDOWS (glazier.hpp:306-351) ----------- */
+/* ---------------- glazier EQUAL_WINDOWS (glazier.hpp:69-115) ----------- */
 struct WindowGeom { double min, max, bs; size_t nbins; };
 std::vector<WindowGeom> construct_windows(double gmin, double gmax, double gbs, size_t nw, double ow) {
     std::vector<WindowGeom> w(nw);
@@ -511,43 +511,62 @@ bool walker_flat(const ora_ctx& c, const Window& win, const Walker& w, const std
     return (double)mn >= c.p.flatness * ((double)sum / (double)ne);
 }
 
-/* count-weighted intra-window ln g average (rewl_simulation.hpp:361-454);
- * replicas folded in replica order: acc = t_0; acc += t_r.  average_mode 1
- * weighs every replica 1/R (the alternative of rewl_simulation.hpp:401). */
+/* MPI_Allreduce(SUM) of one double per rank over a window communicator
+ * (rewl_simulation.hpp:383, 416-429) as MPICH's recursive doubling computes
+ * it, simulated rank by rank: with pof2 the largest power of two <= R and
+ * rem = R - pof2, each odd rank below 2 rem adds the value of the even rank
+ * before it and takes new rank r / 2, the ranks from 2 rem on take r - rem;
+ * then, for mask = 1, 2, ..., pof2 / 2, every new rank adds its partner's
+ * (new rank ^ mask) partial sum.  Every rank ends with the same value. */
+double mpich_allreduce_sum(const std::vector<double>& v) {
+    const int R = (int)v.size();
+    int pof2 = 1;
+    while (pof2 * 2 <= R) pof2 *= 2;
+    const int rem = R - pof2;
+    std::vector<double> part((size_t)pof2);
+    for (int r = 0; r < R; ++r) {
+        if (r < 2 * rem) {
+            if (r % 2 == 1) part[(size_t)(r / 2)] = v[(size_t)(r - 1)] + v[(size_t)r];
+        } else {
+            part[(size_t)(r - rem)] = v[(size_t)r];
+        }
+    }
+    for (int mask = 1; mask < pof2; mask <<= 1) {
+        std::vector<double> next(part.size());
+        for (int k = 0; k < pof2; ++k) next[(size_t)k] = part[(size_t)(k ^ mask)] + part[(size_t)k];
+        part.swap(next);
+    }
+    return part[0];
+}
+
+/* count-weighted intra-window ln g average (rewl_simulation.hpp:361-454): the
+ * weighted values summed over the replicas as the reference's MPI_Allreduce
+ * sums them.  average_mode 1 weighs every replica 1/R (the alternative of
+ * rewl_simulation.hpp:401). */
 void average_window(ora_ctx& c, System& S, int wi, bool final_pass) {
     const int R = S.R;
     if (R == 1) return;
     Window& win = S.windows[wi];
     Walker* rep = &c.w[S.first + wi * R];
+    std::vector<double> t((size_t)R);
     for (size_t b = 0; b < win.nbins; ++b) {
-        double acc = 0.;
         if (c.p.average_mode == 1) {
             const double wgt = 1. / static_cast<double>(R);
-            for (int r = 0; r < R; ++r) {
-                double t = wgt * rep[r].lng[b];
-                acc = (r == 0) ? t : acc + t;
-            }
+            for (int r = 0; r < R; ++r) t[(size_t)r] = wgt * rep[r].lng[b];
         } else if (!final_pass) {
             uint64_t tot = 0;
             for (int r = 0; r < R; ++r) tot += rep[r].cnt[b];
             if (tot == 0) continue;
-            for (int r = 0; r < R; ++r) {
-                double t = (static_cast<double>(rep[r].cnt[b]) / static_cast<double>(tot)) * rep[r].lng[b];
-                acc = (r == 0) ? t : acc + t;
-            }
+            for (int r = 0; r < R; ++r)
+                t[(size_t)r] = (static_cast<double>(rep[r].cnt[b]) / static_cast<double>(tot)) * rep[r].lng[b];
         } else {
-            double tot = 0.;
-            for (int r = 0; r < R; ++r) {
-                double v = (double)(rep[r].visits[b] + rep[r].cnt[b]);
-                tot = (r == 0) ? v : tot + v;
-            }
+            std::vector<double> n((size_t)R);
+            for (int r = 0; r < R; ++r) n[(size_t)r] = (double)(rep[r].visits[b] + rep[r].cnt[b]);
+            const double tot = mpich_allreduce_sum(n);
             if (tot == 0.) continue;
-            for (int r = 0; r < R; ++r) {
-                double v = (double)(rep[r].visits[b] + rep[r].cnt[b]);
-                double t = (v / tot) * rep[r].lng[b];
-                acc = (r == 0) ? t : acc + t;
-            }
+            for (int r = 0; r < R; ++r) t[(size_t)r] = (n[(size_t)r] / tot) * rep[r].lng[b];
         }
+        const double acc = mpich_allreduce_sum(t);
         for (int r = 0; r < R; ++r) rep[r].lng[b] = acc;
     }
 }
@@ -1030,7 +1049,7 @@ void ora_mt19937_64(uint64_t seed, int64_t n, uint64_t* out) {
 }
 
 /* ------------------- results path ------------------------------------- */
-/* 4th-order central / one-sided derivatives (numerical_derivatives.hpp:59-116) */
+/* 4th-order central / one-sided derivatives (numerical_derivatives.hpp:46-70) */
 static double deriv(size_t i, const std::vector<double>& f, const std::vector<double>& a) {
     const size_t n = f.size();
     if (i == 0) return (f[1] - f[0]) / (a[1] - a[0]);
@@ -1041,7 +1060,7 @@ static double deriv(size_t i, const std::vector<double>& f, const std::vector<do
 
 /* The reference calls unqualified abs() on a double, which resolves to
  * int abs(int): the argument is truncated toward zero
- * (vector_matching.hpp:193, 200). */
+ * (vector_matching.hpp:107, 116). */
 static double trunc_abs(double v) { return (double)std::abs((int)v); }
 
 /* concatenate_tables_multiple_overlap (concatenate_rewl_tables.hpp:167-305)
@@ -1066,7 +1085,7 @@ int ora_stitch(const ora_ctx* c, int32_t sysi, double states_per_site, int32_t n
     double shifter = 0.;
     for (size_t b = 0; b < et[0].size(); ++b) { fe.push_back(et[0][b]); fl.push_back(lt[0][b] + shifter); }
     for (int k = 1; k < S.W; ++k) {
-        /* find_left_concatenation_indices (vector_matching.hpp:158-207) */
+        /* find_left_concatenation_indices (vector_matching.hpp:94-142) */
         const double right_e = et[k].front();
         long left_start = -1;
         for (long i = (long)fe.size() - 1; i != -1; --i) if (fe[i] == right_e) { left_start = i; break; }

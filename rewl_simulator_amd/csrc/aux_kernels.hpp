@@ -287,6 +287,28 @@ __device__ __forceinline__ T wave_sum(T v) {
 
 constexpr int kMaxCheckBins = 8192;   // largest window the visited-support flatness test handles
 
+// The sum of v(0) ... v(R-1) over a window's replicas, associated as the
+// reference's MPI_Allreduce(SUM) over the window communicator associates it
+// (rewl_simulation.hpp:383, 416-429) under MPICH's recursive doubling: with
+// pof2 the largest power of two <= R and rem = R - pof2, the first 2 rem
+// values are added in pairs, then the pof2 partial values are added as a
+// balanced binary tree in rank order.  The tree is built left to right with
+// a carry stack (one partial sum per level).
+template <class F>
+__device__ __forceinline__ double replica_sum(int R, F v) {
+    int pof2 = 1, top = 0;
+    while (pof2 * 2 <= R) { pof2 *= 2; ++top; }
+    const int rem = R - pof2;
+    double st[9];
+    for (int k = 0; k < pof2; ++k) {
+        double x = k < rem ? v(2 * k) + v(2 * k + 1) : v(k + rem);
+        int lv = 0;
+        while ((k >> lv) & 1) { x = st[lv] + x; ++lv; }
+        st[lv] = x;
+    }
+    return st[top];
+}
+
 struct CheckParams {
     int32_t flatness_mode;
     int32_t schedule;
@@ -354,11 +376,9 @@ __global__ void __launch_bounds__(kLanes) k_check(KArgs a, CheckParams cp, int32
         if (R > 1 && cp.average_mode == 1) {
             const double wgt = 1. / static_cast<double>(R);
             for (int b = lane; b < nb; b += kLanes) {
-                double acc = 0.;
-                for (int r = 0; r < R; ++r) {
-                    const double t = wgt * a.lng[a.w_hist[W.first_walker + r] + (int64_t)b * kLanes];
-                    acc = (r == 0) ? t : acc + t;
-                }
+                const double acc = replica_sum(R, [&](int r) {
+                    return wgt * a.lng[a.w_hist[W.first_walker + r] + (int64_t)b * kLanes];
+                });
                 for (int r = 0; r < R; ++r) a.lng[a.w_hist[W.first_walker + r] + (int64_t)b * kLanes] = acc;
             }
         } else if (R > 1) {
@@ -366,12 +386,10 @@ __global__ void __launch_bounds__(kLanes) k_check(KArgs a, CheckParams cp, int32
                 unsigned long long tot = 0ull;
                 for (int r = 0; r < R; ++r) tot += a.cnt[a.w_hist[W.first_walker + r] + (int64_t)b * kLanes];
                 if (tot == 0ull) continue;
-                double acc = 0.;
-                for (int r = 0; r < R; ++r) {
+                const double acc = replica_sum(R, [&](int r) {
                     const int64_t i = a.w_hist[W.first_walker + r] + (int64_t)b * kLanes;
-                    const double t = (static_cast<double>(a.cnt[i]) / static_cast<double>(tot)) * a.lng[i];
-                    acc = (r == 0) ? t : acc + t;
-                }
+                    return (static_cast<double>(a.cnt[i]) / static_cast<double>(tot)) * a.lng[i];
+                });
                 for (int r = 0; r < R; ++r) a.lng[a.w_hist[W.first_walker + r] + (int64_t)b * kLanes] = acc;
             }
         }
@@ -421,28 +439,21 @@ __global__ void __launch_bounds__(kLanes) k_finalize(KArgs a, int average_mode) 
     for (int b = threadIdx.x; b < W.nbins; b += kLanes) {
         if (average_mode == 1) {
             const double wgt = 1. / static_cast<double>(R);
-            double acc = 0.;
-            for (int r = 0; r < R; ++r) {
-                const double t = wgt * a.lng[a.w_hist[W.first_walker + r] + (int64_t)b * kLanes];
-                acc = (r == 0) ? t : acc + t;
-            }
+            const double acc = replica_sum(R, [&](int r) {
+                return wgt * a.lng[a.w_hist[W.first_walker + r] + (int64_t)b * kLanes];
+            });
             for (int r = 0; r < R; ++r) a.lng[a.w_hist[W.first_walker + r] + (int64_t)b * kLanes] = acc;
             continue;
         }
-        double tot = 0.;
-        for (int r = 0; r < R; ++r) {
+        auto updates = [&](int r) {
             const int64_t i = a.w_hist[W.first_walker + r] + (int64_t)b * kLanes;
-            const double v = (double)(a.visits[i] + a.cnt[i]);
-            tot = (r == 0) ? v : tot + v;
-        }
+            return (double)(a.visits[i] + a.cnt[i]);
+        };
+        const double tot = replica_sum(R, updates);
         if (tot == 0.) continue;
-        double acc = 0.;
-        for (int r = 0; r < R; ++r) {
-            const int64_t i = a.w_hist[W.first_walker + r] + (int64_t)b * kLanes;
-            const double v = (double)(a.visits[i] + a.cnt[i]);
-            const double t = (v / tot) * a.lng[i];
-            acc = (r == 0) ? t : acc + t;
-        }
+        const double acc = replica_sum(R, [&](int r) {
+            return (updates(r) / tot) * a.lng[a.w_hist[W.first_walker + r] + (int64_t)b * kLanes];
+        });
         for (int r = 0; r < R; ++r) a.lng[a.w_hist[W.first_walker + r] + (int64_t)b * kLanes] = acc;
     }
 }

@@ -60,7 +60,8 @@ def system_strings(system):
     """(model_name, file_name_base, system header) of one system description."""
     L = int(system["L"])
     N = L * L
-    nb = int(round((system["e_max"] - system["e_min"]) / system["bin_size"]))
+    # num_bins as main.cpp:117 prints it: static_cast<size_t>((max - min) / bin)
+    nb = int((system["e_max"] - system["e_min"]) / system["bin_size"])
     field = system.get("field") is not None or float(system.get("h", 0.0)) != 0.0
     if system["model"] == abi.ISING2D:
         model = "Ising2d" + ("_RFIM" if system.get("field") is not None else "")
@@ -109,8 +110,8 @@ def file_header(system_header, rewl_hdr):
     return system_header + "\n#\n" + rewl_hdr + "\n#\n"
 
 
-def _names_block(first, names):
-    out = "# Intensive Observable Names by Column"
+def _names_block(first, names, title="# Intensive Observable Names by Column"):
+    out = title
     counter = 0
     for n in first:
         counter += 1
@@ -134,11 +135,17 @@ def write_microcanonical(path, base, header, obs_names, E, lng, obs, system_size
     return fn
 
 
+# the nonlinear file titles its column list differently
+# (write_nonlinear_observables.hpp:29; write_self_averaged_observables.hpp:31)
+NONLINEAR_TITLE = "# Intensive Nonlinear Observable Names by Column"
+
+
 def write_temperature_table(path, prefix, base, header, names, T, values):
     fn = os.path.join(path, prefix + "-" + base + ".txt")
     rows = [DELIMITER.join([_sci(T[t])] + [_sci(v) for v in values[t]]) for t in range(len(T))]
+    title = NONLINEAR_TITLE if prefix == "nonlinear_observables" else "# Intensive Observable Names by Column"
     with open(fn, "w") as f:
-        f.write(header + _names_block(["Temperature"], names) + "\n".join(rows))
+        f.write(header + _names_block(["Temperature"], names, title) + "\n".join(rows))
     return fn
 
 
