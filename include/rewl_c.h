@@ -14,7 +14,7 @@
  * Entry points and the reference interface each one replaces:
  *   rewl_create            REWL_simulation ctor + glazier::construct_windows +
  *                          REWL_Walker ctor + import_DoFs/import_disorder
- *                          (rewl_simulation.hpp:69-108, glazier.hpp:306-351,
+ *                          (rewl_simulation.hpp:69-108, glazier.hpp:69-115,
  *                           rewl_walker.hpp:118-139, main.cpp:201-208)
  *   rewl_adjust_to_range   REWL_Walker::adjust_state_to_range (rewl_walker.hpp:71-115)
  *   rewl_sweeps            REWL_Walker::wang_landau_walk -> Wang_Landau::
@@ -57,6 +57,12 @@ extern "C" {
 #define REWL_ERR_NO_DEVICE     -5   /* no GPU visible                            */
 #define REWL_ERR_REPLAY        -6   /* replay stream exhausted                   */
 #define REWL_ERR_RANGE         -7   /* a walker could not be brought into range  */
+/* What an error leaves behind: REWL_ERR_INVALID, REWL_ERR_ITER_CAP and
+ * REWL_ERR_RANGE leave the context usable (after REWL_ERR_RANGE, import other
+ * spins or widen the windows and call rewl_adjust_to_range again; each call
+ * starts from a clear range flag).  After REWL_ERR_REPLAY the replay streams
+ * are spent and after REWL_ERR_DEVICE or REWL_ERR_NONFINITE the device state
+ * is undefined: destroy the context. */
 
 /* observables per bin (rewl_params.observables): Ising |m|, m^2, m^4; Ashkin-
  * Teller sigma |m|, m^2, m^4, tau |m|, m^2, m^4, phi, phi^2, phi^4, Baxter |m|,
@@ -142,7 +148,9 @@ typedef struct rewl_params {
     int32_t sweeps_per_exchange; /* reference: 1 (rewl_parameters.hpp:28)             */
     int32_t reduce_logdos;       /* REDUCE_LOGDOS: ln g -= ln g[0] after a flat event */
     int32_t rng_mode;            /* enum rewl_rng                                     */
-    int32_t walkers_per_group;   /* lanes used per wavefront (1..64); 0 = autotune    */
+    int32_t walkers_per_group;   /* walkers per wavefront (1..64); 0 = the largest
+                                    count whose histograms and bound ring fit the
+                                    LDS of one workgroup, per kernel               */
     int32_t exchange;            /* 0 disables replica exchange (independent walkers) */
     int32_t producers;           /* Philox fast path: wavefronts per workgroup that
                                     precompute the acceptance thresholds; 0 = auto,
@@ -269,6 +277,11 @@ int rewl_get_densities(rewl_ctx* ctx, int32_t walker, uint32_t* out, int32_t nbi
 int rewl_set_logdos(rewl_ctx* ctx, int32_t walker, const double* lng, int32_t n);
 int rewl_set_window_lnf(rewl_ctx* ctx, int32_t window_index, double lnf);
 int rewl_set_spins(rewl_ctx* ctx, int32_t walker, const int8_t* spins, int32_t n);
+/* import_DoFs for every walker of one system (main.cpp:194-210: the master's
+ * configuration is broadcast and each rank imports it): n = num_DoF values
+ * +-1 in the reference's layout.  Energies are recomputed once for the whole
+ * context; rewl_adjust_to_range must run before the next sweep. */
+int rewl_set_system_spins(rewl_ctx* ctx, int32_t system, const int8_t* spins, int32_t n);
 
 /* ---- results path ---------------------------------------------------------- */
 /* Stitch the replica-0 ln g of every window of one system and normalise it to

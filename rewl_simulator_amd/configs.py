@@ -112,7 +112,7 @@ def config3(lnf_final=1e-7):
 def gaussian_strain(L, sigma, seed):
     """Per-site random strain h_i ~ N(0, sigma^2) from a realisation seed
     (numpy PCG64; the reference seeds its fields from the clock,
-    random_fields.hpp:279-313, so no realisation of it can be reproduced)."""
+    random_fields.hpp:41-49, so no realisation of it can be reproduced)."""
     rng = np.random.default_rng(seed)
     return rng.normal(0.0, sigma, size=L * L)
 

@@ -75,7 +75,7 @@ __device__ __forceinline__ bool exp_less(double u, double x) {
     return u < exp_rn_slow(x);
 }
 
-// array_statistics (simulated_annealing_utilities.hpp:28-49): the mean, then
+// array_statistics (simulated_annealing_utilities.hpp:26-45): the mean, then
 // the population standard deviation about it, in index order.
 __device__ __forceinline__ void block_statistics(const double* e, int n, double& avg, double& sd) {
     double a = 0.;

@@ -81,7 +81,7 @@ struct KArgs {
 };
 
 // The reference's in-range test adds 0.5 and truncates toward zero
-// (histogram_index.hpp:337-345); for integer energies this maps e -> e + (e<0).
+// (histogram_index.hpp:22-30); for integer energies this maps e -> e + (e<0).
 __device__ __forceinline__ int phi_int(int e) { return e - (e >> 31); }
 
 __device__ __forceinline__ int trunc_half(double e) { return (int)(e + 0.5); }

@@ -1,7 +1,7 @@
 // Host side of the REWL engine: the C ABI declared in include/rewl_c.h.
 //
 // One rewl_ctx owns every walker of one or more REWL systems on one GPU.  The
-// host builds the windows (glazier EQUAL_WINDOWS, glazier.hpp:306-351), packs
+// host builds the windows (glazier EQUAL_WINDOWS, glazier.hpp:69-115), packs
 // walkers into wavefront groups, uploads the tables, and then drives the
 // device with four kinds of launch that replace the reference's per-rank loop
 // and its MPI traffic: sweep (one lane per walker), exchange (one wavefront
@@ -79,7 +79,7 @@ struct HostSystem {
     std::vector<uint8_t> exclude;
 };
 
-// glazier::construct_windows, EQUAL_WINDOWS variant (glazier.hpp:306-351):
+// glazier::construct_windows, EQUAL_WINDOWS variant (glazier.hpp:69-115):
 // equal widths, each window start snapped onto the previous window's bin grid,
 // the end capped at the global maximum; windows are half-open [min, max).
 std::vector<HostWindow> make_windows(const rewl_system_desc& d) {
@@ -230,6 +230,17 @@ int check_err_flag(rewl_ctx* c) {
     HIP_TRY(hipStreamSynchronize(c->stream));
     if (e & 1) return fail(REWL_ERR_REPLAY, "replay stream exhausted");
     if (e & 2) return fail(REWL_ERR_RANGE, "a walker could not be brought into its window");
+    return REWL_OK;
+}
+
+int clear_err_bits(rewl_ctx* c, int32_t bits) {
+    int32_t e = 0;
+    HIP_TRY(hipMemcpyAsync(&e, c->d_err.p, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (!(e & bits)) return REWL_OK;
+    e &= ~bits;
+    HIP_TRY(hipMemcpyAsync(c->d_err.p, &e, sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
     return REWL_OK;
 }
 
@@ -845,6 +856,9 @@ int rewl_destroy(rewl_ctx* c) {
 int rewl_adjust_to_range(rewl_ctx* c) {
     if (!c) return fail(REWL_ERR_INVALID, "null context");
     int rc;
+    // a failed earlier adjustment left the range bit set; it describes that
+    // attempt only (the replay-exhausted bit stays: the streams are spent)
+    if ((rc = clear_err_bits(c, 2))) return rc;
     if ((rc = replay_topup(c))) return rc;
     uint64_t steps = (uint64_t)max_ndof(c) * 4096ull;
     const int mode = c->p.adjust_mode;
@@ -1126,10 +1140,9 @@ int rewl_set_window_lnf(rewl_ctx* c, int32_t wi, double lnf) {
     return REWL_OK;
 }
 
-int rewl_set_spins(rewl_ctx* c, int32_t g, const int8_t* spins, int32_t n) {
-    if (!walker_ok(c, g) || !spins) return fail(REWL_ERR_INVALID, "bad walker");
-    const HostSystem& S = c->sys[c->win[c->w_window[g]].sys];
-    if (n != S.ndof) return fail(REWL_ERR_INVALID, "spin array length must equal the number of degrees of freedom");
+// Bit-packed rows (plane-major, bit x of row y = site y*L+x, 1 = spin -1) of a
+// configuration given in the reference's DoF layout.
+static std::vector<uint64_t> pack_rows(const HostSystem& S, const int8_t* spins) {
     const int planes = S.d.model == REWL_ISING2D ? 1 : 2, L = S.d.L;
     std::vector<uint64_t> words((size_t)planes * L, 0ull);
     for (int i = 0; i < S.N; ++i) {
@@ -1139,8 +1152,18 @@ int rewl_set_spins(rewl_ctx* c, int32_t g, const int8_t* spins, int32_t n) {
             if (v < 0) words[(size_t)q * L + y] |= (1ull << x);
         }
     }
+    return words;
+}
+
+static int upload_rows(rewl_ctx* c, int32_t g, const std::vector<uint64_t>& words) {
     HIP_TRY(hipMemcpy2DAsync(c->d_lat.p + c->w_lat[g], kLanes * sizeof(uint64_t), words.data(), sizeof(uint64_t),
                              sizeof(uint64_t), words.size(), hipMemcpyHostToDevice, c->stream));
+    return REWL_OK;
+}
+
+// Energies of every walker from its lattice, then the walkers count as
+// outside their windows until the next rewl_adjust_to_range.
+static int recalc_all(rewl_ctx* c) {
     hipLaunchKernelGGL(k_recalc, dim3((c->num_walkers + 63) / 64), dim3(64), 0, c->stream, c->args);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(c->stream));
@@ -1148,8 +1171,28 @@ int rewl_set_spins(rewl_ctx* c, int32_t g, const int8_t* spins, int32_t n) {
     return REWL_OK;
 }
 
+int rewl_set_spins(rewl_ctx* c, int32_t g, const int8_t* spins, int32_t n) {
+    if (!walker_ok(c, g) || !spins) return fail(REWL_ERR_INVALID, "bad walker");
+    const HostSystem& S = c->sys[c->win[c->w_window[g]].sys];
+    if (n != S.ndof) return fail(REWL_ERR_INVALID, "spin array length must equal the number of degrees of freedom");
+    int rc;
+    if ((rc = upload_rows(c, g, pack_rows(S, spins)))) return rc;
+    return recalc_all(c);
+}
+
+int rewl_set_system_spins(rewl_ctx* c, int32_t si, const int8_t* spins, int32_t n) {
+    if (!c || si < 0 || si >= (int32_t)c->sys.size() || !spins) return fail(REWL_ERR_INVALID, "bad system");
+    const HostSystem& S = c->sys[si];
+    if (n != S.ndof) return fail(REWL_ERR_INVALID, "spin array length must equal the number of degrees of freedom");
+    const std::vector<uint64_t> words = pack_rows(S, spins);
+    int rc;
+    for (int32_t g = 0; g < c->num_walkers; ++g)
+        if (c->win[c->w_window[g]].sys == si && (rc = upload_rows(c, g, words))) return rc;
+    return recalc_all(c);
+}
+
 // ---- results path -------------------------------------------------------------
-// Derivative of a tabulated function as numerical_derivatives.hpp:93-116 takes
+// Derivative of a tabulated function as numerical_derivatives.hpp:46-70 takes
 // it: one-sided at the ends, second-order next to them, fourth-order inside.
 static double tab_deriv(size_t i, const std::vector<double>& f, const std::vector<double>& e) {
     const size_t n = f.size();
@@ -1160,7 +1203,7 @@ static double tab_deriv(size_t i, const std::vector<double>& f, const std::vecto
 }
 
 // |x| as the reference evaluates it: its unqualified abs() on a double binds
-// to int abs(int), truncating toward zero (vector_matching.hpp:193, 200).
+// to int abs(int), truncating toward zero (vector_matching.hpp:107, 116).
 static double int_abs(double v) { return (double)std::abs((int)v); }
 
 int rewl_stitch(rewl_ctx* c, int32_t si, double states_per_site, double* energies, double* logdos, int32_t capacity,

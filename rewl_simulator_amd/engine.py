@@ -284,6 +284,13 @@ class Engine:
         buf = np.ascontiguousarray(spins, dtype=np.int8)
         abi.check(self.lib.rewl_set_spins(self.ctx, int(g), buf.ctypes.data_as(C.POINTER(C.c_int8)), buf.size))
 
+    def set_system_spins(self, system, spins):
+        """Every walker of one system starts from `spins` (import_DoFs after
+        the master's broadcast, main.cpp:194-210); adjust before sweeping."""
+        buf = np.ascontiguousarray(spins, dtype=np.int8)
+        abi.check(self.lib.rewl_set_system_spins(self.ctx, int(system), buf.ctypes.data_as(C.POINTER(C.c_int8)),
+                                                 buf.size))
+
     # ---- results ------------------------------------------------------------------
     def stitched(self, system=0, states_per_site=None, join="reference"):
         """Stitched ln g of one system; normalised to states_per_site^N when

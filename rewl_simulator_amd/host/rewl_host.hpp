@@ -9,7 +9,7 @@
 //     variants (RFIM, RFAT_BAXTER) as a per-site field;
 //   * the selector aliases Hamiltonian_t / System_Parameters of
 //     model_hamiltonians.hpp:6-40, chosen here by a template argument;
-//   * REWL_Parameters (rewl_parameters.hpp:30-49);
+//   * REWL_Parameters (rewl_parameters.hpp:20-39);
 //   * REWL_simulation<Hamiltonian> with simulate() and the results path of
 //     main.cpp:201-365 (stitch, normalise, thermodynamics).
 // Nothing here computes on the CPU: every step is a call into librewl_hip.so.
@@ -64,7 +64,7 @@ struct Ashkin_Teller2d_Parameters {
     int num_DoF() const { return 2 * N(); }
     double ground_state_degeneracy() const { return 4.0; }
     double states_per_site() const { return 4.0; }
-    // [-(2J+K) 2N, 0), bin 8(J+K) (ashkin_teller2d_parameters.cxx:417-427)
+    // [-(2J+K) 2N, 0), bin 8(J+K) (ashkin_teller2d_parameters.cxx:35-41)
     double energy_min() const { return -0.5 * (2 * J + K) * 4.0 * N(); }
     double energy_max() const { return 0.0; }
     double energy_bin_size() const { return (J + K) * 4.0 * 2.0; }
@@ -98,7 +98,7 @@ struct Ashkin_Teller2d {
     }
 };
 
-// ---- REWL parameters (rewl_parameters.hpp:30-49) ---------------------------------
+// ---- REWL parameters (rewl_parameters.hpp:20-39) ---------------------------------
 struct REWL_Parameters {
     int num_windows = 8;
     int replicas_per_window = 1;
@@ -222,7 +222,6 @@ class REWL_simulation {
         p.observables = rp.observables;
         p.densities = rp.densities ? 1 : 0;
         check(rewl_create(&d, 1, &p, &ctx_));
-        check(rewl_adjust_to_range(ctx_));
     }
 
     // the reference's default range for the model
@@ -230,12 +229,18 @@ class REWL_simulation {
         : REWL_simulation(sp, rp, sp.energy_min(), sp.energy_max(), sp.energy_bin_size()) {}
 
     // import_DoFs (main.cpp:202): every walker starts from `spins` (num_DoF
-    // values +-1 in the reference layout) and is brought into its window again.
+    // values +-1 in the reference layout).  As in main.cpp:202-210 the walkers
+    // are brought into their windows once, after the import (adjust_state_to_range,
+    // or lazily by simulate), so each replay stream spends one start draw.
     void import_DoFs(const std::vector<int8_t>& spins) {
-        int32_t nw = 0;
-        check(rewl_num_walkers(ctx_, &nw));
-        for (int32_t g = 0; g < nw; ++g) check(rewl_set_spins(ctx_, g, spins.data(), (int32_t)spins.size()));
+        check(rewl_set_system_spins(ctx_, 0, spins.data(), (int32_t)spins.size()));
+        adjusted_ = false;
+    }
+
+    // REWL_Walker::adjust_state_to_range for every walker (main.cpp:210).
+    void adjust_state_to_range() {
         check(rewl_adjust_to_range(ctx_));
+        adjusted_ = true;
     }
 
     REWL_simulation(const REWL_simulation&) = delete;
@@ -245,6 +250,7 @@ class REWL_simulation {
     // REWL_simulation::simulate: until every window is at the final increment.
     // Returns false when max_checks check cycles ran first.
     bool simulate(int64_t max_checks = INT64_MAX / 2) {
+        if (!adjusted_) adjust_state_to_range();
         const int rc = rewl_run(ctx_, max_checks);
         if (rc == REWL_ERR_ITER_CAP) return false;
         check(rc);
@@ -391,6 +397,7 @@ class REWL_simulation {
     REWL_Parameters rp_;
     std::vector<uint8_t> exclude_;
     rewl_ctx* ctx_ = nullptr;
+    bool adjusted_ = false;   // walkers inside their windows (adjust_state_to_range ran after the last import)
 };
 
 }  // namespace rewl_host

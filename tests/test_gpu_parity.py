@@ -215,6 +215,48 @@ def test_production_configs_few_sweeps_bit_exact(gpu, cfg):
         np.testing.assert_array_equal(eng.counts(g).astype(np.uint64), ora.counts(g))
 
 
+def test_config4_full_size_bit_exact(gpu):
+    """The batched random-strain Ashkin-Teller workload at its stated size:
+    256 realisations (64 each of L = 16, 24, 32, 48), 8 windows x 4 walkers,
+    512 bins, 8192 walkers in one context, so the field-model kernel runs in
+    its production launch shape (one 32-walker realisation per workgroup,
+    512-bin windows, the L = 48 field and bound ring in LDS, mixed L in one
+    grid).  Sweeps, an exchange of each phase and two checks; the first and
+    last walker of every realisation plus a random sample are compared bit
+    for bit with the oracle, which runs each realisation with its own
+    streams."""
+    print("case: config4 full size (256 realisations, 8192 walkers)", flush=True)
+    systems, params = configs.config4()
+    assert len(systems) == 256 and sorted({s["L"] for s in systems}) == [16, 24, 32, 48]
+    eng, ora = make_pair(systems, params)
+    assert eng.num_walkers == 8192
+    eng.adjust_to_range(); ora.adjust_to_range()
+    eng.sweeps(3); ora.sweeps(3)
+    eng.exchange(); ora.exchange()
+    eng.sweeps(2); ora.sweeps(2)
+    eng.check(); ora.check()
+    eng.sweeps(4); ora.sweeps(4)
+    eng.exchange(); ora.exchange()
+    eng.sweeps(1); ora.sweeps(1)
+    eng.check(); ora.check()
+    per = 8 * 4
+    sample = set()
+    for r in range(len(systems)):
+        sample |= {r * per, r * per + per - 1}
+    sample |= set(np.random.default_rng(4).integers(0, eng.num_walkers, 64).tolist())
+    for g in sorted(sample):
+        st = ora.state(g)
+        assert eng.energy(g) == st["E"], f"energy of walker {g} (L={systems[g // per]['L']})"
+        assert eng.flavor(g) == st["flavor"], g
+        np.testing.assert_array_equal(eng.spins(g), ora.spins(g), err_msg=f"spins of walker {g}")
+        np.testing.assert_array_equal(bits(eng.logdos(g)), bits(ora.logdos(g)), err_msg=f"ln g of walker {g}")
+        np.testing.assert_array_equal(eng.counts(g).astype(np.uint64), ora.counts(g), err_msg=f"counts {g}")
+        np.testing.assert_array_equal(eng.visits(g), ora.visits(g), err_msg=f"visits {g}")
+    window_states_equal(eng, ora)
+    assert eng.stats()["exchanges_accepted"] == ora.stats()["exchanges_accepted"]
+    eng.close()
+
+
 @pytest.mark.parametrize("producers", [-1, 1, 2, 3])
 @pytest.mark.parametrize("model", [abi.ISING2D, abi.ASHKIN_TELLER2D])
 def test_fast_path_producers_bit_exact(gpu, producers, model):
