@@ -14,16 +14,23 @@ CPU-baseline sample of the oracle on the host cores.
 import argparse
 import json
 import os
+import signal
+import subprocess
 import sys
+import tempfile
+import threading
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-# LDS, where the histograms live: 256 CUs x 128 B/clk x 2.4 GHz
-LDS_PEAK_GBS = 256 * 128 * 2.4
+# LDS, where the histograms live: nominal 256 CUs x 128 B/clk x 2.4 GHz; the
+# bench measures the device's own figure (rewl_selfcheck(2)) and uses that
+LDS_NOMINAL_GBS = 256 * 128 * 2.4
 BYTES_PER_ATTEMPT = 32.25   # 8 B read ln g[new] + 16 B RMW ln g[cur] + 8 B RMW count + 0.25 B lattice
+REFERENCE_EXE = os.path.join(ROOT, "oracle", "_ref", "rewl_ref_ising_L64")
+MPIEXEC = "/opt/conda/bin/mpiexec"
 # Measured HBM bytes of one sweep launch of this workload with the default
 # kernel, keyed by sweeps per launch: rocprofv3 --pmc FETCH_SIZE (x 2.0) and
 # WRITE_SIZE (x 1.0), the scales read off a plain 256 MiB copy
@@ -38,8 +45,14 @@ def parse():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--sweeps-per-step", type=int, default=1000)
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target length of the CPU-baseline sample")
-    ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, host cores)")
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="CPU-baseline ranks / threads; 0 = the host CPUs this process may use")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-reference", action="store_true",
+                    help="time only the oracle as the CPU baseline, not the reference program")
+    ap.add_argument("--cadence-sweeps", type=int, default=200,
+                    help="also time this many sweeps with an exchange after every sweep (the reference's cadence); "
+                         "0 to skip")
     ap.add_argument("--walkers-per-group", type=int, default=0)
     ap.add_argument("--producers", type=int, default=0, help="producer wavefronts (0 auto, -1 none)")
     ap.add_argument("--sweep-kernel", type=int, default=0,
@@ -85,6 +98,83 @@ def calibrate_copy(nbytes):
     print(json.dumps({"calibration_copy_bytes": n * 4, "launches": 3}), flush=True)
 
 
+def host_cpus():
+    """The host CPUs this process may use: the cgroup CPU quota when one is set
+    (the GPU box's share of its host), else the affinity mask; with nproc and
+    the CPU model for the record."""
+    info = {"os_cpu_count": os.cpu_count(), "affinity": len(os.sched_getaffinity(0))}
+    quota = None
+    try:
+        q, period = open("/sys/fs/cgroup/cpu.max").read().split()
+        if q != "max":
+            quota = max(1, int(int(q) // int(period)))
+    except (OSError, ValueError):
+        pass
+    info["cgroup_quota"] = quota
+    try:
+        info["nproc"] = int(subprocess.run(["nproc"], capture_output=True, text=True).stdout.strip())
+    except (OSError, ValueError):
+        info["nproc"] = None
+    try:
+        for line in subprocess.run(["lscpu"], capture_output=True, text=True).stdout.splitlines():
+            if line.startswith("Model name:"):
+                info["model"] = line.split(":", 1)[1].strip()
+            elif line.startswith("CPU(s):"):
+                info["lscpu_cpus"] = int(line.split(":", 1)[1])
+    except (OSError, ValueError):
+        pass
+    info["used"] = min(info["affinity"], quota) if quota else info["affinity"]
+    return info
+
+
+def reference_cpu_baseline(target_s, cores):
+    """The reference program itself (built from its sources by
+    oracle/ref_build.sh with its defaults: L = 64 Ising, one walker per rank,
+    exchange every sweep, the 1/t schedule, CORRELATION_LENGTHS) under MPICH's
+    mpiexec with one rank per host core, stopped after about target_s seconds.
+    Its rank 0 prints each iteration's update rate (sweeps x N / seconds,
+    rewl_simulation.hpp:600-614); every rank sweeps as often, so the job's rate
+    is that figure times the rank count.  None when the binary or MPICH is
+    missing."""
+    if not (os.path.exists(REFERENCE_EXE) and os.path.exists(MPIEXEC)):
+        return None
+    rates = []
+    with tempfile.TemporaryDirectory() as d:
+        work = os.path.join(d, "work")
+        os.makedirs(work)
+        proc = subprocess.Popen([MPIEXEC, "-launcher", "fork", "-n", str(cores), REFERENCE_EXE], cwd=work,
+                                stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, start_new_session=True)
+        lines = []
+        reader = threading.Thread(target=lambda: lines.extend(proc.stdout), daemon=True)
+        reader.start()
+        t0 = time.perf_counter()
+        while time.perf_counter() - t0 < target_s and proc.poll() is None:
+            time.sleep(0.5)
+        for sig, wait in ((signal.SIGINT, 5), (signal.SIGKILL, 5)):
+            if proc.poll() is None:
+                try:
+                    os.killpg(proc.pid, sig)
+                except ProcessLookupError:
+                    pass
+                try:
+                    proc.wait(timeout=wait)
+                except subprocess.TimeoutExpired:
+                    pass
+        reader.join(timeout=5)
+        elapsed = time.perf_counter() - t0
+    for line in lines:
+        if "Update rate" in line:
+            rates.append(float(line.split("=")[1].split()[0]))
+    if len(rates) < 2:
+        return None
+    per_rank = sum(rates[1:]) / len(rates[1:])   # the first iteration includes the start-up
+    return dict(value=per_rank * cores, unit="spin-flip attempts/s", cores=cores, kind="reference",
+                sample=f"the reference's L=64 Ising run (its defaults, {cores} MPI ranks = {cores} windows x 1 "
+                       f"walker, exchange every sweep, observables and correlator on) for {elapsed:.0f} s: "
+                       f"{len(rates) - 1} iteration rate(s) of rank 0 after the first, "
+                       f"{per_rank:.3e} attempts/s per rank")
+
+
 def cpu_baseline(systems, params, target_s, threads):
     """The oracle (a single-threaded-per-walker restatement of the reference
     update, FP64 spins) timed on the host cores over a bounded sample of the
@@ -101,7 +191,63 @@ def cpu_baseline(systems, params, target_s, threads):
                        f"initial state, ln f = 1, {att} attempts in {sec:.2f} s")
 
 
-def saturating_batch(systems, params, copies, steps=2):
+def cpu_baselines(a, systems, params, gpu_rate):
+    """cpu_baseline: the reference program on the host cores (kind
+    "reference"), or the oracle when the reference cannot run there (kind
+    "port"); the other one beside it."""
+    cpus = host_cpus()
+    cores = a.cpu_threads or cpus["used"]
+    port = cpu_baseline(systems, params, a.cpu_seconds, cores)
+    port["gpu_over_cpu"] = gpu_rate / port["value"]
+    ref = None if a.no_reference else reference_cpu_baseline(a.cpu_seconds, cores)
+    if ref is None:
+        port["host"] = cpus
+        return port
+    ref["gpu_over_cpu"] = gpu_rate / ref["value"]
+    ref["host"] = cpus
+    ref["port"] = port
+    return ref
+
+
+def lds_peak():
+    """The device's measured LDS read bandwidth (GB/s), rewl_selfcheck(2)."""
+    import ctypes as C
+    from rewl_simulator_amd import abi
+    lib = abi.load_engine()
+    out = C.c_double()
+    abi.check(lib.rewl_selfcheck(2, C.byref(out)))
+    return out.value
+
+
+def reference_cadence(systems, params, sweeps):
+    """The workload at the reference's exchange cadence, an exchange after
+    every sweep (rewl_parameters.hpp:28): one sweep launch and one exchange
+    launch per sweep, the histograms loaded into LDS and written back each
+    time.  Reported beside the headline, which exchanges once per step."""
+    from rewl_simulator_amd.engine import Engine
+    p = dict(params, sweeps_per_exchange=1)
+    eng = Engine(systems, p)
+    eng.adjust_to_range()
+    for _ in range(10):
+        eng.sweeps(1)
+        eng.exchange()
+    eng.synchronize()
+    st0 = eng.stats()
+    t0 = time.perf_counter()
+    for _ in range(sweeps):
+        eng.sweeps(1)
+        eng.exchange()
+    eng.synchronize()
+    t1 = time.perf_counter()
+    st1 = eng.stats()
+    eng.close()
+    att = st1["attempts"] - st0["attempts"]
+    return {"sweeps_per_exchange": 1, "sweeps": sweeps, "attempts_per_s": att / (t1 - t0),
+            "ms_per_sweep_and_exchange": (t1 - t0) * 1e3 / sweeps,
+            "exchanges_accepted": st1["exchanges_accepted"] - st0["exchanges_accepted"]}
+
+
+def saturating_batch(systems, params, copies, peak, steps=2):
     """The same sweep on `copies` independent copies of the workload (own
     seeds) in one context: 2048 walkers fill 63 of 256 CUs, so this shows the
     kernel's throughput on a full chip.  Reported beside the workload's own
@@ -126,7 +272,7 @@ def saturating_batch(systems, params, copies, steps=2):
     rate = att / (t1 - t0)
     return {"copies": copies, "walkers": 2048 * copies, "sweeps_per_launch": n, "launches": steps,
             "attempts_per_s": rate, "kernel_attempts_per_s": att / (kms / 1e3),
-            "lds_GBs": rate * BYTES_PER_ATTEMPT / 1e9, "lds_frac": rate * BYTES_PER_ATTEMPT / 1e9 / LDS_PEAK_GBS}
+            "lds_GBs": rate * BYTES_PER_ATTEMPT / 1e9, "lds_frac": rate * BYTES_PER_ATTEMPT / 1e9 / peak}
 
 
 def exact_binned_lng(E_bins, bin_size):
@@ -195,6 +341,7 @@ def batched(a):
     launches = st1["sweep_launches"] - st0["sweep_launches"]
     avg_launch_s = (st1["kernel_ms_sweep"] - st0["kernel_ms_sweep"]) / 1e3 / max(launches, 1)
     achieved = att / max(launches, 1) * 32.5 / avg_launch_s / 1e9
+    peak = lds_peak()
     line = {
         "metric": w["metric"], "value": rate,
         "unit": "spin-flip attempts/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
@@ -202,15 +349,18 @@ def batched(a):
         "dtype": "f64 energies and ln g" if a.workload == "config4" else "int32 energies, f64 ln g", "data": w["data"],
         "config": {"workload": w["workload"], "walkers": w["walkers"], "sweeps_per_step": n,
                    "observables": a.observables},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": w["kernel"],
-                     "histogram_tier": "LDS", "lds_peak": LDS_PEAK_GBS, "lds_frac": achieved / LDS_PEAK_GBS,
-                     "avg_launch_ms": avg_launch_s * 1e3, "bytes_per_attempt": 32.5},
+        "roofline": {"bound": "lds", "achieved": achieved, "peak": peak, "unit": "GB/s",
+                     "frac": achieved / peak, "traffic": None, "kernel": w["kernel"],
+                     "peak_source": "rewl_selfcheck(2)", "avg_launch_ms": avg_launch_s * 1e3,
+                     "bytes_per_attempt": 32.5,
+                     "hbm": {"peak": HBM_PEAK_GBS, "algorithmic_frac": achieved / HBM_PEAK_GBS}},
     }
     if not a.no_cpu:
-        threads = a.cpu_threads or min(16, os.cpu_count() or 1)
+        cpus = host_cpus()
+        threads = a.cpu_threads or cpus["used"]
         line["cpu_baseline"] = cpu_baseline(systems, params, a.cpu_seconds, threads)
         line["cpu_baseline"]["gpu_over_cpu"] = rate / line["cpu_baseline"]["value"]
+        line["cpu_baseline"]["host"] = cpus
     print(json.dumps(line), flush=True)
 
 
@@ -395,6 +545,10 @@ def main():
     default_launch = a.sweep_kernel in (0, 3) and a.producers == 0 and a.walkers_per_group == 0 and not a.observables
     traffic = MEASURED_TRAFFIC.get(params["sweeps_per_check"]) if default_launch else None
 
+    peak = lds_peak()
+    clock_ghz = 2.4
+    walkers_per_wave = W / max(1, -(-W // 33))
+    cycles = avg_launch_s * clock_ghz * 1e9 / (att_per_launch / W)
     line = {
         "metric": "spin-flip attempts/s; time to ln f=1e-6 (learned warm start vs plain)",
         "value": rate,
@@ -413,6 +567,7 @@ def main():
                         "walkers, flatness 0.8, ln f halving to 1e-8",
             "walkers": W,
             "sweeps_per_step": params["sweeps_per_check"],
+            "sweeps_per_exchange": params["sweeps_per_exchange"],
             "attempts_per_step": attempts // a.steps,
             "parallelism": "one lane per walker, 1 GPU",
             "producers": a.producers,
@@ -420,33 +575,40 @@ def main():
             "observables": a.observables,
         },
         "roofline": {
-            "bound": "hbm",
+            # the histograms (all 32 of the 32.25 algorithmic bytes per
+            # attempt) live in LDS: the bound is the LDS bandwidth measured on
+            # this device; HBM carries only the lattice stream
+            "bound": "lds",
             "achieved": achieved,
-            "peak": HBM_PEAK_GBS,
+            "peak": peak,
             "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
+            "frac": achieved / peak,
             "traffic": traffic,
+            "peak_source": "rewl_selfcheck(2): 16-byte conflict-free LDS reads on every CU of this GPU",
             "kernel": "k_sweep_tuned<Ising, L=64> (row-unrolled sweep)" if a.sweep_kernel in (0, 2, 3, 4)
                       else "k_sweep_fast<Ising> (row kernel)",
-            "histogram_tier": "LDS",
-            "lds_peak": LDS_PEAK_GBS,
-            "lds_frac": achieved / LDS_PEAK_GBS,
-            "hbm_GBs_measured": traffic / avg_launch_s / 1e9 if traffic else None,
             "avg_launch_ms": avg_launch_s * 1e3,
             "attempts_per_launch": att_per_launch,
             "bytes_per_attempt": BYTES_PER_ATTEMPT,
             "kernel_attempts_per_s": att_per_launch / avg_launch_s,
+            "hbm": {"peak": HBM_PEAK_GBS, "measured_GBs": traffic / avg_launch_s / 1e9 if traffic else None,
+                    "frac": traffic / avg_launch_s / 1e9 / HBM_PEAK_GBS if traffic else None},
+            # one walker is one sequential chain: what bounds this workload is
+            # the walker wavefront's time per attempt, not a bandwidth
+            "latency": {"cycles_per_attempt_per_walker": cycles, "clock_ghz_nominal": clock_ghz,
+                        "walker_wavefronts": -(-W // 33), "walkers_per_wavefront": walkers_per_wave,
+                        "cus": 256},
         },
         "state": {"min_lnf_after": lnf, "flat_events": st1["flat_events"],
                   "exchanges_accepted": st1["exchanges_accepted"]},
     }
     eng.close()
     if a.saturate > 0:
-        line["saturating_batch"] = saturating_batch(systems, params, a.saturate)
+        line["saturating_batch"] = saturating_batch(systems, params, a.saturate, peak)
+    if a.cadence_sweeps > 0:
+        line["reference_cadence"] = reference_cadence(systems, params, a.cadence_sweeps)
     if not a.no_cpu:
-        threads = a.cpu_threads or min(16, os.cpu_count() or 1)
-        line["cpu_baseline"] = cpu_baseline(systems, params, a.cpu_seconds, threads)
-        line["cpu_baseline"]["gpu_over_cpu"] = rate / line["cpu_baseline"]["value"]
+        line["cpu_baseline"] = cpu_baselines(a, systems, params, rate)
     else:
         line["cpu_baseline"] = None
     print(json.dumps(line), flush=True)

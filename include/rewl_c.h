@@ -366,7 +366,11 @@ int rewl_anneal(const rewl_system_desc* systems, int32_t num_systems, const rewl
  * Philox word (exhaustive); that kernel's acceptance band is 2^-12.
  * what = 1: number of 32-bit words, of 2^24 sampled (and 0, 1, 2^32-1,
  * 2^32-2), whose acceptance bound B(k) computed on the device differs from
- * the same computation on the host (glibc log). */
+ * the same computation on the host (glibc log).
+ * what = 2: the device's LDS read bandwidth in GB/s, measured with 16-byte
+ * conflict-free reads from every CU (eight workgroups of 256 threads per CU),
+ * best of three timed launches: the peak of the memory tier the sweep kernels
+ * keep their histograms in. */
 int rewl_selfcheck(int32_t what, double* out);
 
 #ifdef __cplusplus

@@ -10,7 +10,7 @@
 # off so that every walker is seeded with 1, rewl_simulation.hpp:93-102), and
 # the lattice size, REWL parameters and field strength are edited in its
 # parameter files, as Scripts/msi_setup_Lsize.sh:16 edits SYSTEM_SIZE_L.  Those
-# edits need a copy of the source tree; it lives in oracle/_ref/src_tmp.<name>
+# edits need a copy of the source tree; it lives in <output dir>/src_tmp.<name>
 # for the duration of the compile only.
 #
 # With REF_FIELD_SEED set, generate_random_field's clock seed
@@ -33,11 +33,12 @@
 #   REF_FIELD_H   SYSTEM_FIELD_H (RFIM) or BAXTER_FIELD_H (RFAT)      [0.6 / 0.8]
 #   REF_FIELD_SEED constant seed of generate_random_field (disorder only)
 #   REF_OPT       optimisation flags       [-O3 -DNDEBUG, CMAKE_BUILD_TYPE Release]
+#   REF_OUT_DIR   where the binary goes                               [oracle/_ref]
 set -euo pipefail
 
 REF=${REF_ROOT:-/root/reference}/src
 HERE=$(cd "$(dirname "${BASH_SOURCE[0]}")" && pwd)
-OUT=$HERE/_ref
+OUT=${REF_OUT_DIR:-$HERE/_ref}
 : "${REF_NAME:?}" "${REF_MODEL:?}" "${REF_L:?}"
 REPL=${REF_REPLICAS:-1}
 FINAL=${REF_FINAL:-1e-7}

@@ -1,6 +1,6 @@
 """Build the HIP engine (gfx950) and the CPU oracle in-tree.
 
-    python -m rewl_simulator_amd.build            # both
+    python -m rewl_simulator_amd.build            # all
     python -m rewl_simulator_amd.build engine     # HIP engine only
 """
 import os
@@ -66,14 +66,35 @@ def build_oracle(verbose=False):
     return ORACLE_SO
 
 
+REFERENCE_SRC = "/root/reference/src"
+REF_BASELINE = os.path.join(ORACLE_DIR, "_ref", "rewl_ref_ising_L64")
+
+
+def build_reference(verbose=False):
+    """The reference program itself for the L = 64 Ising workload (its
+    defaults, every walker seeded with 1), the CPU baseline of kind
+    "reference": compiled from its sources by oracle/ref_build.sh where they
+    exist (the build container); elsewhere the binary built there is used."""
+    if not os.path.isdir(REFERENCE_SRC):
+        return None
+    env = dict(os.environ, REF_NAME="ising_L64", REF_MODEL="ISING2D", REF_L="64")
+    cmd = ["bash", os.path.join(ORACLE_DIR, "ref_build.sh")]
+    if verbose:
+        print("REF_NAME=ising_L64 REF_MODEL=ISING2D REF_L=64 " + " ".join(cmd), flush=True)
+    subprocess.run(cmd, env=env, check=True, stdout=None if verbose else subprocess.DEVNULL)
+    return REF_BASELINE
+
+
 def main(argv):
-    what = argv[1:] or ["engine", "host", "oracle"]
+    what = argv[1:] or ["engine", "host", "oracle", "reference"]
     if "engine" in what:
         print(build_engine(verbose=True))
     if "host" in what:
         print(build_host(verbose=True))
     if "oracle" in what:
         print(build_oracle(verbose=True))
+    if "reference" in what:
+        print(build_reference(verbose=True))
 
 
 if __name__ == "__main__":

@@ -1506,10 +1506,63 @@ int rewl_random_field(uint64_t seed, int32_t distribution, double strength, int3
     return REWL_OK;
 }
 
+// LDS read bandwidth: every thread reads 16 B per iteration from a 16 KiB LDS
+// buffer, consecutive lanes at consecutive 16-byte slots (no bank conflict),
+// eight independent reads in flight per thread; the values are folded into a
+// result that is stored only if it equals a constant it never takes, which
+// keeps the reads live without any global traffic in the loop.
+constexpr int kLdsProbeSlots = 1024;   // 16-byte slots = 16 KiB per workgroup
+__global__ void __launch_bounds__(256) k_lds_probe(int iters, uint32_t* sink) {
+    __shared__ uint4 buf[kLdsProbeSlots];
+    const int t = threadIdx.x;
+    for (int i = t; i < kLdsProbeSlots; i += 256) buf[i] = make_uint4(i, 3 * i, 5 * i, 7 * i);
+    __syncthreads();
+    uint32_t a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+    for (int k = 0; k < iters; k += 8) {
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const uint4 v = buf[(t + (k + j) * 256) & (kLdsProbeSlots - 1)];
+            a0 ^= v.x; a1 += v.y; a2 ^= v.z; a3 += v.w;
+        }
+    }
+    if ((a0 ^ a1 ^ a2 ^ a3) == 0x9e3779b9u) sink[blockIdx.x] = 1u;
+}
+
 int rewl_selfcheck(int32_t what, double* out) {
-    if ((what != 0 && what != 1) || !out) return fail(REWL_ERR_INVALID, "unknown self-check");
+    if ((what < 0 || what > 2) || !out) return fail(REWL_ERR_INVALID, "unknown self-check");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(REWL_ERR_NO_DEVICE, "no HIP device");
+    if (what == 2) {
+        int dev = 0, cus = 0;
+        HIP_TRY(hipGetDevice(&dev));
+        HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        const int blocks = cus * 8, iters = 1 << 16;
+        uint32_t* sink = nullptr;
+        HIP_TRY(hipMalloc((void**)&sink, (size_t)blocks * sizeof(uint32_t)));
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        hipError_t e = hipEventCreate(&e0);
+        if (e == hipSuccess) e = hipEventCreate(&e1);
+        float best = 0.f;
+        for (int rep = 0; rep < 4 && e == hipSuccess; ++rep) {   // the first launch warms the clocks
+            e = hipEventRecord(e0, nullptr);
+            if (e == hipSuccess) {
+                hipLaunchKernelGGL(k_lds_probe, dim3(blocks), dim3(256), 0, nullptr, iters, sink);
+                e = hipGetLastError();
+            }
+            if (e == hipSuccess) e = hipEventRecord(e1, nullptr);
+            if (e == hipSuccess) e = hipEventSynchronize(e1);
+            float ms = 0.f;
+            if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+            if (e == hipSuccess && rep > 0 && (best == 0.f || ms < best)) best = ms;
+        }
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        (void)hipFree(sink);
+        if (e != hipSuccess) return fail(REWL_ERR_DEVICE, std::string("rewl_selfcheck: ") + hipGetErrorString(e));
+        const double bytes = (double)blocks * 256.0 * 16.0 * (double)iters;
+        *out = bytes / (best * 1e-3) / 1e9;   // GB/s
+        return REWL_OK;
+    }
     if (what == 1) {
         // device acceptance bounds (device log) against the same function on the
         // host (glibc log) for 2^24 sampled words and the extreme words

@@ -2,7 +2,7 @@
 
 Each case builds the reference from its sources with oracle/ref_build.sh
 (MPICH under /opt/conda, DIFFERENT_SEEDS off so every walker is seeded with 1),
-runs it with `mpiexec -n <ranks>` in oracle/_ref/runs/<case>/work and keeps
+runs it with `mpiexec -n <ranks>` in oracle/_ref/fixtures/runs/<case>/work and keeps
 what the run wrote:
 
   <case>.microcanonical.txt   the reference's microcanonical_observables file,
@@ -31,7 +31,7 @@ import time
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
 OUT = os.path.join(HERE, "ref")
-REF_OUT = os.path.join(ROOT, "oracle", "_ref")
+REF_OUT = os.path.join(ROOT, "oracle", "_ref", "fixtures")
 MPIEXEC = "/opt/conda/bin/mpiexec"
 
 # name -> build settings (oracle/ref_build.sh) and MPI ranks.  Ranks =
@@ -57,7 +57,7 @@ CASES = {
 
 
 def build(name, c):
-    env = dict(os.environ, REF_NAME=name, REF_MODEL=c["model"], REF_L=str(c["L"]), REF_REPLICAS=str(c["replicas"]),
+    env = dict(os.environ, REF_OUT_DIR=REF_OUT, REF_NAME=name, REF_MODEL=c["model"], REF_L=str(c["L"]), REF_REPLICAS=str(c["replicas"]),
                REF_FINAL=c["final"], REF_ONE_OVER_T=c.get("one_over_t", "ON"), REF_FLATNESS=c.get("flatness", "0.3"),
                REF_DISORDER=c.get("disorder", "none"))
     if "field_h" in c:

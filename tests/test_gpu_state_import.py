@@ -48,25 +48,25 @@ def test_system_import_equals_per_walker_import(gpu):
 
 def test_failed_adjust_then_reachable_state_adjusts(gpu):
     """The reference's range scan flips sigma only on Ashkin-Teller (its flavour
-    toggles twice per wrap), so from all-up the windows above E = N cannot be
-    reached and the adjustment fails.  Starting those walkers from tau in a
-    checkerboard (E in [N, 3N] under sigma flips) the next adjustment
+    toggles twice per wrap), so from all-up it reaches E <= N at most: a
+    system whose range lies above N cannot be brought into its windows and
+    the adjustment fails.  Once that system's walkers start from tau in a
+    checkerboard (E = N, and sigma flips lead up to 3N) the next adjustment
     succeeds: the range error of the first call does not carry over."""
     print("case: failed adjust, import, adjust again", flush=True)
     L, N = 10, 100
-    s = configs.at_system(L, -5.0 * N, 3.0 * N + 8.0, 8.0, 20, 2)
-    p = configs.params(adjust_mode=0, lnf_initial=0.05)
-    eng = Engine([s], p)
+    low = configs.at_system(L, -5.0 * N, -1.0 * N, 8.0, 3, 2)
+    high = configs.at_system(L, 1.2 * N, 2.2 * N, 8.0, 2, 2, seed=9)
+    eng = Engine([low, high], configs.params(adjust_mode=0, lnf_initial=0.05))
     with pytest.raises(abi.REWLError) as e:
         eng.adjust_to_range()
     assert e.value.code == abi.REWL_ERR_RANGE
+    with pytest.raises(abi.REWLError):
+        eng.sweeps(1)
     chk = np.ones(2 * N, dtype=np.int8)
     chk[1::2] = [1 if (i % L + i // L) % 2 == 0 else -1 for i in range(N)]
     assert configs.at_energy(L, chk[0::2], chk[1::2]) == float(N)
-    high = [g for g in range(eng.num_walkers) if eng.windows[eng.walker_window[g]]["e_min"] >= N]
-    assert high
-    for g in high:
-        eng.set_spins(g, chk)
+    eng.set_system_spins(1, chk)
     eng.adjust_to_range()
     eng.sweeps(10)
     for g in range(eng.num_walkers):
