@@ -70,6 +70,9 @@ def parse():
                     help="instead of the throughput steps, time the run from the first sweep until every "
                          "window's ln f is below LNF")
     ap.add_argument("--max-checks", type=int, default=200000)
+    ap.add_argument("--refine", type=int, default=0, metavar="SWEEPS",
+                    help="with --converge: then SWEEPS sweeps of multicanonical refinement (rewl_refine) and "
+                         "the ln g error after it")
     ap.add_argument("--half-range", action="store_true",
                     help="with --workload config3 --converge: the reference's own Ashkin-Teller range [-5N, 0) "
                          "instead of the full spectrum")
@@ -450,13 +453,29 @@ def converge(eng, params, a):
     st1 = eng.stats()
     import numpy as np
     eng.finalize()
-    E, lng = eng.stitched(0, states_per_site=2.0)
-    ex = exact_binned_lng(E, params_bin_size(eng))
-    ok = np.isfinite(ex)
-    d = (lng - ex)[ok]
-    E2, lng2 = eng.stitched(0, states_per_site=2.0, join="overlap")
-    ok2 = np.isfinite(ex[:len(lng2)]) & np.isfinite(lng2)
-    d2 = (lng2 - ex[:len(lng2)])[ok2]
+
+    def errors():
+        E, lng = eng.stitched(0, states_per_site=2.0)
+        ex = exact_binned_lng(E, params_bin_size(eng))
+        ok = np.isfinite(ex)
+        d = (lng - ex)[ok]
+        E2, lng2 = eng.stitched(0, states_per_site=2.0, join="overlap")
+        ok2 = np.isfinite(ex[:len(lng2)]) & np.isfinite(lng2)
+        d2 = (lng2 - ex[:len(lng2)])[ok2]
+        return E, lng, ex, ok, d, d2
+
+    E, lng, ex, ok, d, d2 = errors()
+    refined = None
+    if a.refine > 0:
+        t2 = time.perf_counter()
+        eng.refine(a.refine, params["sweeps_per_exchange"])
+        eng.synchronize()
+        t3 = time.perf_counter()
+        _, _, _, _, r1, r2 = errors()
+        refined = {"sweeps": a.refine, "seconds": t3 - t2,
+                   "lng_max_abs_error_normalised": float(np.max(np.abs(r1))),
+                   "lng_max_abs_error_overlap_join": float(np.max(np.abs(r2))),
+                   "total_seconds": (t1 - t0) + (t3 - t2)}
     if a.dump:
         nbmax = max(w["num_bins"] for w in eng.windows)
         wl = np.full((eng.num_windows, nbmax), np.nan)
@@ -474,6 +493,7 @@ def converge(eng, params, a):
         "lng_max_abs_error_anchored": float(np.max(np.abs(d - d[0]))),
         "lng_bins_compared": int(ok.sum()),
         "lng_max_abs_error_overlap_join": float(np.max(np.abs(d2))),
+        "refined": refined,
         "config": {"workload": "2D Ising REWL, L=64 periodic, 32 windows x 64 walkers, flatness 0.8",
                    "schedule": "1/t" if a.one_over_t else "halving", "average_mode": params["average_mode"]},
     }), flush=True)

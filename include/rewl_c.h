@@ -244,6 +244,17 @@ int rewl_check(rewl_ctx* ctx);
  * done or max_checks cycles have run (REWL_ERR_ITER_CAP). */
 int rewl_run(rewl_ctx* ctx, int64_t max_checks);
 int rewl_finalize(rewl_ctx* ctx);
+/* No counterpart in the reference: a multicanonical refinement after the
+ * Wang-Landau run.  The replicas of every window are averaged (rewl_finalize),
+ * every window's ln f is set to 0, so that walkers sample with the fixed
+ * weights 1/g_est(E), the visit and count histograms are cleared, and
+ * `sweeps` sweeps run with a replica exchange after every sweeps_per_exchange
+ * sweeps (when exchange is on); then every window's ln g(b) becomes
+ * ln g(b) + ln H(b), H the window's histogram summed over its replicas (bins
+ * no replica visited keep their ln g), for every replica.  The statistical
+ * error of the result falls as 1 / sqrt(sweeps), without the saturation of
+ * Wang-Landau's own estimate; stitch afterwards as after rewl_finalize. */
+int rewl_refine(rewl_ctx* ctx, int64_t sweeps, int32_t sweeps_per_exchange);
 int rewl_synchronize(rewl_ctx* ctx);
 
 /* ---- queries ------------------------------------------------------------- */

@@ -33,6 +33,7 @@ def load(path=ORACLE_SO, build_if_missing=True):
         "ora_check": ([P, C.POINTER(C.c_int32)], C.c_int),
         "ora_run": ([P, C.c_int64, C.POINTER(C.c_int32)], C.c_int),
         "ora_finalize": ([P], C.c_int),
+        "ora_refine": ([P, C.c_int64, C.c_int32], C.c_int),
         "ora_num_walkers": ([P], C.c_int32),
         "ora_walker_layout": ([P, C.c_int32] + [C.POINTER(C.c_int32)] * 4, C.c_int),
         "ora_window_geometry": ([P, C.c_int32, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_double),
@@ -137,6 +138,10 @@ class Oracle:
 
     def finalize(self):
         self._chk(self.lib.ora_finalize(self.ctx))
+
+    def refine(self, sweeps, sweeps_per_exchange=None):
+        spe = self.params["sweeps_per_exchange"] if sweeps_per_exchange is None else sweeps_per_exchange
+        self._chk(self.lib.ora_refine(self.ctx, int(sweeps), int(spe)))
 
     def layout(self, g):
         s, w, r, n = C.c_int32(), C.c_int32(), C.c_int32(), C.c_int32()

@@ -904,6 +904,46 @@ int ora_finalize(ora_ctx* c) {
     return REWL_OK;
 }
 
+/* The engine's multicanonical refinement (rewl_refine), restated: the final
+ * replica average, ln f = 0 everywhere, histograms cleared, `sweeps` sweeps
+ * with an exchange after every sweeps_per_exchange, then ln g(b) += ln H(b)
+ * per window (H summed over the replicas; unvisited bins unchanged). */
+int ora_refine(ora_ctx* c, int64_t sweeps, int32_t sweeps_per_exchange) {
+    ora_finalize(c);
+    for (auto& S : c->sys)
+        for (auto& win : S.windows) win.lnf = 0.0;
+    for (auto& w : c->w) {
+        std::fill(w.visits.begin(), w.visits.end(), 0ull);
+        std::fill(w.cnt.begin(), w.cnt.end(), 0ull);
+    }
+    int64_t done = 0, since = 0;
+    while (done < sweeps) {
+        const int64_t k = std::min<int64_t>({1000, (int64_t)sweeps_per_exchange - since, sweeps - done});
+        ora_sweeps(c, (int32_t)k);
+        done += k;
+        since += k;
+        if (since == sweeps_per_exchange) {
+            if (c->p.exchange) ora_exchange(c);
+            since = 0;
+        }
+    }
+    for (auto& S : c->sys)
+        for (int k = 0; k < S.W; ++k) {
+            const Window& win = S.windows[k];
+            for (size_t b = 0; b < win.nbins; ++b) {
+                uint64_t H = 0;
+                for (int r = 0; r < S.R; ++r) {
+                    const Walker& w = c->w[S.first + k * S.R + r];
+                    H += w.visits[b] + w.cnt[b];
+                }
+                if (H == 0) continue;
+                const double v = c->w[S.first + k * S.R].lng[b] + std::log((double)H);
+                for (int r = 0; r < S.R; ++r) c->w[S.first + k * S.R + r].lng[b] = v;
+            }
+        }
+    return REWL_OK;
+}
+
 int32_t ora_num_walkers(const ora_ctx* c) { return (int32_t)c->w.size(); }
 
 int ora_walker_layout(const ora_ctx* c, int32_t walker, int32_t* sys, int32_t* window, int32_t* replica, int32_t* nbins) {

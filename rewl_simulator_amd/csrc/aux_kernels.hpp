@@ -458,6 +458,45 @@ __global__ void __launch_bounds__(kLanes) k_finalize(KArgs a, int average_mode) 
     }
 }
 
+// ---- multicanonical refinement (no counterpart in the reference) -------------
+// Every (bin, walker) histogram entry of the context: visits and counts
+// cleared (a fill kernel, not a memset, so that the step can be captured), or
+// counts rolled into visits.
+__global__ void k_hist_clear(KArgs a, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        a.visits[i] = 0ull;
+        a.cnt[i] = 0u;
+    }
+}
+__global__ void k_hist_roll(KArgs a, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        a.visits[i] += a.cnt[i];
+        a.cnt[i] = 0u;
+    }
+}
+
+// After sampling with fixed weights 1/g (ln f = 0) the histogram of a window
+// is H(b) proportional to g(b) / g_est(b): ln g_est(b) + ln H(b) estimates
+// ln g(b) up to the window's constant.  H sums the window's replicas; bins no
+// replica visited keep their ln g.  One wavefront per window, lanes over bins;
+// every replica receives the corrected curve (replica 0's ln g, which the
+// preceding average made common to all replicas).
+__global__ void __launch_bounds__(kLanes) k_refine(KArgs a) {
+    const int wi = blockIdx.x;
+    if (wi >= a.num_windows) return;
+    const DevWindow W = a.win[wi];
+    for (int b = threadIdx.x; b < W.nbins; b += kLanes) {
+        unsigned long long H = 0ull;
+        for (int r = 0; r < W.R; ++r) {
+            const int64_t i = a.w_hist[W.first_walker + r] + (int64_t)b * kLanes;
+            H += a.visits[i] + a.cnt[i];
+        }
+        if (H == 0ull) continue;
+        const double v = a.lng[a.w_hist[W.first_walker] + (int64_t)b * kLanes] + log((double)H);
+        for (int r = 0; r < W.R; ++r) a.lng[a.w_hist[W.first_walker + r] + (int64_t)b * kLanes] = v;
+    }
+}
+
 // Thermodynamics::calculate_thermodynamics, trapezoid rule, energy part
 // (thermodynamics.hpp:77-236): one thread per temperature, bins summed from
 // the top bin down in FP64 as the reference does.

@@ -235,15 +235,18 @@ __device__ __forceinline__ void update_density(uint32_t* dens, int64_t hidx, int
 // 0.5 (|sum_y R_y e^{i q y}|^2 + |sum_x C_x e^{i q x}|^2), from row sums R
 // (popcounts) and column sums C (bit-sliced vertical counters), divided by
 // N^2 as update_observables does.  cs holds cos(q k), then sin(q k).
-template <typename RowFn>
-__device__ double fourier_qmin(int L, RowFn row, const double* cs) {
+// The four Fourier sums of one +-1 field: f = {sum_y R_y cos(q y),
+// sum_y R_y sin(q y), sum_x C_x cos(q x), sum_x C_x sin(q x)}, rows summed in
+// order (the reference's order, fourier_correlator.hpp:65-118).
+template <typename RowFn, typename CsFn>
+__device__ void fourier_parts(int L, RowFn row, CsFn cs, double f[4]) {
     uint64_t c[7] = {0ull, 0ull, 0ull, 0ull, 0ull, 0ull, 0ull};   // count of set bits per column, bit-sliced
     double rr = 0., ri = 0.;
     for (int y = 0; y < L; ++y) {
         const uint64_t w = row(y);
         const double Ry = (double)(L - 2 * (int)__popcll(w));   // __popcll is unsigned
-        rr += Ry * cs[y];
-        ri += Ry * cs[L + y];
+        rr += Ry * cs(y);
+        ri += Ry * cs(L + y);
         uint64_t carry = w;
 #pragma unroll
         for (int k = 0; k < 7; ++k) {
@@ -258,12 +261,24 @@ __device__ double fourier_qmin(int L, RowFn row, const double* cs) {
 #pragma unroll
         for (int k = 0; k < 7; ++k) n |= (int)((c[k] >> x) & 1ull) << k;
         const double Cx = (double)(L - 2 * n);
-        cr += Cx * cs[x];
-        ci += Cx * cs[L + x];
+        cr += Cx * cs(x);
+        ci += Cx * cs(L + x);
     }
+    f[0] = rr; f[1] = ri; f[2] = cr; f[3] = ci;
+}
+
+// G(qmin) from the four Fourier sums, divided by N^2
+__device__ __forceinline__ double fourier_g(int L, const double f[4]) {
     const double N = (double)L * (double)L;
     const double divide_N = 1. / N;
-    return divide_N * divide_N * (0.5 * ((rr * rr + ri * ri) + (cr * cr + ci * ci)));
+    return divide_N * divide_N * (0.5 * ((f[0] * f[0] + f[1] * f[1]) + (f[2] * f[2] + f[3] * f[3])));
+}
+
+template <typename RowFn>
+__device__ double fourier_qmin(int L, RowFn row, const double* cs) {
+    double f[4];
+    fourier_parts(L, row, [&](int k) { return cs[k]; }, f);
+    return fourier_g(L, f);
 }
 
 // update_qmin_correlator's samples (ising2d_hamiltonian.hpp:247-252,

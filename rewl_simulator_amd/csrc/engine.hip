@@ -334,7 +334,7 @@ int launch_sweep_tuned(rewl_ctx* c, int li, int nsweeps) {
     const size_t lanes = (size_t)c->tuned_maxlanes[MODEL][li];
     const int guard = c->tuned_guard[MODEL][li];
     const size_t lds = ((size_t)c->tuned_maxbins[MODEL][li] + 2 * guard) * lanes * kTunedSlot +
-                       (size_t)tuned_ring_stride(L) * lanes;
+                       (size_t)tuned_ring_stride(L) * lanes + (c->p.observables == 2 ? 16 * (size_t)L : 0);
     const bool simple = c->tuned_simple[MODEL][li];
     const int np = c->tuned_nprod;
     // sweep_kernel 2: no speculation, 3 (and auto): next slot for both outcomes,
@@ -343,8 +343,10 @@ int launch_sweep_tuned(rewl_ctx* c, int li, int nsweeps) {
     void* kern;
     if (sk == 2) kern = simple ? (void*)k_sweep_tuned<MODEL, L, true, 0> : (void*)k_sweep_tuned<MODEL, L, false, 0>;
     else if (sk == 4) kern = simple ? (void*)k_sweep_tuned<MODEL, L, true, 2> : (void*)k_sweep_tuned<MODEL, L, false, 2>;
+    else if (c->p.observables == 2)
+        kern = simple ? (void*)k_sweep_tuned<MODEL, L, true, 1, 2> : (void*)k_sweep_tuned<MODEL, L, false, 1, 2>;
     else if (c->p.observables)
-        kern = simple ? (void*)k_sweep_tuned<MODEL, L, true, 1, true> : (void*)k_sweep_tuned<MODEL, L, false, 1, true>;
+        kern = simple ? (void*)k_sweep_tuned<MODEL, L, true, 1, 1> : (void*)k_sweep_tuned<MODEL, L, false, 1, 1>;
     else kern = simple ? (void*)k_sweep_tuned<MODEL, L, true, 1> : (void*)k_sweep_tuned<MODEL, L, false, 1>;
     HIP_TRY(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     KArgs args = c->args;
@@ -559,8 +561,8 @@ int rewl_create(const rewl_system_desc* systems, int32_t num_systems, const rewl
         c->nprod = P.producers < 0 ? 0 : (P.producers == 0 ? 1 : std::min<int>(P.producers, 3));
         c->tuned_nprod = P.producers == 0 ? 7 : std::min<int>(std::max<int>(P.producers, 1), 7);
         const int li = tuned_index(S.d.L);
-        // observables = 1 (without densities) also run on the row-unrolled kernel, in its default form
-        const bool tuned_obs = P.observables == 1 && !P.densities && P.sweep_kernel != 2 && P.sweep_kernel != 4;
+        // observables (without densities) also run on the row-unrolled kernel, in its default form
+        const bool tuned_obs = P.observables >= 1 && !P.densities && P.sweep_kernel != 2 && P.sweep_kernel != 4;
         const bool tuned = S.intpath && P.rng_mode == REWL_RNG_PHILOX && (!obs || tuned_obs) && P.producers >= 0 &&
                            li >= 0 && P.sweep_kernel != 1;
         if (P.sweep_kernel >= 2 && !tuned)
@@ -574,8 +576,10 @@ int rewl_create(const rewl_system_desc* systems, int32_t num_systems, const rewl
             const long long maxde = 4LL * std::abs(S.twoJ) + (S.d.model == REWL_ASHKIN_TELLER2D ? 4LL * std::abs(S.twoK) : 0LL);
             const long long bs = (long long)c->win[S.first_window].bs;
             guard = (int)((maxde + bs - 1) / bs) + 1;
-            wpg = std::min(kLanes, (int)((size_t)c->lds_max /
-                                         ((size_t)(maxbins + 2 * guard) * kTunedSlot + (size_t)tuned_ring_stride(S.d.L))));
+            // observables = 2: the cos/sin table of the correlator after the rings
+            const size_t room = (size_t)c->lds_max - (P.observables == 2 ? 16 * (size_t)S.d.L : 0);
+            wpg = std::min(kLanes, (int)(room / ((size_t)(maxbins + 2 * guard) * kTunedSlot +
+                                                 (size_t)tuned_ring_stride(S.d.L))));
         }
         else if (fast)
             wpg = std::min(kLanes, (int)((size_t)c->lds_max /
@@ -735,7 +739,8 @@ int rewl_create(const rewl_system_desc* systems, int32_t num_systems, const rewl
         for (int i = 0; i < 5; ++i) {
             if (c->tuned_groups[m][i].empty()) continue;
             const size_t need = ((size_t)c->tuned_maxbins[m][i] + 2 * c->tuned_guard[m][i]) * c->tuned_maxlanes[m][i] *
-                                    kTunedSlot + (size_t)tuned_ring_stride(kTunedL[i]) * c->tuned_maxlanes[m][i];
+                                    kTunedSlot + (size_t)tuned_ring_stride(kTunedL[i]) * c->tuned_maxlanes[m][i] +
+                                (P.observables == 2 ? 16 * (size_t)kTunedL[i] : 0);
             if (need > (size_t)c->lds_max) return bail(fail(REWL_ERR_INVALID, "group too wide for LDS"));
             if ((rc = upload(c->d_tuned_groups[m][i], c->tuned_groups[m][i], c->stream))) return bail(rc);
         }
@@ -965,6 +970,39 @@ int rewl_finalize(rewl_ctx* c) {
     if (!c) return fail(REWL_ERR_INVALID, "null context");
     hipLaunchKernelGGL(k_finalize, dim3((unsigned)c->win.size()), dim3(kLanes), 0, c->stream, c->args,
                        (int)c->p.average_mode);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return REWL_OK;
+}
+
+int rewl_refine(rewl_ctx* c, int64_t sweeps, int32_t sweeps_per_exchange) {
+    if (!c || sweeps < 1 || sweeps_per_exchange < 1) return fail(REWL_ERR_INVALID, "bad arguments");
+    if (!c->in_windows)
+        return fail(REWL_ERR_RANGE, "walkers are not inside their windows: call rewl_adjust_to_range first");
+    int rc;
+    // the replicas' common curve, then fixed weights: ln f = 0 in every window
+    if ((rc = rewl_finalize(c))) return rc;
+    if ((rc = upload(c->d_win_lnf, std::vector<double>(c->win.size(), 0.0), c->stream))) return rc;
+    const int64_t n = (int64_t)c->hist_total;
+    const unsigned grid = (unsigned)std::min<int64_t>(4096, (n + 255) / 256);
+    hipLaunchKernelGGL(k_hist_clear, dim3(grid), dim3(256), 0, c->stream, c->args, n);
+    HIP_TRY(hipGetLastError());
+    // launches of at most 1000 sweeps keep a bin's u32 count far below 2^32;
+    // an exchange after every sweeps_per_exchange sweeps
+    int64_t done = 0, since_exchange = 0;
+    while (done < sweeps) {
+        const int k = (int)std::min<int64_t>({1000, (int64_t)sweeps_per_exchange - since_exchange, sweeps - done});
+        if ((rc = rewl_sweeps(c, k))) return rc;
+        done += k;
+        since_exchange += k;
+        if (since_exchange == sweeps_per_exchange) {
+            if (c->p.exchange && (rc = rewl_exchange(c))) return rc;
+            since_exchange = 0;
+        }
+        hipLaunchKernelGGL(k_hist_roll, dim3(grid), dim3(256), 0, c->stream, c->args, n);
+        HIP_TRY(hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_refine, dim3((unsigned)c->win.size()), dim3(kLanes), 0, c->stream, c->args);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(c->stream));
     return REWL_OK;

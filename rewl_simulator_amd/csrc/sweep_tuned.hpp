@@ -25,6 +25,13 @@
 //    comparison ln g[new] - ln g[cur] < B;
 //  * ln g (FP64) and the u32 count of a (bin, walker) share one 16-byte LDS
 //    slot, so the count increment is a ds_add at offset 8 of the ln g slot.
+//
+// With observables = 2 (the reference's default CORRELATION_LENGTHS build) the
+// slot's last word holds the bin's update count before the launch minus its
+// count, so that one 16-byte read of a candidate slot gives ln g and the
+// bin's all-time update count; the Fourier sums of the correlator are kept in
+// registers, moved by each row's flips at the end of the row, and a sample
+// (every 1000th update of a bin) combines them with the row's flips so far.
 #pragma once
 #include "accept_bound.hpp"
 #include "device.hpp"
@@ -81,6 +88,12 @@ __device__ __forceinline__ void lds_store_f64x2(uint32_t addr, f64x2 v) { *(lds_
 __device__ __forceinline__ void lds_add_f64(uint32_t addr, double v) {
     __hip_atomic_fetch_add((lds_double*)(uintptr_t)addr, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
+typedef uint32_t u32x4v __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(3))) u32x4v lds_u32x4_t;
+__device__ __forceinline__ u32x4v lds_u32x4(uint32_t addr) { return *(lds_u32x4_t*)(uintptr_t)addr; }
+__device__ __forceinline__ double f64_of(uint32_t lo, uint32_t hi) {
+    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
 __device__ __forceinline__ uint32_t lds_u32_at(uint32_t addr) { return *(lds_u32*)(uintptr_t)addr; }
 __device__ __forceinline__ void lds_store_u32(uint32_t addr, uint32_t v) { *(lds_u32*)(uintptr_t)addr = v; }
 __device__ __forceinline__ void lds_add_u32(uint32_t addr, uint32_t v) {
@@ -128,13 +141,33 @@ __device__ __forceinline__ void tuned_attempt(int on, const TunedWindow& tw, int
     a = acc ? 1 : 0;
 }
 
+// The Fourier sums f of one bit plane after the sites in `fw` flipped in row
+// y (old row `old`, bit 1 = -1): a site going from -1 to +1 raises its row
+// and column sums by 2, one going the other way lowers them by 2.  cs is the
+// LDS address of cos(q k), k < L, followed by sin(q k).
+template <int L>
+__device__ __forceinline__ void fourier_flip(double f[4], uint64_t old, uint64_t fw, int y, uint32_t cs) {
+    const uint64_t up = old & fw, dn = ~old & fw;
+    const double dR = (double)(2 * ((int)__popcll(up) - (int)__popcll(dn)));
+    f[0] = fma(dR, lds_f64(cs + 8u * (uint32_t)y), f[0]);
+    f[1] = fma(dR, lds_f64(cs + 8u * (uint32_t)(L + y)), f[1]);
+    uint64_t m = fw;
+    while (m) {
+        const int x = __builtin_ctzll(m);
+        m &= m - 1ull;
+        const double d = ((old >> x) & 1ull) ? 2.0 : -2.0;
+        f[2] = fma(d, lds_f64(cs + 8u * (uint32_t)x), f[2]);
+        f[3] = fma(d, lds_f64(cs + 8u * (uint32_t)(L + x)), f[3]);
+    }
+}
+
 // OBS (with SPEC 1 only): the observables of update_observables at every
 // attempt, from magnetisations kept in registers; the moments of consecutive
 // attempts in one bin are summed in registers and added to the bin's sums
-// when the walker leaves it.
-template <int MODEL, int L, bool SIMPLE, int SPEC, bool OBS = false>
+// when the walker leaves it.  OBS = 2 adds the correlators G(qmin).
+template <int MODEL, int L, bool SIMPLE, int SPEC, int OBS = 0>
 __device__ __forceinline__ void sweep_walker_tuned(const KArgs& a, const DevGroup& G, const DevSystem& S, int lane,
-                                                   int nsweeps, int guard, uint32_t ring_lane) {
+                                                   int nsweeps, int guard, uint32_t ring_lane, uint32_t cs_tab) {
     static_assert(!OBS || SPEC == 1, "observables are tracked by the SPEC 1 form");
     constexpr int NW = (L + 31) / 32;    // 32-bit words per row
     constexpr int NW2 = (L + 15) / 16;   // words of 2-bit fields per row
@@ -158,8 +191,11 @@ __device__ __forceinline__ void sweep_walker_tuned(const KArgs& a, const DevGrou
         for (int b = -guard; b < gbins + guard; ++b) {
             const uint32_t off = zero_off + (uint32_t)(b * stride16);
             const bool inside = b >= 0 && b < nbins;   // a narrower window's spare bins are guards too
+            const uint32_t c = inside ? a.cnt[hb + (int64_t)b * kLanes] : 0u;
             lds_store_f64(off, inside ? a.lng[hb + (int64_t)b * kLanes] : __builtin_inf());
-            lds_store_u32(off + 8, inside ? a.cnt[hb + (int64_t)b * kLanes] : 0u);
+            lds_store_u32(off + 8, c);
+            // all-time updates of the bin = this word + its current count
+            if (OBS == 2) lds_store_u32(off + 12, inside ? a.obs_tot[hb + (int64_t)b * kLanes] - c : 0u);
         }
     }
     __syncthreads();   // histograms loaded, first row of thresholds ready
@@ -184,6 +220,7 @@ __device__ __forceinline__ void sweep_walker_tuned(const KArgs& a, const DevGrou
     int buf = 0;
 
     constexpr int NV = MODEL == 0 ? 3 : 12;
+    constexpr int GK = OBS == 2 ? 4 : 3;   // slot stride of the (|x|, x^2, x^4[, G]) groups
     int mag[3] = {0, 0, 0};
     double accv[NV];
 #pragma unroll
@@ -194,8 +231,17 @@ __device__ __forceinline__ void sweep_walker_tuned(const KArgs& a, const DevGrou
         const int b = (int)((acck - zero_off) / (uint32_t)stride16);
         double* p = a.obs + (hb + (int64_t)b * kLanes) * a.nobs;
 #pragma unroll
-        for (int i = 0; i < NV; ++i) obs_add(p + i, accv[i]);   // observables = 1: slot i
+        for (int i = 0; i < NV; ++i) obs_add(p + (i / 3) * GK + i % 3, accv[i]);
     };
+    // correlator: Fourier sums of the sigma plane and (AT) of the Baxter plane
+    double fs[4] = {0., 0., 0., 0.}, fb[4] = {0., 0., 0., 0.};
+    if (OBS == 2 && active) {
+        const uint64_t* lt = a.lat + G.lat_base + lane;
+        auto csf = [&](int k) { return lds_f64(cs_tab + 8u * (uint32_t)k); };
+        fourier_parts(L, [&](int r) { return lt[(int64_t)r * kLanes]; }, csf, fs);
+        if (MODEL == 1)
+            fourier_parts(L, [&](int r) { return lt[(int64_t)r * kLanes] ^ lt[(int64_t)(L + r) * kLanes]; }, csf, fb);
+    }
     auto observe = [&]() {
         if (cbo != acck) {
             flush_obs();
@@ -351,8 +397,18 @@ __device__ __forceinline__ void sweep_walker_tuned(const KArgs& a, const DevGrou
                                 if (x + 2 < L) bnd_next = lds_f64x2(rrow + 8u * (uint32_t)(x + 2));
                             }
                             const double Bx = (x & 1) ? bnd.y : bnd.x;
-                            const double ln = lds_f64(nbo);
-                            const double lc = lds_f64(cbo);
+                            double ln, lc;
+                            u32x4v vn, vc;
+                            if constexpr (OBS == 2) {
+                                // ln g, count and update-count word of both candidate bins
+                                vn = lds_u32x4(nbo);
+                                vc = lds_u32x4(cbo);
+                                ln = f64_of(vn.x, vn.y);
+                                lc = f64_of(vc.x, vc.y);
+                            } else {
+                                ln = lds_f64(nbo);
+                                lc = lds_f64(cbo);
+                            }
                             int on0 = 0, on1 = 0;
                             uint32_t nbo0 = 0, nbo1 = 0;
                             if (x + 1 < L) {
@@ -388,6 +444,36 @@ __device__ __forceinline__ void sweep_walker_tuned(const KArgs& a, const DevGrou
                                 }
                                 observe();
                             }
+                            if constexpr (OBS == 2) {
+                                // the bin's updates before this one (ising2d_hamiltonian.hpp:247-252:
+                                // a sample at counts 0, 1000, 2000, ...)
+                                const uint32_t t = ac ? vn.z + vn.w : vc.z + vc.w;
+                                if (t % 1000u == 0u) {
+                                    // this row's flips so far (sites 0..x), then the sums of the
+                                    // current configuration
+                                    uint64_t pf = 0;
+#pragma unroll
+                                    for (int hh = 0; hh <= (x >> 5); ++hh) {
+                                        const int nh = hh < (x >> 5) ? 32 : (x & 31) + 1;
+                                        pf |= (uint64_t)(__builtin_bitreverse32(frev[hh]) >> (32 - nh)) << (32 * hh);
+                                    }
+                                    double gs[4] = {fs[0], fs[1], fs[2], fs[3]};
+                                    double gb[4] = {fb[0], fb[1], fb[2], fb[3]};
+                                    if (MODEL == 0 || fl == 0) fourier_flip<L>(gs, P, pf, y, cs_tab);
+                                    if (MODEL == 1) fourier_flip<L>(gb, P ^ Q, pf, y, cs_tab);
+                                    const int b = (int)((cbo - zero_off) / (uint32_t)stride16);
+                                    double* p = a.obs + (hb + (int64_t)b * kLanes) * a.nobs;
+                                    const double g = fourier_g(L, gs);
+                                    obs_add(p + 3, g);
+                                    if (MODEL == 1) {
+                                        // the tau correlator repeats the sigma one and phi is
+                                        // their sum (ashkin_teller2d_hamiltonian.hpp:296-316)
+                                        obs_add(p + 7, g);
+                                        obs_add(p + 11, g + g);
+                                        obs_add(p + 15, fourier_g(L, gb));
+                                    }
+                                }
+                            }
                         }
                     } else {
 #pragma unroll
@@ -412,6 +498,10 @@ __device__ __forceinline__ void sweep_walker_tuned(const KArgs& a, const DevGrou
                         fw |= (uint64_t)(__builtin_bitreverse32(frev[h]) >> (32 - nh)) << (32 * h);
                     }
                     const uint64_t Pn = P ^ fw;
+                    if constexpr (OBS == 2) {
+                        if (MODEL == 0 || fl == 0) fourier_flip<L>(fs, P, fw, y, cs_tab);
+                        if (MODEL == 1) fourier_flip<L>(fb, P ^ Q, fw, y, cs_tab);
+                    }
                     if (MODEL == 0) {
                         cu[0] = Pn;
                     } else {
@@ -439,6 +529,7 @@ __device__ __forceinline__ void sweep_walker_tuned(const KArgs& a, const DevGrou
             const uint32_t off = zero_off + (uint32_t)(b * stride16);
             a.lng[hb + (int64_t)b * kLanes] = lds_f64(off);
             a.cnt[hb + (int64_t)b * kLanes] = lds_u32_at(off + 8);
+            if (OBS == 2) a.obs_tot[hb + (int64_t)b * kLanes] = lds_u32_at(off + 12) + lds_u32_at(off + 8);
         }
         a.w_E[g] = (double)(o + tw.wmin);
         a.w_flavor[g] = fl;
@@ -499,7 +590,7 @@ __device__ __forceinline__ void produce_bounds_tuned(const DevGroup& G, const De
 // slots of 16 bytes, then one ring row of tuned_ring_stride(L) bytes per
 // walker (two buffers of L FP64 bounds, padded by 16 bytes so that the
 // walkers' 16-byte reads fall on different banks).
-template <int MODEL, int L, bool SIMPLE, int SPEC, bool OBS = false>
+template <int MODEL, int L, bool SIMPLE, int SPEC, int OBS = 0>
 __global__ void __launch_bounds__(512) k_sweep_tuned(KArgs a, const int32_t* groups, int ngroups, int nsweeps,
                                                      uint64_t sweep_base, int guard) {
     const int tid = threadIdx.x;
@@ -513,8 +604,18 @@ __global__ void __launch_bounds__(512) k_sweep_tuned(KArgs a, const int32_t* gro
         const uint32_t ring = lds_base() + (uint32_t)((G.nbins + 2 * guard) * G.nwalkers * kTunedSlot);
         const int lane = tid & (kLanes - 1);
         const uint32_t ring_lane = ring + (uint32_t)(lane * tuned_ring_stride(L));
+        // observables = 2: cos and sin of qmin k after the ring (Transform_Kernel,
+        // fourier_correlator.hpp:14-42), visible to the walker after its first barrier
+        const uint32_t cs_tab = ring + (uint32_t)(G.nwalkers * tuned_ring_stride(L));
+        if (OBS == 2) {
+            const double qmin = 2. * acos(-1.) / (double)L;
+            for (int k = tid; k < L; k += (int)blockDim.x) {
+                lds_store_f64(cs_tab + 8u * (uint32_t)k, cos(qmin * (double)k));
+                lds_store_f64(cs_tab + 8u * (uint32_t)(L + k), sin(qmin * (double)k));
+            }
+        }
         if (tid < kLanes)
-            sweep_walker_tuned<MODEL, L, SIMPLE, SPEC, OBS>(a, G, S, tid, nsweeps, guard, ring_lane);
+            sweep_walker_tuned<MODEL, L, SIMPLE, SPEC, OBS>(a, G, S, tid, nsweeps, guard, ring_lane, cs_tab);
         else
             produce_bounds_tuned<MODEL, L>(G, S, tid / kLanes - 1, (int)(blockDim.x / kLanes) - 1, lane, nsweeps,
                                            sweep_base, ring_lane, static_cast<const BoundEntry*>(a.bound_tab));

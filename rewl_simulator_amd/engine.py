@@ -152,6 +152,12 @@ class Engine:
     def finalize(self):
         abi.check(self.lib.rewl_finalize(self.ctx))
 
+    def refine(self, sweeps, sweeps_per_exchange=None):
+        """Multicanonical refinement of the finished run (rewl_refine): fixed
+        weights 1/g for `sweeps` sweeps, then ln g += ln H per window."""
+        spe = self.params["sweeps_per_exchange"] if sweeps_per_exchange is None else sweeps_per_exchange
+        abi.check(self.lib.rewl_refine(self.ctx, int(sweeps), int(spe)))
+
     def synchronize(self):
         abi.check(self.lib.rewl_synchronize(self.ctx))
 

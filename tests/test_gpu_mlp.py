@@ -43,9 +43,12 @@ def test_forward_matches_oracle(gpu, n):
         y = m.forward_raw(x)
     yo, _ = M.forward(th, x, emulate=True)
     err = np.abs(y - yo)
+    scale = max(1.0, np.abs(yo).max())
+    print(f"forward n={n}: max |GPU - oracle| {err.max():.3e}, median {np.median(err):.3e} (scale {scale:.3f})",
+          flush=True)
     # FP32 accumulation order and rare one-ulp bf16 flips of activations
-    assert err.max() <= 2e-3 * max(1.0, np.abs(yo).max()), err.max()
-    assert np.median(err) <= 1e-4 * max(1.0, np.abs(yo).max())
+    assert err.max() <= 1e-3 * scale, err.max()
+    assert np.median(err) <= 1e-4 * scale
 
 
 @pytest.mark.parametrize("weighted", [False, True])
@@ -60,10 +63,14 @@ def test_gradient_matches_oracle(gpu, n, weighted):
     assert loss == pytest.approx(lo, rel=1e-4)
     blocks = [(M.O_W1, M.O_B1), (M.O_B1, M.O_W2), (M.O_W2, M.O_B2), (M.O_B2, M.O_W3), (M.O_W3, M.O_B3),
               (M.O_B3, M.O_W4), (M.O_W4, M.O_B4), (M.O_B4, M.NP)]
+    rels = []
     for lo_, hi_ in blocks:
         a, b = g[lo_:hi_].astype(np.float64), go[lo_:hi_]
-        rel = np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-12)
-        assert rel < 1e-2, (lo_, hi_, rel)
+        rels.append(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-12))
+    print(f"gradient n={n} weighted={weighted}: relative error per block " + " ".join(f"{r:.2e}" for r in rels),
+          flush=True)
+    # every parameter block within 1e-3 relative of the bf16-emulating oracle
+    assert max(rels) < 1e-3, rels
 
 
 def test_gradient_is_deterministic(gpu):

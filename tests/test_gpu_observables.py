@@ -264,29 +264,43 @@ def test_density_plots_from_a_run(gpu, tmp_path):
     assert np.loadtxt(files[0], comments="#").shape == (32, 32)
 
 
+@pytest.mark.parametrize("obs,L", [(1, 16), (2, 16), (2, 48)])
 @pytest.mark.parametrize("model", [abi.ISING2D, abi.ASHKIN_TELLER2D])
-def test_observables_row_kernel_match_oracle(gpu, model):
-    """observables = 1 in Philox mode at L = 16 runs on the row-unrolled
-    kernel, which tracks the magnetisations in registers: chains bit-exact,
-    per-bin sums / counts against the oracle's running means (rtol 1e-9)."""
-    print(f"case: row-kernel observables model={model}", flush=True)
-    L, N = 16, 256
+def test_observables_row_kernel_match_oracle(gpu, model, obs, L):
+    """observables = 1 and 2 in Philox mode at L = 16 and 48 run on the
+    row-unrolled kernel, which tracks the magnetisations in registers and,
+    with 2, the correlator's Fourier sums (moved by each row's flips, sampled
+    at every 1000th update of a bin): chains bit-exact, per-bin sums against
+    the oracle's running means (rtol 1e-9)."""
+    print(f"case: row-kernel observables model={model} observables={obs} L={L}", flush=True)
+    N = L * L
     s = (configs.ising_system(L, -2.0 * N, 2.0 * N + 8.0, 8.0, 3, 3) if model == abi.ISING2D
          else configs.at_system(L, -5.0 * N, 3.0 * N + 12.0, 12.0, 3, 3))
-    p = configs.params(rng_mode=abi.RNG_PHILOX, lnf_initial=0.05, observables=1, sweeps_per_exchange=20, adjust_mode=1)
+    p = configs.params(rng_mode=abi.RNG_PHILOX, lnf_initial=0.05, observables=obs, sweeps_per_exchange=20,
+                       adjust_mode=1)
     eng, ora = Engine([s], p), Oracle([s], p)
     eng.adjust_to_range(); ora.adjust_to_range()
-    for _ in range(3):
-        eng.sweeps(20); ora.sweeps(20)
+    # enough updates that bins pass 1000 (several correlator samples) with obs = 2
+    per = 20 if obs == 1 else (60 if L == 16 else 40)
+    for _ in range(3 if obs == 1 else 10):
+        eng.sweeps(per); ora.sweeps(per)
         eng.exchange(); ora.exchange()
     eng.check(); ora.check()
+    eng.sweeps(20); ora.sweeps(20)
     assert_same(eng, ora, "row kernel, observables on", cursors=False)
+    corr = eng.correlator_slots(model) if obs == 2 else []
+    sampled = 0
     for g in range(eng.num_walkers):
         sums = eng.observables(g)
         n = eng.counts(g).astype(np.float64) + eng.visits(g)
         means, counts = ora.observables(g)
         np.testing.assert_array_equal(n, counts, err_msg=f"counts per bin, walker {g}")
         seen = n > 0
-        np.testing.assert_allclose(sums[seen] / n[seen, None], means[seen], rtol=1e-9, atol=1e-14,
-                                   err_msg=f"walker {g}")
+        k = np.ceil(n[seen] / 1000.0)
+        for c in range(sums.shape[1]):
+            div = k if c in corr else n[seen]
+            np.testing.assert_allclose(sums[seen, c] / div, means[seen, c], rtol=1e-9, atol=1e-13,
+                                       err_msg=f"walker {g}, column {c}")
+        sampled += int((n > 1000).sum())
+    assert obs == 1 or sampled > 0
     eng.close(); ora.close()

@@ -31,5 +31,8 @@ def test_warm_start_cuts_sweeps_on_heldout_disorder(gpu):
     # warm-started run ends where plain WL ends
     assert res["prediction_error_max"] < 5.0
     assert res["warm_vs_plain_max"] < 1.0
-    # fewer sweeps to the final ln f
-    assert warm["sweeps"] < plain["sweeps"], (warm["sweeps"], plain["sweeps"])
+    # fewer sweeps to the final ln f: measured 1.95x on this deterministic
+    # workload (the Philox streams and seeds fix every chain); at least 1.5x
+    factor = plain["sweeps"] / warm["sweeps"]
+    print(f"warm start: {factor:.2f}x fewer sweeps to ln f < 1e-4", flush=True)
+    assert factor >= 1.5, (warm["sweeps"], plain["sweeps"])
