@@ -187,7 +187,11 @@ typedef struct rewl_params {
                                     every update (the AT_DENSITIES build,
                                     ashkin_teller2d_hamiltonian.hpp:291-293); needs
                                     observables >= 1 and Ashkin-Teller systems only  */
-    int32_t reserved;
+    int32_t retire;              /* 1: a system whose windows are all done stops
+                                    (no sweeps, exchanges or checks; its walkers
+                                    keep their state), as a reference run of that
+                                    system alone would end there; 0: every system
+                                    runs until all are done                         */
 } rewl_params;
 
 /* Per-window bookkeeping, one record per (system, window). */

@@ -188,6 +188,7 @@ struct Window {
 
 struct System {
     rewl_system_desc d{};
+    bool retired = false;                  /* rewl_params.retire: all windows done   */
     int L = 0, N = 0, ndof = 0;
     std::vector<double> field;             /* per-site field, empty = uniform h   */
     bool has_field = false;                /* AT: whether the sigma*tau term exists */
@@ -835,16 +836,20 @@ int ora_adjust_to_range(ora_ctx* c) {
 
 int ora_sweeps(ora_ctx* c, int32_t n) {
     for (int k = 0; k < n; ++k) {
-        for (auto& w : c->w) sweep_walker(*c, c->sys[w.sys], w, c->sweep_index);
+        for (auto& w : c->w)
+            if (!c->sys[w.sys].retired) sweep_walker(*c, c->sys[w.sys], w, c->sweep_index);
         ++c->sweep_index;
     }
-    for (auto& S : c->sys) for (auto& win : S.windows) win.sweeps += n;
+    for (auto& S : c->sys)
+        if (!S.retired)
+            for (auto& win : S.windows) win.sweeps += n;
     return REWL_OK;
 }
 
 int ora_exchange(ora_ctx* c) {
     const bool even = c->direction_even != 0;
-    for (auto& S : c->sys) exchange_system(*c, S, even);
+    for (auto& S : c->sys)
+        if (!S.retired) exchange_system(*c, S, even);
     c->direction_even = !c->direction_even;
     ++c->exchange_calls;
     return REWL_OK;
@@ -853,6 +858,7 @@ int ora_exchange(ora_ctx* c) {
 int ora_check(ora_ctx* c, int32_t* all_done) {
     bool all = true;
     for (auto& S : c->sys) {
+        if (S.retired) continue;   /* its windows are all done */
         for (int k = 0; k < S.W; ++k) {
             Window& win = S.windows[k];
             std::vector<uint8_t> unseen(win.nbins, 1);
@@ -880,6 +886,12 @@ int ora_check(ora_ctx* c, int32_t* all_done) {
         }
     }
     ++c->checks;
+    if (c->p.retire)
+        for (auto& S : c->sys) {
+            bool done = true;
+            for (const auto& win : S.windows) done = done && win.done;
+            S.retired = S.retired || done;
+        }
     if (all_done) *all_done = all;
     return REWL_OK;
 }
@@ -909,6 +921,7 @@ int ora_finalize(ora_ctx* c) {
  * with an exchange after every sweeps_per_exchange, then ln g(b) += ln H(b)
  * per window (H summed over the replicas; unvisited bins unchanged). */
 int ora_refine(ora_ctx* c, int64_t sweeps, int32_t sweeps_per_exchange) {
+    for (auto& S : c->sys) S.retired = false;
     ora_finalize(c);
     for (auto& S : c->sys)
         for (auto& win : S.windows) win.lnf = 0.0;
@@ -926,6 +939,8 @@ int ora_refine(ora_ctx* c, int64_t sweeps, int32_t sweeps_per_exchange) {
             if (c->p.exchange) ora_exchange(c);
             since = 0;
         }
+        for (auto& w : c->w)   /* counts into visits after every launch, as the engine does */
+            for (size_t b = 0; b < w.cnt.size(); ++b) { w.visits[b] += w.cnt[b]; w.cnt[b] = 0; }
     }
     for (auto& S : c->sys)
         for (int k = 0; k < S.W; ++k) {

@@ -72,7 +72,7 @@ class Params(C.Structure):
         ("sweep_kernel", C.c_int32),
         ("observables", C.c_int32),
         ("densities", C.c_int32),
-        ("reserved", C.c_int32),
+        ("retire", C.c_int32),
     ]
 
 

@@ -31,6 +31,7 @@ REFERENCE_PARAMS = dict(
     sweep_kernel=0,
     observables=0,
     densities=0,
+    retire=0,
 )
 
 DEFAULT_SEED = 20240501
@@ -159,7 +160,8 @@ def config4(realisations=256, sizes=(16, 24, 32, 48), sigma=0.5, bins=512, windo
     h_i ~ N(0, sigma^2) per realisation, sizes cycled over `sizes`, 8 windows x
     4 walkers each, `bins` uniform bins over [E(all-up), E(checkerboard)] with
     the top level included; all realisations in one context (one launch per
-    sweep step).  Flatness is taken over the bins some replica has visited:
+    sweep step); a realisation stops when all its windows are done.  Flatness
+    is taken over the bins some replica has visited:
     the low end of a disordered spectrum has empty bins whose position depends
     on the realisation."""
     systems = []
@@ -170,5 +172,5 @@ def config4(realisations=256, sizes=(16, 24, 32, 48), sigma=0.5, bins=512, windo
         bs = (e1 - e0) / (bins - 1)
         systems.append(at_system(L, e0, e0 + bs * bins, bs, windows, replicas, field=h, seed=base_seed + r))
     p = params(flatness_mode=abi.FLAT_MIN_OVER_MEAN_VISITED, flatness=0.8, schedule=abi.SCHED_HALVE,
-               lnf_final=lnf_final, sweeps_per_check=1000, sweeps_per_exchange=1000, adjust_mode=1)
+               lnf_final=lnf_final, sweeps_per_check=1000, sweeps_per_exchange=1000, adjust_mode=1, retire=1)
     return systems, p

@@ -326,6 +326,10 @@ __global__ void __launch_bounds__(kLanes) k_check(KArgs a, CheckParams cp, int32
     const int wi = blockIdx.x;
     if (wi >= a.num_windows) return;
     const int lane = threadIdx.x;
+    if (!a.win_active[wi]) {   // a retired system: state kept as it was
+        if (lane == 0) flat_out[wi] = 0;
+        return;
+    }
     const DevWindow W = a.win[wi];
     const int R = W.R, nb = W.nbins;
     const uint8_t* ex = a.exclude + W.excl_off;

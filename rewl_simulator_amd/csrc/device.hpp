@@ -61,6 +61,7 @@ struct KArgs {
     int64_t* win_sweeps;
     int32_t* win_oot;
     int32_t* win_done;
+    const int32_t* win_active;   // 0: the window's system has retired (rewl_params.retire)
     const uint8_t* exclude;
     uint64_t* lat;
     double* lng;

@@ -153,7 +153,10 @@ struct rewl_ctx {
     DevBuf<int64_t> d_w_hist, d_w_lat, d_win_iter, d_win_sweeps;
     DevBuf<double> d_w_E, d_win_lnf, d_fields;
     DevBuf<uint64_t> d_w_cursor, d_lat, d_visits, d_ring, d_ring_filled;
-    DevBuf<int32_t> d_win_oot, d_win_done;
+    DevBuf<int32_t> d_win_oot, d_win_done, d_win_active;
+    std::vector<uint8_t> sys_retired;   // rewl_params.retire: systems whose windows are all done
+    // the lists before any retirement (rewl_refine runs every system again)
+    std::vector<int32_t> class_groups_all[4], tuned_groups_all[2][5], pairs_even_all, pairs_odd_all;
     DevBuf<uint8_t> d_exclude;
     DevBuf<double> d_lng;
     DevBuf<uint32_t> d_cnt;
@@ -189,6 +192,7 @@ void destroy_ctx(rewl_ctx* c) {
     for (auto& e : c->ev_sweep) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
     c->ev_sweep.clear();
     for (auto* b : {&c->d_w_window, &c->d_w_local, &c->d_w_flavor, &c->d_err, &c->d_win_oot, &c->d_win_done,
+                    &c->d_win_active,
                     &c->d_pairs_even, &c->d_pairs_odd, &c->d_flat})
         b->release();
     for (auto& b : c->d_class_groups) b.release();
@@ -206,6 +210,14 @@ void destroy_ctx(rewl_ctx* c) {
 template <typename T>
 int upload(DevBuf<T>& b, const std::vector<T>& v, hipStream_t s) {
     HIP_TRY(b.alloc(v.size()));
+    if (!v.empty()) HIP_TRY(hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));
+    return REWL_OK;
+}
+
+// New contents of a buffer no larger than its allocation (a list that shrank)
+template <typename T>
+int overwrite(DevBuf<T>& b, const std::vector<T>& v, hipStream_t s) {
+    if (v.size() > b.n) return fail(REWL_ERR_INVALID, "internal: list grew beyond its buffer");
     if (!v.empty()) HIP_TRY(hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));
     return REWL_OK;
 }
@@ -464,7 +476,7 @@ int rewl_create(const rewl_system_desc* systems, int32_t num_systems, const rewl
     const rewl_params& P = *params;
     if (P.sweeps_per_check <= 0 || P.sweeps_per_exchange <= 0 || P.lnf_initial <= 0 ||
         (P.flatness_mode < 0 || P.flatness_mode > 3) || (P.schedule < 0 || P.schedule > 1) ||
-        (P.rng_mode < 0 || P.rng_mode > 1) || (P.adjust_mode < 0 || P.adjust_mode > 1) || (P.average_mode < 0 || P.average_mode > 1) || (P.sweep_kernel < 0 || P.sweep_kernel > 4) || (P.observables < 0 || P.observables > 2) || P.walkers_per_group < 0 || P.walkers_per_group > kLanes ||
+        (P.rng_mode < 0 || P.rng_mode > 1) || (P.adjust_mode < 0 || P.adjust_mode > 1) || (P.average_mode < 0 || P.average_mode > 1) || (P.sweep_kernel < 0 || P.sweep_kernel > 4) || (P.observables < 0 || P.observables > 2) || (P.retire < 0 || P.retire > 1) || P.walkers_per_group < 0 || P.walkers_per_group > kLanes ||
         (P.densities < 0 || P.densities > 1))
         return fail(REWL_ERR_INVALID, "invalid rewl_params");
     if (P.densities && !P.observables)
@@ -758,6 +770,13 @@ int rewl_create(const rewl_system_desc* systems, int32_t num_systems, const rewl
     if ((rc = upload(c->d_win_sweeps, zeros64, c->stream))) return bail(rc);
     if ((rc = upload(c->d_win_oot, zeros32, c->stream))) return bail(rc);
     if ((rc = upload(c->d_win_done, zeros32, c->stream))) return bail(rc);
+    if ((rc = upload(c->d_win_active, std::vector<int32_t>(nwin, 1), c->stream))) return bail(rc);
+    c->sys_retired.assign(c->sys.size(), 0);
+    for (int cls = 0; cls < 4; ++cls) c->class_groups_all[cls] = c->class_groups[cls];
+    for (int m = 0; m < 2; ++m)
+        for (int i = 0; i < 5; ++i) c->tuned_groups_all[m][i] = c->tuned_groups[m][i];
+    c->pairs_even_all = c->pairs_even;
+    c->pairs_odd_all = c->pairs_odd;
     if ((rc = upload(c->d_flat, zeros32, c->stream))) return bail(rc);
     if ((rc = upload(c->d_w_flavor, flav, c->stream))) return bail(rc);
     if ((rc = upload(c->d_w_E, E0, c->stream))) return bail(rc);
@@ -833,7 +852,8 @@ int rewl_create(const rewl_system_desc* systems, int32_t num_systems, const rewl
     a.w_window = c->d_w_window.p; a.w_local = c->d_w_local.p; a.w_hist = c->d_w_hist.p; a.w_lat = c->d_w_lat.p;
     a.w_E = c->d_w_E.p; a.w_flavor = c->d_w_flavor.p; a.w_cursor = c->d_w_cursor.p;
     a.win_lnf = c->d_win_lnf.p; a.win_iter = c->d_win_iter.p; a.win_sweeps = c->d_win_sweeps.p;
-    a.win_oot = c->d_win_oot.p; a.win_done = c->d_win_done.p; a.exclude = c->d_exclude.p;
+    a.win_oot = c->d_win_oot.p; a.win_done = c->d_win_done.p; a.win_active = c->d_win_active.p;
+    a.exclude = c->d_exclude.p;
     a.lat = c->d_lat.p; a.lng = c->d_lng.p; a.cnt = c->d_cnt.p; a.visits = c->d_visits.p;
     a.obs = c->d_obs.p; a.nobs = c->nobs; a.obs_tot = c->d_obs_tot.p; a.obs_corr = P.observables == 2 ? 1 : 0;
     a.dens = c->d_dens.p;
@@ -899,7 +919,11 @@ int rewl_sweeps(rewl_ctx* c, int32_t n) {
     c->pending_sweeps += n;
     c->stats.sweeps += (uint64_t)n;
     uint64_t att = 0;
-    for (auto& S : c->sys) att += (uint64_t)S.ndof * (uint64_t)S.d.num_windows * (uint64_t)S.d.replicas_per_window;
+    for (size_t si = 0; si < c->sys.size(); ++si) {
+        if (c->sys_retired[si]) continue;
+        const HostSystem& S = c->sys[si];
+        att += (uint64_t)S.ndof * (uint64_t)S.d.num_windows * (uint64_t)S.d.replicas_per_window;
+    }
     c->stats.attempts += att * (uint64_t)n;
     if (c->p.rng_mode == REWL_RNG_REPLAY) return check_err_flag(c);
     return REWL_OK;
@@ -920,6 +944,76 @@ int rewl_exchange(rewl_ctx* c) {
     c->direction_even = !c->direction_even;
     c->stats.exchange_calls += 1;
     if (c->p.rng_mode == REWL_RNG_REPLAY) return check_err_flag(c);
+    return REWL_OK;
+}
+
+// rewl_params.retire: systems whose windows are all done leave the group
+// lists of the sweep kernels, the exchange pair lists and the check.
+static int retire_done_systems(rewl_ctx* c) {
+    bool changed = false;
+    for (size_t si = 0; si < c->sys.size(); ++si) {
+        if (c->sys_retired[si]) continue;
+        const HostSystem& S = c->sys[si];
+        bool all = true;
+        for (int k = 0; k < S.d.num_windows; ++k) all = all && c->done_host[(size_t)(S.first_window + k)] != 0;
+        if (all) { c->sys_retired[si] = 1; changed = true; }
+    }
+    if (!changed) return REWL_OK;
+    auto keep = [&](std::vector<int32_t>& groups) {
+        std::vector<int32_t> out;
+        for (int32_t gid : groups)
+            if (!c->sys_retired[(size_t)c->groups[(size_t)gid].sys]) out.push_back(gid);
+        groups.swap(out);
+    };
+    int rc;
+    for (int cls = 0; cls < 4; ++cls) {
+        keep(c->class_groups[cls]);
+        if ((rc = overwrite(c->d_class_groups[cls], c->class_groups[cls], c->stream))) return rc;
+    }
+    for (int m = 0; m < 2; ++m)
+        for (int i = 0; i < 5; ++i) {
+            keep(c->tuned_groups[m][i]);
+            if ((rc = overwrite(c->d_tuned_groups[m][i], c->tuned_groups[m][i], c->stream))) return rc;
+        }
+    auto keep_pairs = [&](std::vector<int32_t>& pl) {
+        std::vector<int32_t> out;
+        for (int32_t w : pl)
+            if (!c->sys_retired[(size_t)c->win[(size_t)w].sys]) out.push_back(w);
+        pl.swap(out);
+    };
+    keep_pairs(c->pairs_even);
+    keep_pairs(c->pairs_odd);
+    if ((rc = overwrite(c->d_pairs_even, c->pairs_even, c->stream))) return rc;
+    if ((rc = overwrite(c->d_pairs_odd, c->pairs_odd, c->stream))) return rc;
+    std::vector<int32_t> active(c->win.size(), 1);
+    for (size_t k = 0; k < c->win.size(); ++k) active[k] = c->sys_retired[(size_t)c->win[k].sys] ? 0 : 1;
+    if ((rc = overwrite(c->d_win_active, active, c->stream))) return rc;
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return REWL_OK;
+}
+
+// Every system back in the lists (after retirements)
+static int unretire_all(rewl_ctx* c) {
+    bool any = false;
+    for (uint8_t r : c->sys_retired) any = any || r;
+    if (!any) return REWL_OK;
+    int rc;
+    for (int cls = 0; cls < 4; ++cls) {
+        c->class_groups[cls] = c->class_groups_all[cls];
+        if ((rc = overwrite(c->d_class_groups[cls], c->class_groups[cls], c->stream))) return rc;
+    }
+    for (int m = 0; m < 2; ++m)
+        for (int i = 0; i < 5; ++i) {
+            c->tuned_groups[m][i] = c->tuned_groups_all[m][i];
+            if ((rc = overwrite(c->d_tuned_groups[m][i], c->tuned_groups[m][i], c->stream))) return rc;
+        }
+    c->pairs_even = c->pairs_even_all;
+    c->pairs_odd = c->pairs_odd_all;
+    if ((rc = overwrite(c->d_pairs_even, c->pairs_even, c->stream))) return rc;
+    if ((rc = overwrite(c->d_pairs_odd, c->pairs_odd, c->stream))) return rc;
+    if ((rc = overwrite(c->d_win_active, std::vector<int32_t>(c->win.size(), 1), c->stream))) return rc;
+    c->sys_retired.assign(c->sys.size(), 0);
+    HIP_TRY(hipStreamSynchronize(c->stream));
     return REWL_OK;
 }
 
@@ -948,6 +1042,7 @@ int rewl_check(rewl_ctx* c) {
     }
     c->stats.all_done = all;
     c->stats.checks += 1;
+    if (c->p.retire) return retire_done_systems(c);
     return REWL_OK;
 }
 
@@ -980,9 +1075,11 @@ int rewl_refine(rewl_ctx* c, int64_t sweeps, int32_t sweeps_per_exchange) {
     if (!c->in_windows)
         return fail(REWL_ERR_RANGE, "walkers are not inside their windows: call rewl_adjust_to_range first");
     int rc;
-    // the replicas' common curve, then fixed weights: ln f = 0 in every window
+    // every system takes part (retired ones too); the replicas' common
+    // curve, then fixed weights: ln f = 0 in every window
+    if ((rc = unretire_all(c))) return rc;
     if ((rc = rewl_finalize(c))) return rc;
-    if ((rc = upload(c->d_win_lnf, std::vector<double>(c->win.size(), 0.0), c->stream))) return rc;
+    if ((rc = overwrite(c->d_win_lnf, std::vector<double>(c->win.size(), 0.0), c->stream))) return rc;
     const int64_t n = (int64_t)c->hist_total;
     const unsigned grid = (unsigned)std::min<int64_t>(4096, (n + 255) / 256);
     hipLaunchKernelGGL(k_hist_clear, dim3(grid), dim3(256), 0, c->stream, c->args, n);

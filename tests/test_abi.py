@@ -51,7 +51,7 @@ int main(void) {
   printf("rewl_mlp_train_params %zu\n", sizeof(rewl_mlp_train_params));
   printf("REWL_MLP_NUM_PARAMS %d\n", REWL_MLP_NUM_PARAMS);
   F(rewl_system_desc, field) F(rewl_system_desc, seed) F(rewl_system_desc, walker_seeds)
-  F(rewl_params, lnf_initial) F(rewl_params, exchange) F(rewl_params, producers) F(rewl_params, adjust_mode) F(rewl_params, average_mode) F(rewl_params, sweep_kernel) F(rewl_params, reserved)
+  F(rewl_params, lnf_initial) F(rewl_params, exchange) F(rewl_params, producers) F(rewl_params, adjust_mode) F(rewl_params, average_mode) F(rewl_params, sweep_kernel) F(rewl_params, retire)
   F(rewl_window_info, lnf) F(rewl_window_info, one_over_t)
   F(rewl_stats, kernel_ms_sweep) F(rewl_stats, sweep_launches)
   F(rewl_mlp_train_params, lr) F(rewl_mlp_train_params, seed) F(rewl_mlp_train_params, warmup)

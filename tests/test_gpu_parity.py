@@ -382,3 +382,48 @@ def test_autotuner_picks_a_shape_that_runs(gpu):
     eng.adjust_to_range(); ora.adjust_to_range()
     eng.sweeps(10); ora.sweeps(10)
     assert_same(eng, ora, f"autotuned shape {best}", cursors=False)
+
+
+def test_retired_systems_equal_their_own_runs(gpu):
+    """rewl_params.retire = 1: in a batch, a system whose windows are all done
+    stops sweeping, exchanging and checking, which is where a run of that
+    system alone ends.  Each system of a mixed batch, run to completion, ends
+    in the state of its own single-system run, bit for bit, and the batch is
+    bit-exact with the oracle, which retires systems the same way."""
+    print("case: retirement in a batch", flush=True)
+    systems, params = configs.config4(realisations=4, sizes=(4, 6, 8, 6), bins=64, windows=3, replicas=2,
+                                      lnf_final=2e-2)
+    params = dict(params, sweeps_per_check=40, sweeps_per_exchange=20, flatness=0.5)
+    assert params["retire"] == 1
+    eng, ora = make_pair(systems, params)
+    eng.adjust_to_range(); ora.adjust_to_range()
+    checks = 0
+    retired_seen = False
+    while not eng.stats()["all_done"]:
+        for _ in range(2):
+            eng.sweeps(20); ora.sweeps(20)
+            eng.exchange(); ora.exchange()
+        eng.check(); ora.check()
+        checks += 1
+        done = [all(eng.window_info(k)["done"] for k in range(eng.num_windows) if eng.windows[k]["system"] == s)
+                for s in range(len(systems))]
+        retired_seen = retired_seen or (any(done) and not all(done))
+        assert checks < 2000
+    assert retired_seen   # some system finished while others still ran
+    assert_same(eng, ora, "batch with retirement", cursors=False)
+    window_states_equal(eng, ora)
+    per = 3 * 2
+    for si, s in enumerate(systems):
+        with Engine([s], params) as alone:
+            alone.adjust_to_range()
+            while not alone.stats()["all_done"]:
+                for _ in range(2):
+                    alone.sweeps(20)
+                    alone.exchange()
+                alone.check()
+            for j in range(per):
+                g = si * per + j
+                assert alone.energy(j) == eng.energy(g), (si, j)
+                np.testing.assert_array_equal(alone.spins(j), eng.spins(g))
+                np.testing.assert_array_equal(bits(alone.logdos(j)), bits(eng.logdos(g)))
+    eng.close(); ora.close()
