@@ -73,6 +73,8 @@ def parse():
     ap.add_argument("--refine", type=int, default=0, metavar="SWEEPS",
                     help="with --converge: then SWEEPS sweeps of multicanonical refinement (rewl_refine) and "
                          "the ln g error after it")
+    ap.add_argument("--no-retire", action="store_true",
+                    help="with --workload config4 --converge: keep finished realisations in the launches")
     ap.add_argument("--half-range", action="store_true",
                     help="with --workload config3 --converge: the reference's own Ashkin-Teller range [-5N, 0) "
                          "instead of the full spectrum")
@@ -367,6 +369,73 @@ def batched(a):
     print(json.dumps(line), flush=True)
 
 
+def converge_batch(a):
+    """Time to ln f < a.converge for every realisation of the batched
+    random-strain workload (config 4) in one context: checks of
+    sweeps_per_step sweeps, an exchange and a check; with retirement (the
+    default) a realisation whose windows are all done leaves the launches.
+    Reports when the last realisation of each lattice size finished, the
+    sweep-launch time per check at the start and at the end (what retirement
+    saves), and how many realisations finished within --max-checks."""
+    from rewl_simulator_amd import configs
+    from rewl_simulator_amd.engine import Engine
+    systems, params = configs.config4(lnf_final=a.converge)
+    n = a.sweeps_per_step
+    params = dict(params, sweeps_per_check=n, sweeps_per_exchange=n, producers=a.producers,
+                  sweep_kernel=a.sweep_kernel, walkers_per_group=a.walkers_per_group,
+                  retire=0 if a.no_retire else 1)
+    eng = Engine(systems, params)
+    eng.adjust_to_range()
+    wins = {}
+    for k in range(eng.num_windows):
+        wins.setdefault(eng.windows[k]["system"], []).append(k)
+    sizes = [s["L"] for s in systems]
+    finished = {}
+    launch_ms = []
+    eng.synchronize()
+    st0 = eng.stats()
+    t0 = time.perf_counter()
+    checks = 0
+    while checks < a.max_checks and len(finished) < len(systems):
+        ms0 = eng.stats()["kernel_ms_sweep"]
+        eng.sweeps(n)
+        eng.exchange()
+        eng.check()
+        checks += 1
+        launch_ms.append(eng.stats()["kernel_ms_sweep"] - ms0)
+        if checks % 5 == 0:
+            now = time.perf_counter() - t0
+            for si, ks in wins.items():
+                if si not in finished and all(eng.window_info(k)["done"] for k in ks):
+                    finished[si] = now
+            if checks % 20 == 0:
+                print(f"# check {checks}: {len(finished)} of {len(systems)} realisations done, {now:.1f} s",
+                      flush=True)
+    eng.synchronize()
+    t1 = time.perf_counter()
+    st1 = eng.stats()
+    eng.close()
+    by_L = {}
+    for L in sorted(set(sizes)):
+        idx = [i for i, x in enumerate(sizes) if x == L]
+        done = [finished[i] for i in idx if i in finished]
+        by_L[str(L)] = {"realisations": len(idx), "finished": len(done),
+                        "last_finished_s": max(done) if len(done) == len(idx) else None}
+    head = launch_ms[:10]
+    tail = launch_ms[-10:]
+    print(json.dumps({
+        "metric": "time to ln f < %g, every realisation of the batched random-strain workload" % a.converge,
+        "value": t1 - t0 if len(finished) == len(systems) else None, "unit": "s", "higher_is_better": False,
+        "converged": len(finished) == len(systems), "checks": checks, "sweeps_per_check": n,
+        "retire": not a.no_retire, "sweeps": st1["sweeps"] - st0["sweeps"],
+        "attempts": st1["attempts"] - st0["attempts"], "wall_s": t1 - t0,
+        "by_L": by_L,
+        "sweep_launch_ms_per_check": {"first_10_mean": sum(head) / max(len(head), 1),
+                                      "last_10_mean": sum(tail) / max(len(tail), 1)},
+        "config": {"workload": AT_WORKLOADS["config4"]["workload"]},
+    }), flush=True)
+
+
 def converge_at(a):
     """Time to ln f < a.converge on the Ashkin-Teller workload (config 3):
     wall time from the first sweep until the slowest window's ln f is below
@@ -515,6 +584,8 @@ def main():
     if a.workload in AT_WORKLOADS:
         if a.converge > 0 and a.workload == "config3":
             converge_at(a)
+        elif a.converge > 0:
+            converge_batch(a)
         else:
             batched(a)
         return

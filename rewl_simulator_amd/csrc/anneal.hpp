@@ -125,7 +125,7 @@ __global__ void __launch_bounds__(kLanes) k_anneal(AnnealArgs a) {
     __syncthreads();
     if (chain >= a.chains) return;   // no barrier follows
     uint64_t* lat = a.lat + (int64_t)gid * kAnnealGroupWords + lane;
-    constexpr int NP = MODEL == 0 ? 1 : 2;
+    constexpr int NP = Model<MODEL>::kPlanes;
     const int L = S.L;
 
     // randomize_dofs: spin_array[j] = u < 0.5 ? +1 : -1, j in the reference
@@ -147,8 +147,7 @@ __global__ void __launch_bounds__(kLanes) k_anneal(AnnealArgs a) {
             for (int q = 0; q < NP; ++q) lat[(int64_t)(q * L + y) * kLanes] = row[q];
         }
     }
-    double E = recalc_energy(S, a.lat + (int64_t)gid * kAnnealGroupWords, lane);
-    const double mJ = -S.J, mK = -S.K;
+    double E = recalc_energy<MODEL>(S, a.lat + (int64_t)gid * kAnnealGroupWords, lane);
     double emin = __builtin_inf();   // the reference leaves min_energy_found uninitialised
     unsigned long long total = 0ull;
     uint64_t ctr = 0ull;
@@ -163,7 +162,7 @@ __global__ void __launch_bounds__(kLanes) k_anneal(AnnealArgs a) {
         int blocks = 0;
         while (incomplete) {
             for (int b = 0; b < a.block_size; ++b) {
-                for (int pass = 0; pass < NP; ++pass) {
+                for (int pass = 0; pass < Model<MODEL>::kPasses; ++pass) {
                     uint64_t up[NP], cu[NP], dn[NP], r0[NP], nx[NP];
 #pragma unroll
                     for (int q = 0; q < NP; ++q) {
@@ -184,52 +183,24 @@ __global__ void __launch_bounds__(kLanes) k_anneal(AnnealArgs a) {
                             const int site = y * L + x;
                             const int xl = (x == 0) ? L - 1 : x - 1;
                             const int xr = (x == L - 1) ? 0 : x + 1;
-                            double En;
-                            if constexpr (MODEL == 0) {
-                                // J s_n is exactly +-J: a select, summed in the reference order
-                                const double sv = pm(bit(cu[0], x));
-                                double f = S.field ? s_field[site] : S.h;
-                                f += bit(up[0], x) ? mJ : S.J;
-                                f += bit(cu[0], xl) ? mJ : S.J;
-                                f += bit(cu[0], xr) ? mJ : S.J;
-                                f += bit(dn[0], x) ? mJ : S.J;
-                                const double tv = -sv;
-                                En = E + -1. * tv * f - -1. * sv * f;
-                                if (metropolis(E, En, beta, u)) {
-                                    cu[0] ^= 1ull << x;
-                                    E = En;
-                                }
-                            } else {
-                                // Every product of couplings and spins in the local energy
-                                // (J s s', J t t', K s s' t t', h s t) is exactly +-J, +-K or
-                                // +-h, so each term is a select on the parity of its spin
-                                // bits (bit 1 = -1); the terms are summed in the reference
-                                // order, so the sums round as there.
-                                const uint32_t sb = bit(cu[0], x), tb = bit(cu[NP - 1], x);
-                                const uint32_t tsb = sb ^ (uint32_t)(fl == 0), ttb = tb ^ (uint32_t)(fl == 1);
-                                const uint32_t nsb[4] = {bit(up[0], x), bit(cu[0], xl), bit(cu[0], xr), bit(dn[0], x)};
-                                const uint32_t ntb[4] = {bit(up[NP - 1], x), bit(cu[NP - 1], xl), bit(cu[NP - 1], xr),
-                                                         bit(dn[NP - 1], x)};
-                                double en_new = 0., en_old = 0.;
-                                if (S.has_field) {
-                                    const double h = S.field ? s_field[site] : S.h;
-                                    en_new += (tsb ^ ttb) ? -h : h;
-                                    en_old += (sb ^ tb) ? -h : h;
-                                }
+                            // the spins as +-1 (every product of couplings and spins
+                            // in the local energy is exact), summed in the reference
+                            // order by the model's change_state
+                            double own[NP], nb[NP][4];
 #pragma unroll
-                                for (int k = 0; k < 4; ++k) {
-                                    en_new += (tsb ^ nsb[k]) ? mJ : S.J;
-                                    en_new += (ttb ^ ntb[k]) ? mJ : S.J;
-                                    en_new += (tsb ^ ttb ^ nsb[k] ^ ntb[k]) ? mK : S.K;
-                                    en_old += (sb ^ nsb[k]) ? mJ : S.J;
-                                    en_old += (tb ^ ntb[k]) ? mJ : S.J;
-                                    en_old += (sb ^ tb ^ nsb[k] ^ ntb[k]) ? mK : S.K;
-                                }
-                                En = E + -en_new - -en_old;
-                                if (metropolis(E, En, beta, u)) {
-                                    cu[fl] ^= 1ull << x;
-                                    E = En;
-                                }
+                            for (int q = 0; q < NP; ++q) {
+                                own[q] = pm(bit(cu[q], x));
+                                nb[q][0] = pm(bit(up[q], x));
+                                nb[q][1] = pm(bit(cu[q], xl));
+                                nb[q][2] = pm(bit(cu[q], xr));
+                                nb[q][3] = pm(bit(dn[q], x));
+                            }
+                            const double h = S.field ? s_field[site] : S.h;
+                            const double En = Model<MODEL>::proposal_energy(S.J, S.K, S.has_field, h, E, own, nb, fl);
+                            if (metropolis(E, En, beta, u)) {
+                                const int P = Model<MODEL>::flip_plane(fl);
+                                plane_set(cu, P, plane_pick(cu, P) ^ (1ull << x));
+                                E = En;
                             }
                             emin = E < emin ? E : emin;
                         }
@@ -242,7 +213,7 @@ __global__ void __launch_bounds__(kLanes) k_anneal(AnnealArgs a) {
                             dn[q] = (y + 2 < L) ? nx[q] : r0[q];
                         }
                     }
-                    if (MODEL == 1) fl ^= 1;   // the flavour switches after site N - 1
+                    fl = Model<MODEL>::next_flavour(fl);   // the flavour switches after site N - 1
                 }
                 eb[(int64_t)b * kLanes] = E;
             }

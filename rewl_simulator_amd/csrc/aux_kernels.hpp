@@ -16,18 +16,15 @@ __device__ __forceinline__ int win_bin(const DevWindow& W, double e) {
 }
 
 // Ising2d::recalculate_state / Ashkin_Teller2d::recalculate_state
-// (ising2d_hamiltonian.hpp:190-210, ashkin_teller2d_hamiltonian.hpp:193-217).
+// (ising2d_hamiltonian.hpp:190-210, ashkin_teller2d_hamiltonian.hpp:193-217):
+// the per-site terms of the model summed over the sites in order.
+template <int M>
 __device__ double recalc_energy(const DevSystem& S, const uint64_t* lat, int64_t base) {
     double E = 0.;
     for (int i = 0; i < S.N; ++i) {
-        if (S.model == 0) {
-            const double sv = spin_at(lat, base, S.L, 0, i);
-            E += -0.5 * field_at(S, i) * sv + 0.5 * ising_local_energy_mem(S, lat, base, i, sv);
-        } else {
-            const double sg = spin_at(lat, base, S.L, 0, i), ta = spin_at(lat, base, S.L, 1, i);
-            if (S.has_field) E += -0.5 * field_at(S, i) * sg * ta;
-            E += 0.5 * at_local_energy_mem(S, lat, base, i, sg, ta);
-        }
+        double own[Model<M>::kPlanes], nb[Model<M>::kPlanes][4];
+        site_spins_mem<M>(S, lat, base, i, own, nb);
+        E = Model<M>::add_site_energy(E, S.J, S.K, S.has_field, field_at(S, i), own, nb);
     }
     return E;
 }
@@ -35,42 +32,45 @@ __device__ double recalc_energy(const DevSystem& S, const uint64_t* lat, int64_t
 __global__ void k_recalc(KArgs a) {
     const int g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= a.num_walkers) return;
-    a.w_E[g] = recalc_energy(sys_of(a, g), a.lat, a.w_lat[g]);
-}
-
-// Proposal of REWL_Walker::adjust_state_to_range via change_state; FP64 as in
-// the reference.  Returns the proposed energy; the AT flavour advances after
-// site N-1 exactly as change_state does.
-__device__ double propose_fp(const DevSystem& S, const uint64_t* lat, int64_t base, int site, int& fl, double E,
-                             bool toggle_at_end = true) {
-    double En;
-    if (S.model == 0) {
-        const double sv = spin_at(lat, base, S.L, 0, site);
-        const double t = -sv;
-        En = E + ising_local_energy_mem(S, lat, base, site, t) - ising_local_energy_mem(S, lat, base, site, sv);
-    } else {
-        const double cs = spin_at(lat, base, S.L, 0, site), ct = spin_at(lat, base, S.L, 1, site);
-        double ts = cs, tt = ct;
-        ts += -2. * ts * (double)(fl == 0);
-        tt += -2. * tt * (double)(fl == 1);
-        En = E + at_local_energy_mem(S, lat, base, site, ts, tt) - at_local_energy_mem(S, lat, base, site, cs, ct);
-        if (toggle_at_end && site == S.N - 1) fl ^= 1;
-    }
-    return En;
+    const DevSystem& S = sys_of(a, g);
+    a.w_E[g] = with_model(S.model, [&](auto m) { return recalc_energy<decltype(m)::value>(S, a.lat, a.w_lat[g]); });
 }
 
 // REWL_Walker::adjust_state_to_range (rewl_walker.hpp:71-115): one draw picks
 // the starting site, then sites are scanned in order, accepting flips that move
-// the energy toward the window.
-// mode 1 keeps only the toggle at the wrap, so sigma and tau passes alternate.
+// the energy toward the window.  Each proposal is change_state's (FP64, the
+// reference's order); the flavour of a model with several advances at the
+// wrap to site 0 and, with mode 0, also after site N-1 as change_state does
+// (mode 1 keeps only the toggle at the wrap, so sigma and tau passes alternate).
+template <int M>
+__device__ void adjust_walker(const DevWindow& W, const DevSystem& S, uint64_t* lat, int64_t base, double& E,
+                              int& fl, double u, uint64_t max_steps, int mode, int32_t* err) {
+    uint64_t site = (uint64_t)(u * S.N);
+    bool in = win_in_range(W, E);
+    uint64_t steps = 0;
+    while (!in) {
+        if (++steps > max_steps) { atomicOr(err, 2); break; }
+        ++site;
+        if (site == (uint64_t)S.N) { site = 0; fl = Model<M>::next_flavour(fl); }
+        const int plane = Model<M>::flip_plane(fl);
+        const double En = proposal_energy_mem<M>(S, lat, base, (int)site, fl, E);
+        if (mode == 0 && site == (uint64_t)(S.N - 1)) fl = Model<M>::next_flavour(fl);
+        const bool low = trunc_half(E) < W.phi_lo, high = trunc_half(E) >= W.phi_hi;
+        if ((low && En > E) || (!low && high && En <= E)) {
+            const int x = (int)site % S.L, y = (int)site / S.L;
+            lat[base + (int64_t)(plane * S.L + y) * kLanes] ^= (1ull << x);
+            E = En;
+        }
+        in = win_in_range(W, E);
+    }
+}
+
 template <bool REPLAY>
 __global__ void k_adjust(KArgs a, uint64_t max_steps, int mode) {
     const int g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= a.num_walkers) return;
     const DevWindow W = a.win[a.w_window[g]];
     const DevSystem& S = a.sys[W.sys];
-    uint64_t* lat = a.lat;
-    const int64_t base = a.w_lat[g];
     double E = a.w_E[g];
     int fl = a.w_flavor[g];
     double u;
@@ -82,23 +82,10 @@ __global__ void k_adjust(KArgs a, uint64_t max_steps, int mode) {
     } else {
         u = canonical_from_u64(philox_u64(S.k0, S.k1, DOMAIN_INIT, (uint32_t)a.w_local[g], 0ull));
     }
-    uint64_t site = (uint64_t)(u * S.N);
-    bool in = win_in_range(W, E);
-    uint64_t steps = 0;
-    while (!in) {
-        if (++steps > max_steps) { atomicOr(a.err, 2); break; }
-        ++site;
-        if (site == (uint64_t)S.N) { site = 0; if (S.model == 1) fl ^= 1; }
-        const int plane = (S.model == 1 && fl == 1) ? 1 : 0;
-        const double En = propose_fp(S, lat, base, (int)site, fl, E, mode == 0);
-        const bool low = trunc_half(E) < W.phi_lo, high = trunc_half(E) >= W.phi_hi;
-        if ((low && En > E) || (!low && high && En <= E)) {
-            const int x = (int)site % S.L, y = (int)site / S.L;
-            lat[base + (int64_t)(plane * S.L + y) * kLanes] ^= (1ull << x);
-            E = En;
-        }
-        in = win_in_range(W, E);
-    }
+    with_model(S.model, [&](auto m) {
+        adjust_walker<decltype(m)::value>(W, S, a.lat, a.w_lat[g], E, fl, u, max_steps, mode, a.err);
+        return 0;
+    });
     a.w_E[g] = E;
     a.w_flavor[g] = fl;
 }
@@ -228,7 +215,7 @@ __global__ void __launch_bounds__(kLanes) k_exchange(KArgs a, const int32_t* pai
         }
     }
     __syncthreads();
-    const int words = (S.model == 0 ? 1 : 2) * S.L;
+    const int words = with_model(S.model, [](auto m) { return Model<decltype(m)::value>::kPlanes; }) * S.L;
     for (int r = 0; r < R; ++r) {
         if (!accept[r]) continue;
         const int ga = WL.first_walker + r, gb = WU.first_walker + shuffled[r] - R;
@@ -253,16 +240,20 @@ __global__ void __launch_bounds__(kLanes) k_exchange(KArgs a, const int32_t* pai
             const int64_t hi[2] = {ia, ib};
             const int gw[2] = {ga, gb};
             for (int j = 0; j < 2; ++j) {
-                int m[3];
                 const uint64_t* lt = a.lat + a.w_lat[gw[j]];
-                lattice_magnetisations(lt, S.L, S.model, m);
-                update_observables(a.obs + hi[j] * a.nobs, S.model, S.N, m, a.obs_corr != 0);
-                if (S.model == 1 && a.dens) update_density(a.dens, hi[j], S.N, m);
-                if (a.obs_corr) {
-                    const uint32_t t = a.obs_tot[hi[j]];
-                    a.obs_tot[hi[j]] = t + 1u;
-                    if (t % 1000u == 0u) add_correlators(a.obs + hi[j] * a.nobs, lt, S.L, S.model, -1, nullptr, s_cs);
-                }
+                with_model(S.model, [&](auto mt) {
+                    constexpr int M = decltype(mt)::value;
+                    int m[3];
+                    lattice_magnetisations<M>(lt, S.L, m);
+                    update_observables<M>(a.obs + hi[j] * a.nobs, S.N, m, a.obs_corr != 0);
+                    if (Model<M>::kDensities && a.dens) update_density(a.dens, hi[j], S.N, m);
+                    if (a.obs_corr) {
+                        const uint32_t t = a.obs_tot[hi[j]];
+                        a.obs_tot[hi[j]] = t + 1u;
+                        if (t % 1000u == 0u) add_correlators<M>(a.obs + hi[j] * a.nobs, lt, S.L, -1, nullptr, s_cs);
+                    }
+                    return 0;
+                });
             }
         }
     }

@@ -14,6 +14,7 @@
 #include <stdint.h>
 
 #include "philox.hpp"
+#include "models.hpp"
 
 namespace rewl {
 
@@ -97,6 +98,22 @@ __device__ __forceinline__ void neighbors(int site, int L, int nb[4]) {
     nb[3] = (y == L - 1 ? 0 : y + 1) * L + x;
 }
 
+// w[p] of a per-plane register array, and its assignment, as selects over
+// the planes (a runtime index into a register array would place the array in
+// scratch memory).
+template <int NP, class T>
+__device__ __forceinline__ T plane_pick(const T (&w)[NP], int p) {
+    T r = w[0];
+#pragma unroll
+    for (int q = 1; q < NP; ++q) r = p == q ? w[q] : r;
+    return r;
+}
+template <int NP, class T, class V>
+__device__ __forceinline__ void plane_set(T (&w)[NP], int p, V v) {
+#pragma unroll
+    for (int q = 0; q < NP; ++q) w[q] = p == q ? (T)v : w[q];
+}
+
 // Spin of plane p at site as +-1 double, read from the packed lattice.
 __device__ __forceinline__ double spin_at(const uint64_t* lat, int64_t base, int L, int plane, int site) {
     const int x = site % L, y = site / L;
@@ -108,47 +125,44 @@ __device__ __forceinline__ double field_at(const DevSystem& S, int site) {
     return S.field ? S.field[site] : S.h;
 }
 
-// Ising2d::local_energy (ising2d_hamiltonian.hpp:162-188), FP64, reference order.
-__device__ __forceinline__ double ising_local_energy_mem(const DevSystem& S, const uint64_t* lat, int64_t base,
-                                                         int site, double sv) {
-    int nb[4];
-    neighbors(site, S.L, nb);
-    double f = field_at(S, site);
+// The spins of one site and of its four neighbours in the reference order
+// (x,y-1),(x-1,y),(x+1,y),(x,y+1), every plane of model M, read from memory.
+template <int M>
+__device__ __forceinline__ void site_spins_mem(const DevSystem& S, const uint64_t* lat, int64_t base, int site,
+                                               double own[Model<M>::kPlanes], double nb[Model<M>::kPlanes][4]) {
+    int n[4];
+    neighbors(site, S.L, n);
 #pragma unroll
-    for (int k = 0; k < 4; ++k) f += S.J * spin_at(lat, base, S.L, 0, nb[k]);
-    return -1. * sv * f;
-}
-
-// Ashkin_Teller2d::local_energy (ashkin_teller2d_hamiltonian.hpp:169-191).
-__device__ __forceinline__ double at_local_energy_mem(const DevSystem& S, const uint64_t* lat, int64_t base,
-                                                      int site, double sg, double ta) {
-    int nb[4];
-    neighbors(site, S.L, nb);
-    double en = 0.;
-    if (S.has_field) en += field_at(S, site) * (sg * ta);
+    for (int q = 0; q < Model<M>::kPlanes; ++q) {
+        own[q] = spin_at(lat, base, S.L, q, site);
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        const double sn = spin_at(lat, base, S.L, 0, nb[k]);
-        const double tn = spin_at(lat, base, S.L, 1, nb[k]);
-        en += S.J * sg * sn;
-        en += S.J * ta * tn;
-        en += S.K * sg * ta * sn * tn;
+        for (int k = 0; k < 4; ++k) nb[q][k] = spin_at(lat, base, S.L, q, n[k]);
     }
-    return -en;
 }
 
-// Magnetisations of a walker's lattice (bit 1 = spin -1): Ising m[0] = M;
-// Ashkin-Teller m[0] = sum sigma, m[1] = sum tau, m[2] = sum sigma tau.
-__device__ __forceinline__ void lattice_magnetisations(const uint64_t* lat, int L, int model, int m[3]) {
+// E' of the proposal at `site` with flavour fl, from the lattice in memory
+// (change_state in the reference order).
+template <int M>
+__device__ __forceinline__ double proposal_energy_mem(const DevSystem& S, const uint64_t* lat, int64_t base, int site,
+                                                      int fl, double E) {
+    double own[Model<M>::kPlanes], nb[Model<M>::kPlanes][4];
+    site_spins_mem<M>(S, lat, base, site, own, nb);
+    return Model<M>::proposal_energy(S.J, S.K, S.has_field, field_at(S, site), E, own, nb, fl);
+}
+
+// Magnetisations of a walker's lattice (bit 1 = spin -1): m[0] = sum of
+// plane 0 (Ising M, AT sigma), m[1] = sum of the last plane (AT tau), m[2] =
+// sum of their product (AT Baxter).
+template <int M>
+__device__ __forceinline__ void lattice_magnetisations(const uint64_t* lat, int L, int m[3]) {
+    constexpr int NP = Model<M>::kPlanes;
     int ps = 0, pt = 0, px = 0;
     for (int y = 0; y < L; ++y) {
         const uint64_t s = lat[(int64_t)y * kLanes];
+        const uint64_t t = lat[(int64_t)((NP - 1) * L + y) * kLanes];
         ps += __popcll(s);
-        if (model == 1) {
-            const uint64_t t = lat[(int64_t)(L + y) * kLanes];
-            pt += __popcll(t);
-            px += __popcll(s ^ t);
-        }
+        pt += __popcll(t);
+        px += __popcll(s ^ t);
     }
     const int N = L * L;
     m[0] = N - 2 * ps;
@@ -164,54 +178,17 @@ __device__ __forceinline__ void lattice_magnetisations(const uint64_t* lat, int 
 __device__ __forceinline__ void obs_add(double* p, double v) {
     __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-// With corr the layout leaves a correlator slot after each group of three
-// (the reference's CORRELATION_LENGTHS enum order).
-__device__ __forceinline__ void update_observables(double* obs, int model, int N, const int m[3], bool corr = false) {
-    const double divide_N = 1. / (double)N;
-    if (model == 0) {
-        const double v = divide_N * fabs((double)m[0]);
-        obs_add(obs + 0, v);
-        obs_add(obs + 1, v * v);
-        obs_add(obs + 2, v * v * v * v);
-    } else {
-        const int k = corr ? 4 : 3;
-        const double sv = divide_N * fabs((double)m[0]), tv = divide_N * fabs((double)m[1]);
-        const double bv = divide_N * fabs((double)m[2]);
-        const double phi2 = sv * sv + tv * tv;
-        obs_add(obs + 0, sv);
-        obs_add(obs + 1, sv * sv);
-        obs_add(obs + 2, sv * sv * sv * sv);
-        obs_add(obs + k + 0, tv);
-        obs_add(obs + k + 1, tv * tv);
-        obs_add(obs + k + 2, tv * tv * tv * tv);
-        obs_add(obs + 2 * k + 0, sqrt(phi2));
-        obs_add(obs + 2 * k + 1, phi2);
-        obs_add(obs + 2 * k + 2, phi2 * phi2);
-        obs_add(obs + 3 * k + 0, bv);
-        obs_add(obs + 3 * k + 1, bv * bv);
-        obs_add(obs + 3 * k + 2, bv * bv * bv * bv);
-    }
-}
-
-// The values update_observables adds, in group order: Ising |m|, m^2, m^4;
-// AT sigma, tau, phi, Baxter, each (|x|, x^2, x^4) (phi: sqrt(phi2), phi2,
-// phi2^2).  Value i goes to slot (i / 3) * k + i % 3, k = 3, or 4 with the
-// correlator slots.
-template <int MODEL>
-__device__ __forceinline__ void observable_values(int N, const int m[3], double* v) {
-    const double divide_N = 1. / (double)N;
-    if (MODEL == 0) {
-        const double x = divide_N * fabs((double)m[0]);
-        v[0] = x; v[1] = x * x; v[2] = x * x * x * x;
-    } else {
-        const double sv = divide_N * fabs((double)m[0]), tv = divide_N * fabs((double)m[1]);
-        const double bv = divide_N * fabs((double)m[2]);
-        const double phi2 = sv * sv + tv * tv;
-        v[0] = sv; v[1] = sv * sv; v[2] = sv * sv * sv * sv;
-        v[3] = tv; v[4] = tv * tv; v[5] = tv * tv * tv * tv;
-        v[6] = sqrt(phi2); v[7] = phi2; v[8] = phi2 * phi2;
-        v[9] = bv; v[10] = bv * bv; v[11] = bv * bv * bv * bv;
-    }
+// update_observables at a bin (the model's values, Model<M>::values), added
+// to the bin's sums: value i goes to slot (i / 3) * k + i % 3, k = 3, or 4
+// when the layout leaves a correlator slot after each group of three (the
+// reference's CORRELATION_LENGTHS enum order).
+template <int M>
+__device__ __forceinline__ void update_observables(double* obs, int N, const int m[3], bool corr = false) {
+    double v[Model<M>::kValues];
+    Model<M>::values(N, m, v);
+    const int k = corr ? 4 : 3;
+#pragma unroll
+    for (int i = 0; i < Model<M>::kValues; ++i) obs_add(obs + (i / 3) * k + i % 3, v[i]);
 }
 
 // update_densities (ashkin_teller_densities.hpp:17-34, called at
@@ -284,23 +261,19 @@ __device__ double fourier_qmin(int L, RowFn row, const double* cs) {
 
 // update_qmin_correlator's samples (ising2d_hamiltonian.hpp:247-252,
 // ashkin_teller2d_hamiltonian.hpp:296-316) added to the correlator slots of
-// one bin: Ising slot 3; AT sigma 3, tau 7 (computed from sigma, as the
-// reference does), phi 11 = sigma + tau, Baxter 15 (field sigma tau).  Row y
-// of the lattice is taken from `cur` (the sweep's register copy), the others
+// one bin where the model places them (Model<M>::add_correlators).  Row y of
+// the lattice is taken from `cur` (the sweep's register copy), the others
 // from HBM; y < 0 reads every row from HBM.
-__device__ __noinline__ void add_correlators(double* obs, const uint64_t* lat, int L, int model, int y,
-                                             const uint64_t* cur, const double* cs) {
+template <int M>
+__device__ __noinline__ void add_correlators(double* obs, const uint64_t* lat, int L, int y, const uint64_t* cur,
+                                             const double* cs) {
+    constexpr int NP = Model<M>::kPlanes;
     auto plane = [&](int q, int r) -> uint64_t { return r == y ? cur[q] : lat[(int64_t)(q * L + r) * kLanes]; };
     const double gs = fourier_qmin(L, [&](int r) { return plane(0, r); }, cs);
-    if (model == 0) {
-        obs_add(obs + 3, gs);
-        return;
-    }
-    const double gb = fourier_qmin(L, [&](int r) { return plane(0, r) ^ plane(1, r); }, cs);
-    obs_add(obs + 3, gs);
-    obs_add(obs + 7, gs);
-    obs_add(obs + 11, gs + gs);
-    obs_add(obs + 15, gb);
+    double gb = 0.;
+    if (Model<M>::kBaxterCorrelator)
+        gb = fourier_qmin(L, [&](int r) { return plane(0, r) ^ plane(NP - 1, r); }, cs);
+    Model<M>::add_correlators([&](int slot, double g) { obs_add(obs + slot, g); }, gs, gb);
 }
 
 }  // namespace rewl

@@ -27,6 +27,8 @@
 #include "sweep_tuned.hpp"
 #include "anneal.hpp"
 
+static_assert(REWL_ISING2D == 0 && REWL_ASHKIN_TELLER2D == 1, "the C ABI's model index is Model<M>'s M");
+
 using namespace rewl;
 
 namespace {
@@ -499,7 +501,7 @@ int rewl_create(const rewl_system_desc* systems, int32_t num_systems, const rewl
         HostSystem S;
         S.d = d;
         S.N = d.L * d.L;
-        S.ndof = d.model == REWL_ISING2D ? S.N : 2 * S.N;
+        S.ndof = model_info(d.model).planes * S.N;
         if (d.field) S.field.assign(d.field, d.field + S.N);
         S.d.field = nullptr;
         S.has_field = d.field != nullptr || d.h != 0.0;
@@ -507,7 +509,7 @@ int rewl_create(const rewl_system_desc* systems, int32_t num_systems, const rewl
         S.twoJ = (int)tJ;
         S.twoK = (int)tK;
         auto wins = make_windows(d);
-        bool ip = !S.has_field && integral(tJ) && (d.model == REWL_ISING2D || integral(tK));
+        bool ip = !S.has_field && integral(tJ) && (!model_info(d.model).product_coupling || integral(tK));
         for (auto& w : wins) {
             if (w.nbins < 1) return bail(fail(REWL_ERR_INVALID, "a window has no bins"));
             if (P.flatness_mode == REWL_FLAT_MIN_OVER_MEAN_VISITED && w.nbins > kMaxCheckBins)
@@ -558,7 +560,7 @@ int rewl_create(const rewl_system_desc* systems, int32_t num_systems, const rewl
     for (int si = 0; si < (int)c->sys.size(); ++si) {
         const HostSystem& S = c->sys[si];
         const int nw = S.d.num_windows * S.d.replicas_per_window;
-        const int planes = S.d.model == REWL_ISING2D ? 1 : 2;
+        const int planes = model_info(S.d.model).planes;
         int maxbins = 1;
         for (int k = 0; k < S.d.num_windows; ++k) maxbins = std::max(maxbins, c->win[S.first_window + k].nbins);
         // The throughput kernel keeps FP64 ln g and u32 counts of every walker
@@ -585,7 +587,8 @@ int rewl_create(const rewl_system_desc* systems, int32_t num_systems, const rewl
         // guard bins: |Delta E| <= 4 (2J) (+ 4 (2K)) on the integer path
         int guard = 0;
         if (tuned) {
-            const long long maxde = 4LL * std::abs(S.twoJ) + (S.d.model == REWL_ASHKIN_TELLER2D ? 4LL * std::abs(S.twoK) : 0LL);
+            const long long maxde =
+                4LL * std::abs(S.twoJ) + (model_info(S.d.model).product_coupling ? 4LL * std::abs(S.twoK) : 0LL);
             const long long bs = (long long)c->win[S.first_window].bs;
             guard = (int)((maxde + bs - 1) / bs) + 1;
             // observables = 2: the cos/sin table of the correlator after the rings
@@ -609,10 +612,9 @@ int rewl_create(const rewl_system_desc* systems, int32_t num_systems, const rewl
         {
             // E = -1 collides with E = 0 in the reference's +0.5 range test; it
             // matters only if E = -1 is reachable and a window bound is 0.
-            const long long g = S.d.model == REWL_ISING2D ? 2LL * S.twoJ
-                                                          : 2LL * std::gcd((long long)S.twoJ, (long long)S.twoK);
-            const long long e0 = S.d.model == REWL_ISING2D ? -(long long)S.N * S.twoJ
-                                                           : -(long long)S.N * (2LL * S.twoJ + S.twoK);
+            const ModelInfo mi = model_info(S.d.model);
+            const long long g = mi.energy_step(S.twoJ, S.twoK);
+            const long long e0 = mi.ground_energy(S.N, S.twoJ, S.twoK);
             bool zero_bound = false;
             for (int k = 0; k < S.d.num_windows; ++k)
                 zero_bound = zero_bound || c->win[S.first_window + k].emin == 0.0 || c->win[S.first_window + k].emax == 0.0;
@@ -1194,8 +1196,8 @@ int rewl_get_observables(rewl_ctx* c, int32_t g, double* sums, int32_t n, int32_
     if (!c->d_obs.p) return fail(REWL_ERR_INVALID, "observables are off (rewl_params.observables)");
     const HostSystem& S = c->sys[c->win[c->w_window[g]].sys];
     const bool corr = c->p.observables == 2;
-    const int want = S.d.model == REWL_ISING2D ? (corr ? REWL_NUM_OBS_ISING_CORR : REWL_NUM_OBS_ISING)
-                                               : (corr ? REWL_NUM_OBS_AT_CORR : REWL_NUM_OBS_AT);
+    // groups of three moments, each followed by a correlator slot with corr
+    const int want = model_info(S.d.model).values / 3 * (corr ? 4 : 3);
     if (nobs != want) return fail(REWL_ERR_INVALID, "nobs does not match the walker's model");
     const int nb = c->win[c->w_window[g]].nbins;
     if (n < nb) return fail(REWL_ERR_INVALID, "output buffer too small");
@@ -1222,7 +1224,7 @@ int rewl_get_spins(rewl_ctx* c, int32_t g, int8_t* out, int32_t n) {
     if (!walker_ok(c, g) || !out) return fail(REWL_ERR_INVALID, "bad walker");
     const HostSystem& S = c->sys[c->win[c->w_window[g]].sys];
     if (n < S.ndof) return fail(REWL_ERR_INVALID, "output buffer too small");
-    const int planes = S.d.model == REWL_ISING2D ? 1 : 2, L = S.d.L;
+    const int planes = model_info(S.d.model).planes, L = S.d.L;
     std::vector<uint64_t> words((size_t)planes * L);
     HIP_TRY(hipMemcpy2DAsync(words.data(), sizeof(uint64_t), c->d_lat.p + c->w_lat[g], kLanes * sizeof(uint64_t),
                              sizeof(uint64_t), words.size(), hipMemcpyDeviceToHost, c->stream));
@@ -1278,7 +1280,7 @@ int rewl_set_window_lnf(rewl_ctx* c, int32_t wi, double lnf) {
 // Bit-packed rows (plane-major, bit x of row y = site y*L+x, 1 = spin -1) of a
 // configuration given in the reference's DoF layout.
 static std::vector<uint64_t> pack_rows(const HostSystem& S, const int8_t* spins) {
-    const int planes = S.d.model == REWL_ISING2D ? 1 : 2, L = S.d.L;
+    const int planes = model_info(S.d.model).planes, L = S.d.L;
     std::vector<uint64_t> words((size_t)planes * L, 0ull);
     for (int i = 0; i < S.N; ++i) {
         const int x = i % L, y = i / L;
@@ -1539,7 +1541,7 @@ int rewl_anneal(const rewl_system_desc* systems, int32_t num_systems, const rewl
         if (d.model != REWL_ISING2D && d.model != REWL_ASHKIN_TELLER2D) return fail(REWL_ERR_INVALID, "unknown model");
         if (d.L < 2 || d.L > 64) return fail(REWL_ERR_INVALID, "L must lie in [2, 64]");
         DevSystem& D = ds[(size_t)si];
-        D.model = d.model; D.L = d.L; D.N = d.L * d.L; D.ndof = d.model == REWL_ISING2D ? D.N : 2 * D.N;
+        D.model = d.model; D.L = d.L; D.N = d.L * d.L; D.ndof = model_info(d.model).planes * D.N;
         D.has_field = d.field != nullptr || d.h != 0.0;
         D.J = d.J; D.K = d.K; D.h = d.h;
         D.k0 = (uint32_t)d.seed; D.k1 = (uint32_t)(d.seed >> 32);
@@ -1599,7 +1601,7 @@ int rewl_anneal(const rewl_system_desc* systems, int32_t num_systems, const rewl
         int8_t* o = final_spins;
         for (int si = 0; si < num_systems; ++si) {
             const DevSystem& D = ds[(size_t)si];
-            const int planes = D.model == REWL_ISING2D ? 1 : 2;
+            const int planes = model_info(D.model).planes;
             for (int ch = 0; ch < P.chains; ++ch) {
                 const uint64_t* g = lat.data() + ((size_t)si * gps + (size_t)(ch / kLanes)) * kAnnealGroupWords;
                 const int lane = ch % kLanes;

@@ -167,11 +167,11 @@ __device__ void sweep_walker(const KArgs& a, const DevGroup& G, const DevSystem&
     int mag[3] = {0, 0, 0};
     double* obs = OBS ? a.obs + hb * a.nobs : nullptr;
     const int64_t obs_bin = (int64_t)kLanes * a.nobs;
-    if (OBS) lattice_magnetisations(lat, S.L, MODEL, mag);
+    if (OBS) lattice_magnetisations<MODEL>(lat, S.L, mag);
     // The moments of consecutive updates in one bin are summed in registers
     // and added to the bin's sums when the walker changes bin (and at the end
     // of the launch): one atomic per moment per run instead of per update.
-    constexpr int NV = MODEL == 0 ? 3 : 12;
+    constexpr int NV = Model<MODEL>::kValues;
     constexpr int GK = OBS == 2 ? 4 : 3;   // slot stride of the (|x|, x^2, x^4[, G]) groups
     double acc[NV];
 #pragma unroll
@@ -193,18 +193,18 @@ __device__ void sweep_walker(const KArgs& a, const DevGroup& G, const DevSystem&
             for (int i = 0; i < NV; ++i) acc[i] = 0.;
         }
         double v[NV];
-        observable_values<MODEL>(S.N, mag, v);
+        Model<MODEL>::values(S.N, mag, v);
 #pragma unroll
         for (int i = 0; i < NV; ++i) acc[i] += v[i];
-        if (MODEL == 1 && a.dens) update_density(a.dens, hb + (int64_t)b * kLanes, S.N, mag);
+        if (Model<MODEL>::kDensities && a.dens) update_density(a.dens, hb + (int64_t)b * kLanes, S.N, mag);
         if (OBS == 2) {
             const uint32_t t = s_tot[b * ls + lane];
             s_tot[b * ls + lane] = t + 1u;
-            if (t % 1000u == 0u) add_correlators(obs + b * obs_bin, lat, L, MODEL, y, rows, s_cs);
+            if (t % 1000u == 0u) add_correlators<MODEL>(obs + b * obs_bin, lat, L, y, rows, s_cs);
         }
     };
-    constexpr int NP = MODEL == 0 ? 1 : 2;   // bit planes
-    constexpr int NPASS = MODEL == 0 ? 1 : 2;
+    constexpr int NP = Model<MODEL>::kPlanes;
+    constexpr int NPASS = Model<MODEL>::kPasses;
 
     for (int s = 0; s < nsweeps; ++s) {
         const uint64_t blk0 = (sweep_base + (uint64_t)s) * B;
@@ -229,85 +229,41 @@ __device__ void sweep_walker(const KArgs& a, const DevGroup& G, const DevSystem&
                     }
                     const int xl = (x == 0) ? L - 1 : x - 1;
                     const int xr = (x == L - 1) ? 0 : x + 1;
-                    if (MODEL == 0) {
-                        const uint32_t si = bit(cu[0], x);
-                        if (INTP) {
-                            const uint32_t na = (si ^ bit(up[0], x)) + (si ^ bit(cu[0], xl)) + (si ^ bit(cu[0], xr)) +
-                                                (si ^ bit(dn[0], x));
-                            const int En = Ei + S.twoJ * (4 - 2 * (int)na);
-                            if (wl_accept_int<REPLAY, PACKED>(En, Ei, cb, lc, lnf, Wn.imin, Wn.phi_lo, pspan, Wn.shift,
-                                                             s_lng, s_cnt, lane, ls, ring, cap, filled, cursor, a.err, rnd, p)) {
-                                cu[0] ^= (1ull << x);
-                                if (OBS) mag[0] += si ? 2 : -2;
-                            }
-                            if (OBS) observe(cb, y, cu);
-                        } else {
-                            const int site = y * L + x;
-                            const double sv = pm(si);
-                            double f = field_at(S, site);
-                            f += S.J * pm(bit(up[0], x));
-                            f += S.J * pm(bit(cu[0], xl));
-                            f += S.J * pm(bit(cu[0], xr));
-                            f += S.J * pm(bit(dn[0], x));
-                            const double t = -sv;
-                            const double En = Ed + -1. * t * f - -1. * sv * f;
-                            if (wl_accept_fp<REPLAY, PACKED>(En, Ed, cb, lc, lnf, Wn.emin, Wn.bs, Wn.phi_lo, Wn.phi_hi,
-                                                            Wn.nbins - 1, s_lng, s_cnt, lane, ls, ring, cap, filled, cursor, a.err, rnd, p)) {
-                                cu[0] ^= (1ull << x);
-                                if (OBS) mag[0] += si ? 2 : -2;
-                            }
-                            if (OBS) observe(cb, y, cu);
-                        }
-                    } else {
-                        const uint32_t sb = bit(cu[0], x), tb = bit(cu[1], x);
-                        if (INTP) {
-                            const uint32_t ms = (sb ^ bit(up[0], x)) | ((sb ^ bit(cu[0], xl)) << 1) |
-                                                ((sb ^ bit(cu[0], xr)) << 2) | ((sb ^ bit(dn[0], x)) << 3);
-                            const uint32_t mt = (tb ^ bit(up[1], x)) | ((tb ^ bit(cu[1], xl)) << 1) |
-                                                ((tb ^ bit(cu[1], xr)) << 2) | ((tb ^ bit(dn[1], x)) << 3);
-                            const int pf = __popc(fl ? mt : ms);
-                            const int qq = __popc(ms ^ mt);
-                            const int En = Ei + S.twoJ * (4 - 2 * pf) + S.twoK * (4 - 2 * qq);
-                            if (wl_accept_int<REPLAY, PACKED>(En, Ei, cb, lc, lnf, Wn.imin, Wn.phi_lo, pspan, Wn.shift,
-                                                             s_lng, s_cnt, lane, ls, ring, cap, filled, cursor, a.err, rnd, p)) {
-                                if (fl) cu[1] ^= (1ull << x); else cu[0] ^= (1ull << x);
-                                if (OBS) {
-                                    mag[fl] += (fl ? tb : sb) ? 2 : -2;
-                                    mag[2] += (sb ^ tb) ? 2 : -2;
-                                }
-                            }
-                            if (OBS) observe(cb, y, cu);
-                        } else {
-                            const int site = y * L + x;
-                            const double sg = pm(sb), ta = pm(tb);
-                            double ts = sg, tt = ta;
-                            ts += -2. * ts * (double)(fl == 0);
-                            tt += -2. * tt * (double)(fl == 1);
-                            const double hfield = S.has_field ? field_at(S, site) : 0.0;
-                            const double ns[4] = {pm(bit(up[0], x)), pm(bit(cu[0], xl)), pm(bit(cu[0], xr)), pm(bit(dn[0], x))};
-                            const double nt[4] = {pm(bit(up[1], x)), pm(bit(cu[1], xl)), pm(bit(cu[1], xr)), pm(bit(dn[1], x))};
-                            double en_new = 0., en_old = 0.;
-                            if (S.has_field) { en_new += hfield * (ts * tt); en_old += hfield * (sg * ta); }
+                    // the bits (1 = -1) of the site and of its neighbours in the
+                    // reference order, every plane
+                    uint32_t ob[NP], nbb[NP][4];
 #pragma unroll
-                            for (int k = 0; k < 4; ++k) {
-                                en_new += S.J * ts * ns[k];
-                                en_new += S.J * tt * nt[k];
-                                en_new += S.K * ts * tt * ns[k] * nt[k];
-                                en_old += S.J * sg * ns[k];
-                                en_old += S.J * ta * nt[k];
-                                en_old += S.K * sg * ta * ns[k] * nt[k];
-                            }
-                            const double En = Ed + -en_new - -en_old;
-                            if (wl_accept_fp<REPLAY, PACKED>(En, Ed, cb, lc, lnf, Wn.emin, Wn.bs, Wn.phi_lo, Wn.phi_hi,
-                                                            Wn.nbins - 1, s_lng, s_cnt, lane, ls, ring, cap, filled, cursor, a.err, rnd, p)) {
-                                if (fl) cu[1] ^= (1ull << x); else cu[0] ^= (1ull << x);
-                                if (OBS) {
-                                    mag[fl] += (fl ? tb : sb) ? 2 : -2;
-                                    mag[2] += (sb ^ tb) ? 2 : -2;
-                                }
-                            }
-                            if (OBS) observe(cb, y, cu);
+                    for (int q = 0; q < NP; ++q) {
+                        ob[q] = bit(cu[q], x);
+                        nbb[q][0] = bit(up[q], x);
+                        nbb[q][1] = bit(cu[q], xl);
+                        nbb[q][2] = bit(cu[q], xr);
+                        nbb[q][3] = bit(dn[q], x);
+                    }
+                    const int P = Model<MODEL>::flip_plane(fl);
+                    bool acc;
+                    if (INTP) {
+                        const int En = Ei + Model<MODEL>::int_energy_change(S.twoJ, S.twoK, ob, nbb, fl);
+                        acc = wl_accept_int<REPLAY, PACKED>(En, Ei, cb, lc, lnf, Wn.imin, Wn.phi_lo, pspan, Wn.shift,
+                                                           s_lng, s_cnt, lane, ls, ring, cap, filled, cursor, a.err, rnd, p);
+                    } else {
+                        double own[NP], nb[NP][4];
+#pragma unroll
+                        for (int q = 0; q < NP; ++q) {
+                            own[q] = pm(ob[q]);
+#pragma unroll
+                            for (int k = 0; k < 4; ++k) nb[q][k] = pm(nbb[q][k]);
                         }
+                        const double En =
+                            Model<MODEL>::proposal_energy(S.J, S.K, S.has_field, field_at(S, y * L + x), Ed, own, nb, fl);
+                        acc = wl_accept_fp<REPLAY, PACKED>(En, Ed, cb, lc, lnf, Wn.emin, Wn.bs, Wn.phi_lo, Wn.phi_hi,
+                                                          Wn.nbins - 1, s_lng, s_cnt, lane, ls, ring, cap, filled, cursor,
+                                                          a.err, rnd, p);
+                    }
+                    if (acc) plane_set(cu, P, plane_pick(cu, P) ^ (1ull << x));
+                    if (OBS) {
+                        Model<MODEL>::magnetise(mag, acc, (int)plane_pick(ob, P), (int)(ob[0] ^ ob[NP - 1]), fl);
+                        observe(cb, y, cu);
                     }
                 }
 #pragma unroll
@@ -319,7 +275,7 @@ __device__ void sweep_walker(const KArgs& a, const DevGroup& G, const DevSystem&
                     dn[q] = (y + 2 < L) ? nx[q] : r0[q];
                 }
             }
-            if (MODEL == 1) fl ^= 1;   // flavour switches after site N-1
+            fl = Model<MODEL>::next_flavour(fl);   // flavour switches after site N-1
         }
         if (PACKED && ((s + 1) % flush_every) == 0) flush_counts<true>(s_cnt, a.cnt, hb, nbins, lane, ls);
     }

@@ -114,8 +114,9 @@ __device__ void sweep_walker_fast(const KArgs& a, const DevGroup& G, const DevSy
     double lc = active ? *(const double*)((const char*)s_lng + cbo) : 0.0;
     int buf = 0;
     const uint64_t lmask = (L == 64) ? ~0ull : ((1ull << L) - 1ull);
-    constexpr int NP = MODEL == 0 ? 1 : 2;
-    constexpr int NPASS = MODEL == 0 ? 1 : 2;
+    constexpr int NP = Model<MODEL>::kPlanes;
+    constexpr int NPASS = Model<MODEL>::kPasses;
+    constexpr bool KQ = Model<MODEL>::kProductCoupling;
 
     for (int s = 0; s < nsweeps; ++s) {
         const uint64_t blk0 = (sweep_base + (uint64_t)s) * B;
@@ -134,18 +135,15 @@ __device__ void sweep_walker_fast(const KArgs& a, const DevGroup& G, const DevSy
 #pragma unroll
                 for (int q = 0; q < NP; ++q) nx[q] = (y + 2 < L) ? lat[(int64_t)(q * L + y + 2) * kLanes] : 0ull;
                 // ---- per-row static disagreement counts --------------------------
-                // P: the plane being flipped (Ising: the only plane), Q: the other.
-                uint64_t P = cu[0], Pu = up[0], Pd = dn[0];
-                uint64_t Q = 0, Qu = 0, Qd = 0;
-                if (MODEL == 1) {
-                    const bool t = fl != 0;
-                    P = t ? cu[1] : cu[0]; Pu = t ? up[1] : up[0]; Pd = t ? dn[1] : dn[0];
-                    Q = t ? cu[0] : cu[1]; Qu = t ? up[0] : up[1]; Qd = t ? dn[0] : dn[1];
-                }
+                // P: the plane being flipped, Q: the other one (a one-plane
+                // model has Q = P, unused)
+                const int pf = Model<MODEL>::flip_plane(fl), qf = NP - 1 - pf;
+                const uint64_t P = plane_pick(cu, pf), Pu = plane_pick(up, pf), Pd = plane_pick(dn, pf);
+                const uint64_t Q = plane_pick(cu, qf), Qu = plane_pick(up, qf), Qd = plane_pick(dn, qf);
                 const uint64_t pu = P ^ Pu, pd = P ^ Pd, pr = (P ^ rotr_row(P, L)) & lmask;
                 const uint64_t c0 = pu ^ pd ^ pr, c1 = (pu & pd) | (pr & (pu ^ pd));
                 uint64_t m0 = 0, m1 = 0, ql = 0;
-                if (MODEL == 1) {
+                if constexpr (KQ) {
                     const uint64_t qu = Q ^ Qu, qd = Q ^ Qd, qr = (Q ^ rotr_row(Q, L)) & lmask;
                     const uint64_t xu = pu ^ qu, xd = pd ^ qd, xr = pr ^ qr;
                     m0 = xu ^ xd ^ xr;
@@ -176,7 +174,7 @@ __device__ void sweep_walker_fast(const KArgs& a, const DevGroup& G, const DevSy
                     const uint32_t dl = sP ^ sl;
                     int cP = (int)((uint32_t)(c0 >> x) & 1u) + 2 * (int)((uint32_t)(c1 >> x) & 1u);
                     int En;
-                    if (MODEL == 0) {
+                    if constexpr (!KQ) {
                         if (x == L - 1) {
                             // right neighbour is site 0 of this row: undo the static guess if it flipped
                             const int f0 = (int)(flips & 1ull), dr = (int)((uint32_t)(pr >> x) & 1u);
@@ -203,12 +201,7 @@ __device__ void sweep_walker_fast(const KArgs& a, const DevGroup& G, const DevSy
                     flips |= (uint64_t)acc << x;
                     sl = sP ^ (uint32_t)acc;
                 }
-                const uint64_t Pn = P ^ flips;
-                if (MODEL == 0) {
-                    cu[0] = Pn;
-                } else {
-                    if (fl) cu[1] = Pn; else cu[0] = Pn;
-                }
+                plane_set(cu, pf, P ^ flips);
 #pragma unroll
                 for (int q = 0; q < NP; ++q) {
                     lat[(int64_t)(q * L + y) * kLanes] = cu[q];
@@ -223,7 +216,7 @@ __device__ void sweep_walker_fast(const KArgs& a, const DevGroup& G, const DevSy
                   buf ^= 1;
               }
             }
-            if (MODEL == 1) fl ^= 1;
+            fl = Model<MODEL>::next_flavour(fl);
         }
     }
 
@@ -244,7 +237,7 @@ __device__ void sweep_walker_fast(const KArgs& a, const DevGroup& G, const DevSy
 template <int MODEL>
 __device__ void fill_row(const DevGroup& G, const DevSystem& S, int wave, int nwaves, int lane, uint64_t sweep_base,
                          int row, float* ring_buf) {
-    constexpr int NPASS = MODEL == 0 ? 1 : 2;
+    constexpr int NPASS = Model<MODEL>::kPasses;
     const int L = S.L, stride = G.nwalkers;
     if (lane >= stride) return;
     // walkers of a group are consecutive local ids of one system
@@ -280,7 +273,7 @@ __device__ void fill_row(const DevGroup& G, const DevSystem& S, int wave, int nw
 template <int MODEL>
 __device__ void produce_thresholds(const DevGroup& G, const DevSystem& S, int wave, int nwaves, int lane, int nsweeps,
                                    uint64_t sweep_base, float* ring) {
-    constexpr int NPASS = MODEL == 0 ? 1 : 2;
+    constexpr int NPASS = Model<MODEL>::kPasses;
     const int L = S.L;
     const int rows = nsweeps * NPASS * L;
     const size_t bufsz = (size_t)L * G.nwalkers;

@@ -27,19 +27,13 @@ namespace rewl {
 
 __device__ __forceinline__ double pm_bit(uint64_t w, int x) { return ((w >> x) & 1ull) ? -1.0 : 1.0; }
 
-// The reference's sums add products c * s1 * ... of a coupling (or field) c
-// and spins s = +-1 one by one: en += c s1 s2.  The product is exact, so
-// en = fma(c s1, s2, en) rounds once exactly where en + (c s1 s2) does: one
-// FMA per term in place of two multiplies and an add.
-__device__ __forceinline__ double add_term(double en, double cs, double s) { return fma(cs, s, en); }
-
 // Bounds for every attempt position of one row of one sweep pass, for every
 // walker of the group (producer lane j serves walker j); the producer
 // wavefronts split the row's Philox blocks.
 template <int MODEL>
 __device__ void field_fill_row(const DevGroup& G, const DevSystem& S, int wave, int nwaves, int lane,
                                uint64_t sweep_base, int row, double* ring_buf, const BoundEntry* tab) {
-    constexpr int NPASS = MODEL == 0 ? 1 : 2;
+    constexpr int NPASS = Model<MODEL>::kPasses;
     const int L = S.L, stride = G.nwalkers;
     if (lane >= stride) return;
     const uint32_t walker = (uint32_t)(G.first_walker - S.first_walker + lane);
@@ -69,7 +63,7 @@ __device__ void field_fill_row(const DevGroup& G, const DevSystem& S, int wave, 
 template <int MODEL>
 __device__ void field_produce(const DevGroup& G, const DevSystem& S, int wave, int nwaves, int lane, int nsweeps,
                               uint64_t sweep_base, double* ring, const BoundEntry* tab) {
-    constexpr int NPASS = MODEL == 0 ? 1 : 2;
+    constexpr int NPASS = Model<MODEL>::kPasses;
     const int rows = nsweeps * NPASS * S.L;
     const size_t bufsz = (size_t)S.L * G.nwalkers;
     field_fill_row<MODEL>(G, S, wave, nwaves, lane, sweep_base, 0, ring, tab);
@@ -111,11 +105,11 @@ __device__ void sweep_walker_field(const KArgs& a, const DevGroup& G, const DevS
     double lc = active ? s_lng[cb * stride + lane] : 0.0;
     const int plo = Wn.phi_lo, phi = Wn.phi_hi, nbmax = Wn.nbins - 1;
     const double emin = Wn.emin, bs = Wn.bs;
-    constexpr int NP = MODEL == 0 ? 1 : 2;
-    constexpr int NPASS = MODEL == 0 ? 1 : 2;
+    constexpr int NP = Model<MODEL>::kPlanes;
+    constexpr int NPASS = Model<MODEL>::kPasses;
     int buf = 0;
 
-    constexpr int NV = MODEL == 0 ? 3 : 12;
+    constexpr int NV = Model<MODEL>::kValues;
     int mag[3] = {0, 0, 0};
     double accv[NV];
 #pragma unroll
@@ -127,7 +121,7 @@ __device__ void sweep_walker_field(const KArgs& a, const DevGroup& G, const DevS
 #pragma unroll
         for (int i = 0; i < NV; ++i) obs_add(p + i, accv[i]);   // observables = 1: slot i
     };
-    if (OBS && active) lattice_magnetisations(lat, L, MODEL, mag);
+    if (OBS && active) lattice_magnetisations<MODEL>(lat, L, mag);
 
     for (int s = 0; s < nsweeps; ++s) {
         for (int pass = 0; pass < NPASS; ++pass) {
@@ -150,42 +144,19 @@ __device__ void sweep_walker_field(const KArgs& a, const DevGroup& G, const DevS
                     const double hfield = frow[x];
                     const int xl = (x == 0) ? L - 1 : x - 1;
                     const int xr = (x == L - 1) ? 0 : x + 1;
-                    double En;
-                    if (MODEL == 0) {
-                        // f = h + J s_up + J s_left + J s_right + J s_down, then
-                        // E' = E + (-1 t) f - (-1 s) f with t = -s (reference order)
-                        const double sv = pm_bit(cu[0], x);
-                        double f = hfield;
-                        f = add_term(f, S.J, pm_bit(up[0], x));
-                        f = add_term(f, S.J, pm_bit(cu[0], xl));
-                        f = add_term(f, S.J, pm_bit(cu[0], xr));
-                        f = add_term(f, S.J, pm_bit(dn[0], x));
-                        const double t = -sv;
-                        En = Ed + -1. * t * f - -1. * sv * f;
-                    } else {
-                        // en = h (s t) + sum_k [J s s_k + J t t_k + K s t s_k t_k] for the
-                        // proposed (ts, tt) and the current (sg, ta) spins, then
-                        // E' = E + -en_new - -en_old (reference order)
-                        const double sg = pm_bit(cu[0], x), ta = pm_bit(cu[1], x);
-                        const double ts = fl == 0 ? -sg : sg, tt = fl == 1 ? -ta : ta;
-                        const double ns[4] = {pm_bit(up[0], x), pm_bit(cu[0], xl), pm_bit(cu[0], xr), pm_bit(dn[0], x)};
-                        const double nt[4] = {pm_bit(up[1], x), pm_bit(cu[1], xl), pm_bit(cu[1], xr), pm_bit(dn[1], x)};
-                        const double Jts = S.J * ts, Jtt = S.J * tt, Ktt = S.K * ts * tt;
-                        const double Jsg = S.J * sg, Jta = S.J * ta, Kst = S.K * sg * ta;
-                        double en_new = 0., en_old = 0.;
-                        if (S.has_field) { en_new += hfield * (ts * tt); en_old += hfield * (sg * ta); }
+                    // the spins of the site and of its neighbours in the
+                    // reference order, every plane; E' by the model's
+                    // change_state in the reference's order
+                    double own[NP], nbs[NP][4];
 #pragma unroll
-                        for (int k = 0; k < 4; ++k) {
-                            const double nn = ns[k] * nt[k];
-                            en_new = add_term(en_new, Jts, ns[k]);
-                            en_new = add_term(en_new, Jtt, nt[k]);
-                            en_new = add_term(en_new, Ktt, nn);
-                            en_old = add_term(en_old, Jsg, ns[k]);
-                            en_old = add_term(en_old, Jta, nt[k]);
-                            en_old = add_term(en_old, Kst, nn);
-                        }
-                        En = Ed + -en_new - -en_old;
+                    for (int q = 0; q < NP; ++q) {
+                        own[q] = pm_bit(cu[q], x);
+                        nbs[q][0] = pm_bit(up[q], x);
+                        nbs[q][1] = pm_bit(cu[q], xl);
+                        nbs[q][2] = pm_bit(cu[q], xr);
+                        nbs[q][3] = pm_bit(dn[q], x);
                     }
+                    const double En = Model<MODEL>::proposal_energy(S.J, S.K, S.has_field, hfield, Ed, own, nbs, fl);
                     // histogram_index.hpp:8-31, the bin clamped to the window
                     const int pe = trunc_half(En);
                     const bool in = pe >= plo && pe < phi;
@@ -197,17 +168,11 @@ __device__ void sweep_walker_field(const KArgs& a, const DevGroup& G, const DevS
                     s_lng[cb * stride + lane] = lc;
                     atomicAdd(&s_cnt[cb * stride + lane], 1u);
                     if constexpr (OBS) {
-                        // the flipped spin's bit before the flip (1 = -1), in its plane
-                        const uint64_t pw = (MODEL == 1 && fl) ? cu[NP - 1] : cu[0];
-                        const int pb = (int)((pw >> x) & 1ull);
-                        const int d = acc ? 4 * pb - 2 : 0;
-                        if (MODEL == 0) {
-                            mag[0] += d;
-                        } else {
-                            mag[0] += fl ? 0 : d;
-                            mag[1] += fl ? d : 0;
-                            mag[2] += acc ? 4 * (int)(((cu[0] ^ cu[NP - 1]) >> x) & 1ull) - 2 : 0;
-                        }
+                        // the flipped spin's bit before the flip (1 = -1), in its
+                        // plane, and its Baxter bit
+                        const int pb = (int)((plane_pick(cu, Model<MODEL>::flip_plane(fl)) >> x) & 1ull);
+                        const int xb = (int)(((cu[0] ^ cu[NP - 1]) >> x) & 1ull);
+                        Model<MODEL>::magnetise(mag, acc, pb, xb, fl);
                         if (cb != accb) {
                             flush_obs();
                             accb = cb;
@@ -215,13 +180,13 @@ __device__ void sweep_walker_field(const KArgs& a, const DevGroup& G, const DevS
                             for (int i = 0; i < NV; ++i) accv[i] = 0.;
                         }
                         double v[NV];
-                        observable_values<MODEL>(S.N, mag, v);
+                        Model<MODEL>::values(S.N, mag, v);
 #pragma unroll
                         for (int i = 0; i < NV; ++i) accv[i] += v[i];
                     }
                     if (acc) {
-                        if (MODEL == 1 && fl) cu[1] ^= (1ull << x);
-                        else cu[0] ^= (1ull << x);
+                        const int P = Model<MODEL>::flip_plane(fl);
+                        plane_set(cu, P, plane_pick(cu, P) ^ (1ull << x));
                     }
                 }
 #pragma unroll
@@ -236,7 +201,7 @@ __device__ void sweep_walker_field(const KArgs& a, const DevGroup& G, const DevS
               __syncthreads();   // this row's ring buffer consumed, the next one filled
               buf ^= 1;
             }
-            if (MODEL == 1) fl ^= 1;   // flavour switches after site N-1
+            fl = Model<MODEL>::next_flavour(fl);   // flavour switches after site N-1
         }
     }
 

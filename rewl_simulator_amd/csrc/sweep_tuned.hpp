@@ -171,8 +171,10 @@ __device__ __forceinline__ void sweep_walker_tuned(const KArgs& a, const DevGrou
     static_assert(!OBS || SPEC == 1, "observables are tracked by the SPEC 1 form");
     constexpr int NW = (L + 31) / 32;    // 32-bit words per row
     constexpr int NW2 = (L + 15) / 16;   // words of 2-bit fields per row
-    constexpr int NP = MODEL == 0 ? 1 : 2;
-    constexpr int NPASS = MODEL == 0 ? 1 : 2;
+    using Mdl = Model<MODEL>;
+    constexpr int NP = Mdl::kPlanes;
+    constexpr int NPASS = Mdl::kPasses;
+    constexpr bool KQ = Mdl::kProductCoupling;
     const bool active = lane < G.nwalkers;
     const int g = G.first_walker + (active ? lane : 0);
     const int wi = a.w_window[g];
@@ -219,7 +221,7 @@ __device__ __forceinline__ void sweep_walker_tuned(const KArgs& a, const DevGrou
     const int m4J = -2 * S.twoJ, m4K = -2 * S.twoK;
     int buf = 0;
 
-    constexpr int NV = MODEL == 0 ? 3 : 12;
+    constexpr int NV = Mdl::kValues;
     constexpr int GK = OBS == 2 ? 4 : 3;   // slot stride of the (|x|, x^2, x^4[, G]) groups
     int mag[3] = {0, 0, 0};
     double accv[NV];
@@ -233,14 +235,16 @@ __device__ __forceinline__ void sweep_walker_tuned(const KArgs& a, const DevGrou
 #pragma unroll
         for (int i = 0; i < NV; ++i) obs_add(p + (i / 3) * GK + i % 3, accv[i]);
     };
-    // correlator: Fourier sums of the sigma plane and (AT) of the Baxter plane
+    // correlator: Fourier sums of plane 0 and, for a model with a Baxter
+    // correlator, of the product of the first and last planes
     double fs[4] = {0., 0., 0., 0.}, fb[4] = {0., 0., 0., 0.};
     if (OBS == 2 && active) {
         const uint64_t* lt = a.lat + G.lat_base + lane;
         auto csf = [&](int k) { return lds_f64(cs_tab + 8u * (uint32_t)k); };
         fourier_parts(L, [&](int r) { return lt[(int64_t)r * kLanes]; }, csf, fs);
-        if (MODEL == 1)
-            fourier_parts(L, [&](int r) { return lt[(int64_t)r * kLanes] ^ lt[(int64_t)(L + r) * kLanes]; }, csf, fb);
+        if (Mdl::kBaxterCorrelator)
+            fourier_parts(L, [&](int r) { return lt[(int64_t)r * kLanes] ^ lt[(int64_t)((NP - 1) * L + r) * kLanes]; },
+                          csf, fb);
     }
     auto observe = [&]() {
         if (cbo != acck) {
@@ -250,11 +254,11 @@ __device__ __forceinline__ void sweep_walker_tuned(const KArgs& a, const DevGrou
             for (int i = 0; i < NV; ++i) accv[i] = 0.;
         }
         double v[NV];
-        observable_values<MODEL>(L * L, mag, v);
+        Mdl::values(L * L, mag, v);
 #pragma unroll
         for (int i = 0; i < NV; ++i) accv[i] += v[i];
     };
-    if (OBS && active) lattice_magnetisations(a.lat + G.lat_base + lane, L, MODEL, mag);
+    if (OBS && active) lattice_magnetisations<MODEL>(a.lat + G.lat_base + lane, L, mag);
 
     for (int s = 0; s < nsweeps; ++s) {
         for (int pass = 0; pass < NPASS; ++pass) {
@@ -270,41 +274,29 @@ __device__ __forceinline__ void sweep_walker_tuned(const KArgs& a, const DevGrou
                 if (active) {
 #pragma unroll
                     for (int q = 0; q < NP; ++q) nx[q] = (y + 2 < L) ? lat[(int64_t)(q * L + y + 2) * kLanes] : 0ull;
-                    uint64_t P = cu[0], Pu = up[0], Pd = dn[0];
-                    uint64_t Q = 0, Qu = 0, Qd = 0;
-                    if (MODEL == 1) {
-                        const bool t = fl != 0;
-                        P = t ? cu[1] : cu[0]; Pu = t ? up[1] : up[0]; Pd = t ? dn[1] : dn[0];
-                        Q = t ? cu[0] : cu[1]; Qu = t ? up[0] : up[1]; Qd = t ? dn[0] : dn[1];
-                    }
-                    // ---- per-row static bit slices ---------------------------------
-                    const uint64_t pu = P ^ Pu, pd = P ^ Pd, pr = P ^ rotr1<L>(P);
-                    const uint64_t c0 = pu ^ pd ^ pr, c1n = ~((pu & pd) | (pr & (pu ^ pd)));
-                    const uint64_t pdl = P ^ rotl1<L>(P);
-                    uint64_t m0 = 0, m1n = 0, ql = 0, xr = 0;
-                    if (MODEL == 1) {
-                        const uint64_t qu = Q ^ Qu, qd = Q ^ Qd, qr = Q ^ rotr1<L>(Q);
-                        const uint64_t xu = pu ^ qu, xd = pd ^ qd;
-                        xr = pr ^ qr;
-                        m0 = xu ^ xd ^ xr;
-                        m1n = ~((xu & xd) | (xr & (xu ^ xd)));
-                        ql = Q ^ rotl1<L>(Q);
-                    }
+                    // P: the plane this pass flips, Q: the other one (the
+                    // product plane's partner); a one-plane model has Q = P
+                    const int pf = Mdl::flip_plane(fl), qf = NP - 1 - pf;
+                    const uint64_t P = plane_pick(cu, pf), Pu = plane_pick(up, pf), Pd = plane_pick(dn, pf);
+                    const uint64_t Q = plane_pick(cu, qf), Qu = plane_pick(up, qf), Qd = plane_pick(dn, qf);
+                    // ---- per-row static bit slices (the model's) --------------------
+                    const auto sl = Mdl::template slices<L>(P, Pu, Pd, Q, Qu, Qd, rotr1<L>(P), rotl1<L>(P),
+                                                            rotr1<L>(Q), rotl1<L>(Q));
                     // signed 2-bit counts, site x at bits 2 (x % 16) of word x / 16:
                     // (c0, c1n) interleaved is c0 + 2 c1 - 2 in two's complement
                     uint32_t w_cp[NW2], w_cq[NW2], w_pdl[NW], w_ql[NW], frev[NW];
 #pragma unroll
                     for (int w = 0; w < NW2; ++w) {
-                        w_cp[w] = spread16((uint32_t)(c0 >> (16 * w)) & 0xffffu) |
-                                  (spread16((uint32_t)(c1n >> (16 * w)) & 0xffffu) << 1);
-                        if (MODEL == 1)
-                            w_cq[w] = spread16((uint32_t)(m0 >> (16 * w)) & 0xffffu) |
-                                      (spread16((uint32_t)(m1n >> (16 * w)) & 0xffffu) << 1);
+                        w_cp[w] = spread16((uint32_t)(sl.c0 >> (16 * w)) & 0xffffu) |
+                                  (spread16((uint32_t)(sl.c1n >> (16 * w)) & 0xffffu) << 1);
+                        if constexpr (KQ)
+                            w_cq[w] = spread16((uint32_t)(sl.m0 >> (16 * w)) & 0xffffu) |
+                                      (spread16((uint32_t)(sl.m1n >> (16 * w)) & 0xffffu) << 1);
                     }
 #pragma unroll
                     for (int h = 0; h < NW; ++h) {
-                        w_pdl[h] = (uint32_t)(pdl >> (32 * h));
-                        w_ql[h] = (uint32_t)(ql >> (32 * h));
+                        w_pdl[h] = (uint32_t)(sl.pdl >> (32 * h));
+                        if constexpr (KQ) w_ql[h] = (uint32_t)(sl.ql >> (32 * h));
                         frev[h] = 0u;
                     }
                     const uint32_t rrow = ring_lane + (uint32_t)(buf * 8 * L);
@@ -318,20 +310,20 @@ __device__ __forceinline__ void sweep_walker_tuned(const KArgs& a, const DevGrou
                         // Delta E = -4J (cP' + dl) [ - 4K (cQ' + (dl ^ ql)) ]
                         int kp = (int)dl + __builtin_amdgcn_sbfe((int)w_cp[x >> 4], 2 * (x & 15), 2);
                         int kq = 0;
-                        if (MODEL == 1)
+                        if constexpr (KQ)
                             kq = (int)(dl ^ bit_of(w_ql[h], b)) + __builtin_amdgcn_sbfe((int)w_cq[x >> 4], 2 * (x & 15), 2);
                         if (x == L - 1) {
                             // the right neighbour is site 0 of this row: its static
                             // disagreement toggles if it flipped
-                            const int prl = (int)bit_of((uint32_t)(pr >> (32 * h)), b);
+                            const int prl = (int)bit_of((uint32_t)(sl.pr >> (32 * h)), b);
                             kp += f0 * (1 - 2 * prl);
-                            if (MODEL == 1) {
-                                const int xrl = (int)bit_of((uint32_t)(xr >> (32 * h)), b);
+                            if constexpr (KQ) {
+                                const int xrl = (int)bit_of((uint32_t)(sl.xr >> (32 * h)), b);
                                 kq += f0 * (1 - 2 * xrl);
                             }
                         }
                         int on = __mul24(m4J, kp) + from;
-                        if (MODEL == 1) on += __mul24(m4K, kq);
+                        if constexpr (KQ) on += __mul24(m4K, kq);
                         return on;
                     };
                     if (SPEC == 2) {
@@ -432,16 +424,9 @@ __device__ __forceinline__ void sweep_walker_tuned(const KArgs& a, const DevGrou
                             on = ac ? on1 : on0;
                             nbo = ac ? nbo1 : nbo0;
                             if constexpr (OBS) {
-                                // the flipped spin's bit in its plane (1 = -1 before the flip)
-                                const int pb = (int)((P >> x) & 1ull);
-                                const int d = ac ? 4 * pb - 2 : 0;
-                                if (MODEL == 0) {
-                                    mag[0] += d;
-                                } else {
-                                    mag[0] += fl ? 0 : d;
-                                    mag[1] += fl ? d : 0;
-                                    mag[2] += ac ? 4 * (int)(((P ^ Q) >> x) & 1ull) - 2 : 0;
-                                }
+                                // the flipped spin's bit in its plane and its product
+                                // bit (1 = -1 before the flip)
+                                Mdl::magnetise(mag, ac, (int)((P >> x) & 1ull), (int)(((P ^ Q) >> x) & 1ull), fl);
                                 observe();
                             }
                             if constexpr (OBS == 2) {
@@ -451,27 +436,21 @@ __device__ __forceinline__ void sweep_walker_tuned(const KArgs& a, const DevGrou
                                 if (t % 1000u == 0u) {
                                     // this row's flips so far (sites 0..x), then the sums of the
                                     // current configuration
-                                    uint64_t pf = 0;
+                                    uint64_t fx = 0;
 #pragma unroll
                                     for (int hh = 0; hh <= (x >> 5); ++hh) {
                                         const int nh = hh < (x >> 5) ? 32 : (x & 31) + 1;
-                                        pf |= (uint64_t)(__builtin_bitreverse32(frev[hh]) >> (32 - nh)) << (32 * hh);
+                                        fx |= (uint64_t)(__builtin_bitreverse32(frev[hh]) >> (32 - nh)) << (32 * hh);
                                     }
                                     double gs[4] = {fs[0], fs[1], fs[2], fs[3]};
                                     double gb[4] = {fb[0], fb[1], fb[2], fb[3]};
-                                    if (MODEL == 0 || fl == 0) fourier_flip<L>(gs, P, pf, y, cs_tab);
-                                    if (MODEL == 1) fourier_flip<L>(gb, P ^ Q, pf, y, cs_tab);
+                                    if (pf == 0) fourier_flip<L>(gs, P, fx, y, cs_tab);
+                                    if (Mdl::kBaxterCorrelator) fourier_flip<L>(gb, P ^ Q, fx, y, cs_tab);
                                     const int b = (int)((cbo - zero_off) / (uint32_t)stride16);
                                     double* p = a.obs + (hb + (int64_t)b * kLanes) * a.nobs;
-                                    const double g = fourier_g(L, gs);
-                                    obs_add(p + 3, g);
-                                    if (MODEL == 1) {
-                                        // the tau correlator repeats the sigma one and phi is
-                                        // their sum (ashkin_teller2d_hamiltonian.hpp:296-316)
-                                        obs_add(p + 7, g);
-                                        obs_add(p + 11, g + g);
-                                        obs_add(p + 15, fourier_g(L, gb));
-                                    }
+                                    Mdl::add_correlators([&](int slot, double gv) { obs_add(p + slot, gv); },
+                                                         fourier_g(L, gs),
+                                                         Mdl::kBaxterCorrelator ? fourier_g(L, gb) : 0.);
                                 }
                             }
                         }
@@ -499,14 +478,10 @@ __device__ __forceinline__ void sweep_walker_tuned(const KArgs& a, const DevGrou
                     }
                     const uint64_t Pn = P ^ fw;
                     if constexpr (OBS == 2) {
-                        if (MODEL == 0 || fl == 0) fourier_flip<L>(fs, P, fw, y, cs_tab);
-                        if (MODEL == 1) fourier_flip<L>(fb, P ^ Q, fw, y, cs_tab);
+                        if (pf == 0) fourier_flip<L>(fs, P, fw, y, cs_tab);
+                        if (Mdl::kBaxterCorrelator) fourier_flip<L>(fb, P ^ Q, fw, y, cs_tab);
                     }
-                    if (MODEL == 0) {
-                        cu[0] = Pn;
-                    } else {
-                        if (fl) cu[1] = Pn; else cu[0] = Pn;
-                    }
+                    plane_set(cu, pf, Pn);
 #pragma unroll
                     for (int q = 0; q < NP; ++q) {
                         lat[(int64_t)(q * L + y) * kLanes] = cu[q];
@@ -519,7 +494,7 @@ __device__ __forceinline__ void sweep_walker_tuned(const KArgs& a, const DevGrou
                 __syncthreads();   // this row's bounds consumed, the next row's written
                 buf ^= 1;
             }
-            if (MODEL == 1) fl ^= 1;
+            fl = Mdl::next_flavour(fl);
         }
     }
 
@@ -545,9 +520,9 @@ template <int MODEL, int L>
 __device__ __forceinline__ void fill_row_tuned(const DevGroup& G, const DevSystem& S, int wave, int nwaves, int lane,
                                                uint64_t sweep_base, int row, uint32_t ring_lane_buf,
                                                const BoundEntry* tab) {
-    constexpr int NPASS = MODEL == 0 ? 1 : 2;
+    constexpr int NPASS = Model<MODEL>::kPasses;
     constexpr int N = L * L;
-    constexpr uint64_t BLK = (uint64_t)((MODEL == 0 ? N : 2 * N) + 3) / 4;
+    constexpr uint64_t BLK = (uint64_t)(NPASS * N + 3) / 4;   // Philox blocks per sweep
     const uint32_t walker = (uint32_t)(G.first_walker - S.first_walker + lane);
     constexpr int per_sweep = NPASS * L;
     const int s = row / per_sweep, rem = row - s * per_sweep;
@@ -572,7 +547,7 @@ template <int MODEL, int L>
 __device__ __forceinline__ void produce_bounds_tuned(const DevGroup& G, const DevSystem& S, int wave, int nwaves,
                                                      int lane, int nsweeps, uint64_t sweep_base, uint32_t ring_lane,
                                                      const BoundEntry* tab) {
-    constexpr int NPASS = MODEL == 0 ? 1 : 2;
+    constexpr int NPASS = Model<MODEL>::kPasses;
     const int rows = nsweeps * NPASS * L;
     const bool mine = lane < G.nwalkers;
     if (mine) fill_row_tuned<MODEL, L>(G, S, wave, nwaves, lane, sweep_base, 0, ring_lane, tab);
