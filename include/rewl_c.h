@@ -317,6 +317,15 @@ int rewl_stitch_overlap(rewl_ctx* ctx, int32_t system, double states_per_site,
 int rewl_thermodynamics(const double* energies, const double* logdos, int32_t num_bins,
                         int32_t system_size, double t_min, double t_max, int32_t num_t,
                         double* out);
+/* No counterpart in the reference: rewl_thermodynamics for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Same arguments and results, bit for bit;
+ * its device staging is a static array of 2.75 MiB per GPU, held until the
+ * process ends, so num_bins <= 16384 and num_t <= 65536 (REWL_ERR_INVALID
+ * beyond).  Calls that overlap on one GPU share that array and race. */
+int rewl_thermodynamics_noalloc(const double* energies, const double* logdos, int32_t num_bins,
+                                int32_t system_size, double t_min, double t_max, int32_t num_t,
+                                double* out);
 
 /* Canonical averages of per-bin observables (the observable part of
  * Thermodynamics::calculate_thermodynamics, thermodynamics.hpp:156-236): on the

@@ -191,6 +191,8 @@ SIGNATURES = {
                      C.POINTER(C.c_int32)], C.c_int),
     "rewl_thermodynamics": ([C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int32, C.c_int32, C.c_double,
                              C.c_double, C.c_int32, C.POINTER(C.c_double)], C.c_int),
+    "rewl_thermodynamics_noalloc": ([C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int32, C.c_int32,
+                                     C.c_double, C.c_double, C.c_int32, C.POINTER(C.c_double)], C.c_int),
     "rewl_selfcheck": ([C.c_int32, C.POINTER(C.c_double)], C.c_int),
     # include/rewl_mlp.h
     "rewl_mlp_create": ([C.c_uint64, C.POINTER(_P)], C.c_int),

@@ -1423,6 +1423,64 @@ int rewl_thermodynamics(const double* E, const double* lng, int32_t nb, int32_t 
     return REWL_OK;
 }
 
+// ---- rewl_thermodynamics without allocation ------------------------------------
+// The staging buffers of rewl_thermodynamics (the two input curves and the
+// output table) live in one static device array instead of hipMalloc'd
+// memory.  Each GPU holds its own copy of `scratch` at its own address, which
+// hipGetSymbolAddress returns once per device with that device current; host
+// code uses only that pointer.  Every call on a GPU shares its copy: the
+// inputs are copied in and the output is written in full by every call, so
+// nothing needs clearing, and calls that overlap on one GPU race.
+constexpr int kStaticThermoMaxBins = 16384;
+constexpr int kStaticThermoMaxTemps = 65536;
+constexpr size_t kStaticThermoDoubles = 2 * (size_t)kStaticThermoMaxBins + 5 * (size_t)kStaticThermoMaxTemps;
+static __device__ __align__(256) double scratch[kStaticThermoDoubles];   // 2.75 MiB per GPU
+
+namespace {
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxScratchDevices] = {};
+
+// scratch's address on the current device (looked up on the first call there)
+int scratch_on_current_device(double** out) {
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxScratchDevices) return fail(REWL_ERR_DEVICE, "device index beyond the scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return fail(REWL_ERR_DEVICE, "hipGetSymbolAddress returned no address for the scratch array");
+        g_scratch_ptr[dev] = static_cast<double*>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return REWL_OK;
+}
+}  // namespace
+
+int rewl_thermodynamics_noalloc(const double* E, const double* lng, int32_t nb, int32_t N, double tmin, double tmax,
+                                int32_t nT, double* out) {
+    if (!E || !lng || !out || nb < 2 || nT < 1 || N < 1) return fail(REWL_ERR_INVALID, "bad arguments");
+    if (nb > kStaticThermoMaxBins || nT > kStaticThermoMaxTemps)
+        return fail(REWL_ERR_INVALID, "rewl_thermodynamics_noalloc: at most 16384 bins and 65536 temperatures "
+                                      "(the size of its static device scratch)");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(REWL_ERR_NO_DEVICE, "no HIP device");
+    double* base = nullptr;
+    int rc;
+    if ((rc = scratch_on_current_device(&base))) return rc;
+    double* dE = base;
+    double* dl = base + kStaticThermoMaxBins;
+    double* dout = base + 2 * (size_t)kStaticThermoMaxBins;
+    HIP_TRY(hipMemcpy(dE, E, (size_t)nb * sizeof(double), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(dl, lng, (size_t)nb * sizeof(double), hipMemcpyHostToDevice));
+    const double dT = (tmax - tmin) / (double)nT;
+    hipLaunchKernelGGL(k_thermo, dim3((nT + 127) / 128), dim3(128), 0, nullptr, dE, dl, nb, N, tmin, dT, nT, dout);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpy(out, dout, (size_t)nT * 5 * sizeof(double), hipMemcpyDeviceToHost));
+    return REWL_OK;
+}
+
 int rewl_thermo_observables(const double* E, const double* lng, const double* obs, int32_t nb, int32_t nobs,
                             double tmin, double tmax, int32_t nT, double* out) {
     if (!E || !lng || !obs || !out || nb < 2 || nT < 1 || nobs < 1 || nobs > kThermoMaxObs)
