@@ -29,7 +29,16 @@ HBM_PEAK_GBS = 8000.0
 # bench measures the device's own figure (rewl_selfcheck(2)) and uses that
 LDS_NOMINAL_GBS = 256 * 128 * 2.4
 BYTES_PER_ATTEMPT = 32.25   # 8 B read ln g[new] + 16 B RMW ln g[cur] + 8 B RMW count + 0.25 B lattice
-REFERENCE_EXE = os.path.join(ROOT, "oracle", "_ref", "rewl_ref_ising_L64")
+# The reference program as built by oracle/ref_build.sh (rewl_simulator_amd/build.py):
+# at this workload's cadence without the correlator (the headline baseline),
+# and with its defaults.
+REFERENCE_RUNS = {
+    "workload": (os.path.join(ROOT, "oracle", "_ref", "rewl_ref_ising_L64_plain"),
+                 "an exchange every 1000 sweeps, CORRELATION_LENGTHS off; it still updates its magnetisation "
+                 "moments at every attempt"),
+    "defaults": (os.path.join(ROOT, "oracle", "_ref", "rewl_ref_ising_L64"),
+                 "its defaults: an exchange every sweep, the correlator on"),
+}
 MPIEXEC = "/opt/conda/bin/mpiexec"
 # Measured HBM bytes of one sweep launch of this workload with the default
 # kernel, keyed by sweeps per launch: rocprofv3 --pmc FETCH_SIZE (x 2.0) and
@@ -44,7 +53,7 @@ def parse():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--sweeps-per-step", type=int, default=1000)
-    ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target length of the CPU-baseline sample")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0, help="target length of the CPU-baseline sample")
     ap.add_argument("--cpu-threads", type=int, default=0,
                     help="CPU-baseline ranks / threads; 0 = the host CPUs this process may use")
     ap.add_argument("--no-cpu", action="store_true")
@@ -132,22 +141,23 @@ def host_cpus():
     return info
 
 
-def reference_cpu_baseline(target_s, cores):
+def reference_cpu_baseline(target_s, cores, run="workload"):
     """The reference program itself (built from its sources by
-    oracle/ref_build.sh with its defaults: L = 64 Ising, one walker per rank,
-    exchange every sweep, the 1/t schedule, CORRELATION_LENGTHS) under MPICH's
-    mpiexec with one rank per host core, stopped after about target_s seconds.
-    Its rank 0 prints each iteration's update rate (sweeps x N / seconds,
+    oracle/ref_build.sh: L = 64 Ising, one walker per rank, the 1/t schedule;
+    REFERENCE_RUNS names the two builds) under MPICH's mpiexec with one rank
+    per host core, stopped after about target_s seconds.  Its rank 0 prints
+    each iteration's update rate (sweeps x N / seconds,
     rewl_simulation.hpp:600-614); every rank sweeps as often, so the job's rate
     is that figure times the rank count.  None when the binary or MPICH is
     missing."""
-    if not (os.path.exists(REFERENCE_EXE) and os.path.exists(MPIEXEC)):
+    exe, what = REFERENCE_RUNS[run]
+    if not (os.path.exists(exe) and os.path.exists(MPIEXEC)):
         return None
     rates = []
     with tempfile.TemporaryDirectory() as d:
         work = os.path.join(d, "work")
         os.makedirs(work)
-        proc = subprocess.Popen([MPIEXEC, "-launcher", "fork", "-n", str(cores), REFERENCE_EXE], cwd=work,
+        proc = subprocess.Popen([MPIEXEC, "-launcher", "fork", "-n", str(cores), exe], cwd=work,
                                 stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, start_new_session=True)
         lines = []
         reader = threading.Thread(target=lambda: lines.extend(proc.stdout), daemon=True)
@@ -174,10 +184,9 @@ def reference_cpu_baseline(target_s, cores):
         return None
     per_rank = sum(rates[1:]) / len(rates[1:])   # the first iteration includes the start-up
     return dict(value=per_rank * cores, unit="spin-flip attempts/s", cores=cores, kind="reference",
-                sample=f"the reference's L=64 Ising run (its defaults, {cores} MPI ranks = {cores} windows x 1 "
-                       f"walker, exchange every sweep, observables and correlator on) for {elapsed:.0f} s: "
-                       f"{len(rates) - 1} iteration rate(s) of rank 0 after the first, "
-                       f"{per_rank:.3e} attempts/s per rank")
+                sample=f"the reference's L=64 Ising program ({what}; {cores} MPI ranks = {cores} windows x 1 "
+                       f"walker) for {elapsed:.0f} s: {len(rates) - 1} iteration rate(s) of rank 0 after the "
+                       f"first, {per_rank:.3e} attempts/s per rank")
 
 
 def cpu_baseline(systems, params, target_s, threads):
@@ -197,20 +206,31 @@ def cpu_baseline(systems, params, target_s, threads):
 
 
 def cpu_baselines(a, systems, params, gpu_rate):
-    """cpu_baseline: the reference program on the host cores (kind
-    "reference"), or the oracle when the reference cannot run there (kind
-    "port"); the other one beside it."""
+    """cpu_baseline: the reference program on the host cores at this
+    workload's cadence (kind "reference"), or the oracle when the reference
+    cannot run there (kind "port"); beside it the reference with its
+    defaults, the oracle on the workload, and the oracle calibrated against
+    the reference on the same work (moment observables at every attempt, as
+    the reference build keeps them)."""
     cpus = host_cpus()
     cores = a.cpu_threads or cpus["used"]
     port = cpu_baseline(systems, params, a.cpu_seconds, cores)
     port["gpu_over_cpu"] = gpu_rate / port["value"]
-    ref = None if a.no_reference else reference_cpu_baseline(a.cpu_seconds, cores)
+    ref = None if a.no_reference else reference_cpu_baseline(a.cpu_seconds, cores, "workload")
     if ref is None:
         port["host"] = cpus
         return port
     ref["gpu_over_cpu"] = gpu_rate / ref["value"]
     ref["host"] = cpus
     ref["port"] = port
+    same = cpu_baseline(systems, dict(params, observables=1), a.cpu_seconds, cores)
+    ref["calibration"] = {"port_with_moments": same["value"], "reference": ref["value"],
+                          "port_over_reference": same["value"] / ref["value"],
+                          "sample": same["sample"] + " (observables = 1)"}
+    dflt = reference_cpu_baseline(a.cpu_seconds, cores, "defaults")
+    if dflt is not None:
+        dflt["gpu_over_cpu"] = gpu_rate / dflt["value"]
+        ref["reference_defaults"] = dflt
     return ref
 
 

@@ -70,18 +70,30 @@ REFERENCE_SRC = "/root/reference/src"
 REF_BASELINE = os.path.join(ORACLE_DIR, "_ref", "rewl_ref_ising_L64")
 
 
+# The reference program builds timed as CPU baselines: its defaults, and the
+# L = 64 Ising workload's own cadence (an exchange every 1000 sweeps) without
+# the correlator (CORRELATION_LENGTHS off; the reference still updates its
+# magnetisation moments at every attempt).
+REFERENCE_BUILDS = {
+    "ising_L64": dict(REF_MODEL="ISING2D", REF_L="64"),
+    "ising_L64_plain": dict(REF_MODEL="ISING2D", REF_L="64", REF_CORR="OFF", REF_SPE="1000"),
+}
+
+
 def build_reference(verbose=False):
-    """The reference program itself for the L = 64 Ising workload (its
-    defaults, every walker seeded with 1), the CPU baseline of kind
-    "reference": compiled from its sources by oracle/ref_build.sh where they
-    exist (the build container); elsewhere the binary built there is used."""
+    """The reference program itself for the L = 64 Ising workload (every
+    walker seeded with 1), the CPU baseline of kind "reference": compiled from
+    its sources by oracle/ref_build.sh where they exist (the build container);
+    elsewhere the binaries built there are used."""
     if not os.path.isdir(REFERENCE_SRC):
         return None
-    env = dict(os.environ, REF_NAME="ising_L64", REF_MODEL="ISING2D", REF_L="64")
     cmd = ["bash", os.path.join(ORACLE_DIR, "ref_build.sh")]
-    if verbose:
-        print("REF_NAME=ising_L64 REF_MODEL=ISING2D REF_L=64 " + " ".join(cmd), flush=True)
-    subprocess.run(cmd, env=env, check=True, stdout=None if verbose else subprocess.DEVNULL)
+    for name, opts in REFERENCE_BUILDS.items():
+        env = dict(os.environ, REF_NAME=name, **opts)
+        if verbose:
+            print(" ".join(f"{k}={v}" for k, v in dict(REF_NAME=name, **opts).items()) + " " + " ".join(cmd),
+                  flush=True)
+        subprocess.run(cmd, env=env, check=True, stdout=None if verbose else subprocess.DEVNULL)
     return REF_BASELINE
 
 

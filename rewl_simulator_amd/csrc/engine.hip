@@ -318,7 +318,7 @@ constexpr int kFieldProducers = 3;
 
 // LDS of one k_sweep_field group: ln g and counts, the field, the bound ring.
 size_t field_group_lds(int nbins, int nwalkers, int N, int L) {
-    return (((size_t)nbins * nwalkers * 12 + 15) & ~(size_t)15) + (size_t)N * 8 + 2 * (size_t)L * nwalkers * 8;
+    return (((size_t)nbins * nwalkers * 12 + 15) & ~(size_t)15) + (size_t)N * 8 + (2 * (size_t)L + 1) * nwalkers * 8;
 }
 
 size_t field_lds(const rewl_ctx* c, int cls) { return c->class_field_lds[cls]; }
@@ -327,8 +327,9 @@ template <int MODEL>
 int launch_sweep_field(rewl_ctx* c, int cls, int nsweeps) {
     const int ng = (int)c->class_groups[cls].size();
     const size_t lds = field_lds(c, cls);
+    // without observables the row-unrolled walker serves L in {16, 24, 32, 48}
     const void* kern = c->p.observables ? (const void*)k_sweep_field<MODEL, kFieldProducers, true>
-                                        : (const void*)k_sweep_field<MODEL, kFieldProducers>;
+                                        : (const void*)k_sweep_field<MODEL, kFieldProducers, false, true>;
     HIP_TRY(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     KArgs args = c->args;
     const int32_t* gl = c->d_class_groups[cls].p;
@@ -747,6 +748,14 @@ int rewl_create(const rewl_system_desc* systems, int32_t num_systems, const rewl
     if ((rc = upload(c->d_w_local, c->w_local, c->stream))) return bail(rc);
     if ((rc = upload(c->d_w_hist, c->w_hist, c->stream))) return bail(rc);
     if ((rc = upload(c->d_w_lat, c->w_lat, c->stream))) return bail(rc);
+    // A launch has one workgroup per group and the dispatcher starts them in
+    // list order: the longest first (attempts per sweep, DoF of the system),
+    // so that when a batch has more groups than the chip holds, the groups
+    // that start last are short ones (longest-processing-time order).
+    for (int cls = 0; cls < 4; ++cls)
+        std::stable_sort(c->class_groups[cls].begin(), c->class_groups[cls].end(), [&](int32_t x, int32_t y) {
+            return c->sys[(size_t)c->groups[(size_t)x].sys].ndof > c->sys[(size_t)c->groups[(size_t)y].sys].ndof;
+        });
     for (int cls = 0; cls < 4; ++cls)
         if ((rc = upload(c->d_class_groups[cls], c->class_groups[cls], c->stream))) return bail(rc);
     for (int m = 0; m < 2; ++m)
@@ -1714,9 +1723,14 @@ __global__ void __launch_bounds__(256) k_lds_probe(int iters, uint32_t* sink) {
     __syncthreads();
     uint32_t a0 = 0, a1 = 0, a2 = 0, a3 = 0;
     for (int k = 0; k < iters; k += 8) {
+        // the slots repeat with a period of a few iterations: an opaque base
+        // keeps the compiler from proving the reads loop-invariant and
+        // hoisting them out of the loop
+        int base = t + k * 256;
+        asm volatile("" : "+v"(base));
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
-            const uint4 v = buf[(t + (k + j) * 256) & (kLdsProbeSlots - 1)];
+            const uint4 v = buf[(base + j * 256) & (kLdsProbeSlots - 1)];
             a0 ^= v.x; a1 += v.y; a2 ^= v.z; a3 += v.w;
         }
     }

@@ -351,6 +351,43 @@ def test_row_unrolled_kernel_bit_exact(gpu, model, L, kernel):
     window_states_equal(eng, ora)
 
 
+@pytest.mark.parametrize("field", ["gaussian", "dyadic"])
+@pytest.mark.parametrize("L", [16, 24, 32, 48])
+@pytest.mark.parametrize("model", [abi.ISING2D, abi.ASHKIN_TELLER2D])
+def test_row_unrolled_field_walker_bit_exact(gpu, model, L, field):
+    """The field-model kernel's row-unrolled walker (L in {16, 24, 32, 48})
+    against the oracle: a Gaussian field (arbitrary FP64 energies, bins of 8),
+    and a field of multiples of 1/2 with bins of width 3, where E' - min is
+    often an exact multiple of the bin width, so that the reciprocal
+    quotient lands on an integer and the walker divides (the exact
+    fallback of its bin computation)."""
+    print(f"case: row-unrolled field walker model={model} L={L} field={field}", flush=True)
+    N = L * L
+    if field == "gaussian":
+        h = configs.gaussian_strain(L, 0.5, seed=100 + L)
+        bs = 8.0
+    else:
+        h = np.round(configs.gaussian_strain(L, 1.0, seed=200 + L) * 2.0) / 2.0
+        bs = 3.0
+    span = float(np.ceil(np.abs(h).sum())) + 8.0
+    if model == abi.ISING2D:
+        s = configs.ising_system(L, -2.0 * N - span, 2.0 * N + span, bs, 6, 2, field=h)
+    else:
+        s = configs.at_system(L, -5.0 * N - span, 3.0 * N + span, bs, 6, 2, field=h)
+    p = configs.params(lnf_initial=0.02, flatness_mode=abi.FLAT_MIN_OVER_MEAN, flatness=0.5,
+                       schedule=abi.SCHED_HALVE, sweeps_per_check=20, sweeps_per_exchange=5,
+                       adjust_mode=1 if model == abi.ASHKIN_TELLER2D else 0)
+    eng, ora = make_pair([s], p)
+    eng.adjust_to_range(); ora.adjust_to_range()
+    for _ in range(2):
+        for _ in range(4):
+            eng.sweeps(5); ora.sweeps(5)
+            eng.exchange(); ora.exchange()
+        eng.check(); ora.check()
+    assert_same(eng, ora, "after 40 sweeps", cursors=False)
+    window_states_equal(eng, ora)
+
+
 def test_row_unrolled_kernel_needs_a_built_size(gpu):
     s = configs.ising_full_range(12, 8.0, 2, 2)
     with pytest.raises(abi.REWLError):
