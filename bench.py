@@ -79,6 +79,12 @@ def parse():
                     help="instead of the throughput steps, time the run from the first sweep until every "
                          "window's ln f is below LNF")
     ap.add_argument("--max-checks", type=int, default=200000)
+    ap.add_argument("--time-to-lnf", type=float, default=1e-6, metavar="LNF",
+                    help="also time the workload from ln f = 1 to LNF and report it in the line (the metric's "
+                         "second half); 0 to skip")
+    ap.add_argument("--refine-copies", type=int, default=1, metavar="R",
+                    help="with --refine: refine in a context with R times the replicas per window, every walker "
+                         "seeded with the converged ln g and configurations")
     ap.add_argument("--refine", type=int, default=0, metavar="SWEEPS",
                     help="with --converge: then SWEEPS sweeps of multicanonical refinement (rewl_refine) and "
                          "the ln g error after it")
@@ -430,7 +436,7 @@ def converge_batch(a):
                     finished[si] = now
             if checks % 20 == 0:
                 print(f"# check {checks}: {len(finished)} of {len(systems)} realisations done, {now:.1f} s",
-                      flush=True)
+                      file=sys.stderr, flush=True)
     eng.synchronize()
     t1 = time.perf_counter()
     st1 = eng.stats()
@@ -491,7 +497,7 @@ def converge_at(a):
         if checks % 10 == 0 or eng.stats()["all_done"]:
             lnf = max(eng.window_info(k)["lnf"] for k in range(eng.num_windows))
             if checks % 100 == 0:
-                print(f"# check {checks}: max ln f {lnf:.3e}, {time.perf_counter() - t0:.1f} s", flush=True)
+                print(f"# check {checks}: max ln f {lnf:.3e}, {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
             if lnf < a.converge:
                 break
     eng.synchronize()
@@ -514,7 +520,28 @@ def converge_at(a):
     }), flush=True)
 
 
-def converge(eng, params, a):
+def widen(eng, systems, params, copies):
+    """A context like eng's (one system) with `copies` times the replicas per
+    window: every walker starts from eng's finalised ln g of its window and
+    the configuration of one of eng's walkers there.  More walkers sample the
+    same fixed weights in the same time while the chip has room for them."""
+    from rewl_simulator_amd.engine import Engine
+    s = dict(systems[0])
+    r0 = s["replicas_per_window"]
+    s["replicas_per_window"] = r0 * copies
+    big = Engine([s], params)
+    for k, w in enumerate(big.windows):
+        w0 = eng.windows[k]
+        lng = eng.logdos(w0["first_walker"])
+        for r in range(s["replicas_per_window"]):
+            g = w["first_walker"] + r
+            big.set_logdos(g, lng)
+            big.set_spins(g, eng.spins(w0["first_walker"] + r % r0))
+    big.adjust_to_range()
+    return big
+
+
+def converge(eng, params, a, systems=None, print_line=True):
     """Time to ln f < a.converge: wall time from the first sweep until the
     slowest window's ln f drops below the threshold (the reference's
     f < final_increment test, rewl_simulation.hpp:594, 636-671); then the
@@ -534,7 +561,7 @@ def converge(eng, params, a):
         if checks % 10 == 0 or eng.stats()["all_done"]:
             lnf = max(eng.window_info(k)["lnf"] for k in range(eng.num_windows))
             if checks % 200 == 0:
-                print(f"# check {checks}: max ln f {lnf:.3e}, {time.perf_counter() - t0:.1f} s", flush=True)
+                print(f"# check {checks}: max ln f {lnf:.3e}, {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
             if lnf < a.converge:
                 break
     eng.synchronize()
@@ -543,28 +570,37 @@ def converge(eng, params, a):
     import numpy as np
     eng.finalize()
 
-    def errors():
-        E, lng = eng.stitched(0, states_per_site=2.0)
-        ex = exact_binned_lng(E, params_bin_size(eng))
+    def errors(e):
+        E, lng = e.stitched(0, states_per_site=2.0)
+        ex = exact_binned_lng(E, params_bin_size(e))
         ok = np.isfinite(ex)
         d = (lng - ex)[ok]
-        E2, lng2 = eng.stitched(0, states_per_site=2.0, join="overlap")
+        E2, lng2 = e.stitched(0, states_per_site=2.0, join="overlap")
         ok2 = np.isfinite(ex[:len(lng2)]) & np.isfinite(lng2)
         d2 = (lng2 - ex[:len(lng2)])[ok2]
         return E, lng, ex, ok, d, d2
 
-    E, lng, ex, ok, d, d2 = errors()
+    E, lng, ex, ok, d, d2 = errors(eng)
     refined = None
     if a.refine > 0:
         t2 = time.perf_counter()
-        eng.refine(a.refine, params["sweeps_per_exchange"])
-        eng.synchronize()
+        target = widen(eng, systems, params, a.refine_copies) if a.refine_copies > 1 else eng
+        t2b = time.perf_counter()
+
+        def progress(done):
+            print(f"# refinement: {done} of {a.refine} sweeps, {time.perf_counter() - t2b:.1f} s", file=sys.stderr, flush=True)
+
+        target.refine(a.refine, params["sweeps_per_exchange"], chunk=25000, progress=progress)
+        target.synchronize()
         t3 = time.perf_counter()
-        _, _, _, _, r1, r2 = errors()
-        refined = {"sweeps": a.refine, "seconds": t3 - t2,
+        _, _, _, _, r1, r2 = errors(target)
+        refined = {"sweeps": a.refine, "walkers": target.num_walkers, "seconds": t3 - t2,
+                   "seconds_to_seed_the_walkers": t2b - t2,
                    "lng_max_abs_error_normalised": float(np.max(np.abs(r1))),
                    "lng_max_abs_error_overlap_join": float(np.max(np.abs(r2))),
                    "total_seconds": (t1 - t0) + (t3 - t2)}
+        if target is not eng:
+            target.close()
     if a.dump:
         nbmax = max(w["num_bins"] for w in eng.windows)
         wl = np.full((eng.num_windows, nbmax), np.nan)
@@ -572,7 +608,7 @@ def converge(eng, params, a):
             wl[k, :w["num_bins"]] = eng.logdos(w["first_walker"])
         np.savez(a.dump, E=E, lng=lng, exact=ex, window_emin=np.array([w["e_min"] for w in eng.windows]),
                  window_lng=wl)
-    print(json.dumps({
+    result = {
         "metric": "time to ln f < %g" % a.converge, "value": t1 - t0, "unit": "s", "higher_is_better": False,
         "converged": bool(lnf < a.converge), "checks": checks, "sweeps": st1["sweeps"] - st0["sweeps"],
         "attempts": st1["attempts"] - st0["attempts"],
@@ -585,7 +621,27 @@ def converge(eng, params, a):
         "refined": refined,
         "config": {"workload": "2D Ising REWL, L=64 periodic, 32 windows x 64 walkers, flatness 0.8",
                    "schedule": "1/t" if a.one_over_t else "halving", "average_mode": params["average_mode"]},
-    }), flush=True)
+    }
+    if print_line:
+        print(json.dumps(result), flush=True)
+    return result
+
+
+def time_to_lnf_leg(systems, params, a):
+    """The metric's second half for the line: a fresh context of the same
+    workload run from ln f = 1 until every window's ln f is below
+    a.time_to_lnf (halving, equal-weight averaging, an exchange and a check
+    every 1000 sweeps), with the stitched ln g against the exact DoS.  The
+    learned warm start is not part of this workload (plain flat init)."""
+    from rewl_simulator_amd.engine import Engine
+    p = dict(params, sweeps_per_check=1000, sweeps_per_exchange=1000, average_mode=1)
+    leg = argparse.Namespace(converge=a.time_to_lnf, max_checks=2000, refine=0, refine_copies=1, dump=None,
+                             one_over_t=False)
+    with Engine(systems, p) as eng:
+        eng.adjust_to_range()
+        r = converge(eng, p, leg, systems, print_line=False)
+    return {k: r[k] for k in ("metric", "value", "unit", "converged", "checks", "sweeps", "attempts",
+                              "lng_max_abs_error_normalised", "lng_max_abs_error_overlap_join")}
 
 
 def params_bin_size(eng):
@@ -632,7 +688,7 @@ def main():
         eng.check()
 
     if a.converge > 0:
-        converge(eng, params, a)
+        converge(eng, params, a, systems)
         return
     for _ in range(a.warmup):
         step()
@@ -718,6 +774,8 @@ def main():
         line["saturating_batch"] = saturating_batch(systems, params, a.saturate, peak)
     if a.cadence_sweeps > 0:
         line["reference_cadence"] = reference_cadence(systems, params, a.cadence_sweeps)
+    if a.time_to_lnf > 0:
+        line["time_to_lnf"] = time_to_lnf_leg(systems, params, a)
     if not a.no_cpu:
         line["cpu_baseline"] = cpu_baselines(a, systems, params, rate)
     else:

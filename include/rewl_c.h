@@ -259,6 +259,14 @@ int rewl_finalize(rewl_ctx* ctx);
  * error of the result falls as 1 / sqrt(sweeps), without the saturation of
  * Wang-Landau's own estimate; stitch afterwards as after rewl_finalize. */
 int rewl_refine(rewl_ctx* ctx, int64_t sweeps, int32_t sweeps_per_exchange);
+/* rewl_refine in parts, for callers that report progress between them:
+ * rewl_refine_begin (average, ln f = 0, clear), any number of
+ * rewl_refine_sweeps (the exchange cadence counted across calls), then
+ * rewl_refine_end (ln g += ln H).  begin + sweeps(a) + sweeps(b) + end is
+ * rewl_refine(a + b), bit for bit. */
+int rewl_refine_begin(rewl_ctx* ctx);
+int rewl_refine_sweeps(rewl_ctx* ctx, int64_t sweeps, int32_t sweeps_per_exchange);
+int rewl_refine_end(rewl_ctx* ctx);
 int rewl_synchronize(rewl_ctx* ctx);
 
 /* ---- queries ------------------------------------------------------------- */

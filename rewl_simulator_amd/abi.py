@@ -184,6 +184,9 @@ SIGNATURES = {
     "rewl_set_window_lnf": ([_P, C.c_int32, C.c_double], C.c_int),
     "rewl_set_spins": ([_P, C.c_int32, C.POINTER(C.c_int8), C.c_int32], C.c_int),
     "rewl_refine": ([_P, C.c_int64, C.c_int32], C.c_int),
+    "rewl_refine_begin": ([_P], C.c_int),
+    "rewl_refine_sweeps": ([_P, C.c_int64, C.c_int32], C.c_int),
+    "rewl_refine_end": ([_P], C.c_int),
     "rewl_set_system_spins": ([_P, C.c_int32, C.POINTER(C.c_int8), C.c_int32], C.c_int),
     "rewl_stitch": ([_P, C.c_int32, C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int32,
                      C.POINTER(C.c_int32)], C.c_int),

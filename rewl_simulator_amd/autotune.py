@@ -70,13 +70,14 @@ def measure(systems, params, shape, sweeps):
 def autotune(systems, params, sweeps=20, cache_path=None, verbose=False):
     """Returns (best_shape, {shape: attempts/s}) and remembers the choice."""
     key = signature(systems, params)
-    if key in _CACHE:
-        return _CACHE[key], {}
-    if cache_path and os.path.exists(cache_path):
-        disk = json.load(open(cache_path))
-        if key in disk:
-            _CACHE[key] = disk[key]
-            return disk[key], {}
+    disk = json.load(open(cache_path)) if cache_path and os.path.exists(cache_path) else {}
+    if key in _CACHE or key in disk:
+        best = _CACHE[key] if key in _CACHE else disk[key]
+        _CACHE[key] = best
+        if cache_path and key not in disk:
+            disk[key] = best
+            json.dump(disk, open(cache_path, "w"), indent=1)
+        return best, {}
     results = {}
     best, best_rate = None, -1.0
     for shape in candidates(systems, params):
@@ -91,7 +92,6 @@ def autotune(systems, params, sweeps=20, cache_path=None, verbose=False):
             best, best_rate = shape, rate
     _CACHE[key] = best
     if cache_path:
-        disk = json.load(open(cache_path)) if os.path.exists(cache_path) else {}
         disk[key] = best
         json.dump(disk, open(cache_path, "w"), indent=1)
     return best, results

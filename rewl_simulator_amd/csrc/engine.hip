@@ -181,6 +181,8 @@ struct rewl_ctx {
     int64_t pending_sweeps = 0;
     bool direction_even = true;
     bool in_windows = false;   // every walker inside its window (rewl_adjust_to_range ran)
+    bool refining = false;     // between rewl_refine_begin and rewl_refine_end
+    int64_t refine_since_exchange = 0;
     rewl_stats stats{};
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_sweep;
     std::vector<int32_t> done_host;
@@ -1081,8 +1083,8 @@ int rewl_finalize(rewl_ctx* c) {
     return REWL_OK;
 }
 
-int rewl_refine(rewl_ctx* c, int64_t sweeps, int32_t sweeps_per_exchange) {
-    if (!c || sweeps < 1 || sweeps_per_exchange < 1) return fail(REWL_ERR_INVALID, "bad arguments");
+int rewl_refine_begin(rewl_ctx* c) {
+    if (!c) return fail(REWL_ERR_INVALID, "null context");
     if (!c->in_windows)
         return fail(REWL_ERR_RANGE, "walkers are not inside their windows: call rewl_adjust_to_range first");
     int rc;
@@ -1095,25 +1097,53 @@ int rewl_refine(rewl_ctx* c, int64_t sweeps, int32_t sweeps_per_exchange) {
     const unsigned grid = (unsigned)std::min<int64_t>(4096, (n + 255) / 256);
     hipLaunchKernelGGL(k_hist_clear, dim3(grid), dim3(256), 0, c->stream, c->args, n);
     HIP_TRY(hipGetLastError());
+    c->refining = true;
+    c->refine_since_exchange = 0;
+    return REWL_OK;
+}
+
+int rewl_refine_sweeps(rewl_ctx* c, int64_t sweeps, int32_t sweeps_per_exchange) {
+    if (!c || sweeps < 0 || sweeps_per_exchange < 1) return fail(REWL_ERR_INVALID, "bad arguments");
+    if (!c->refining) return fail(REWL_ERR_INVALID, "rewl_refine_sweeps outside rewl_refine_begin / rewl_refine_end");
+    const int64_t n = (int64_t)c->hist_total;
+    const unsigned grid = (unsigned)std::min<int64_t>(4096, (n + 255) / 256);
+    int rc;
     // launches of at most 1000 sweeps keep a bin's u32 count far below 2^32;
-    // an exchange after every sweeps_per_exchange sweeps
-    int64_t done = 0, since_exchange = 0;
+    // an exchange after every sweeps_per_exchange sweeps, counted across calls
+    int64_t done = 0;
     while (done < sweeps) {
-        const int k = (int)std::min<int64_t>({1000, (int64_t)sweeps_per_exchange - since_exchange, sweeps - done});
+        const int k = (int)std::min<int64_t>(
+            {1000, (int64_t)sweeps_per_exchange - c->refine_since_exchange, sweeps - done});
         if ((rc = rewl_sweeps(c, k))) return rc;
         done += k;
-        since_exchange += k;
-        if (since_exchange == sweeps_per_exchange) {
+        c->refine_since_exchange += k;
+        if (c->refine_since_exchange >= sweeps_per_exchange) {
             if (c->p.exchange && (rc = rewl_exchange(c))) return rc;
-            since_exchange = 0;
+            c->refine_since_exchange = 0;
         }
         hipLaunchKernelGGL(k_hist_roll, dim3(grid), dim3(256), 0, c->stream, c->args, n);
         HIP_TRY(hipGetLastError());
     }
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return REWL_OK;
+}
+
+int rewl_refine_end(rewl_ctx* c) {
+    if (!c) return fail(REWL_ERR_INVALID, "null context");
+    if (!c->refining) return fail(REWL_ERR_INVALID, "rewl_refine_end without rewl_refine_begin");
     hipLaunchKernelGGL(k_refine, dim3((unsigned)c->win.size()), dim3(kLanes), 0, c->stream, c->args);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(c->stream));
+    c->refining = false;
     return REWL_OK;
+}
+
+int rewl_refine(rewl_ctx* c, int64_t sweeps, int32_t sweeps_per_exchange) {
+    if (!c || sweeps < 1 || sweeps_per_exchange < 1) return fail(REWL_ERR_INVALID, "bad arguments");
+    int rc;
+    if ((rc = rewl_refine_begin(c))) return rc;
+    if ((rc = rewl_refine_sweeps(c, sweeps, sweeps_per_exchange))) return rc;
+    return rewl_refine_end(c);
 }
 
 int rewl_synchronize(rewl_ctx* c) {
@@ -1725,12 +1755,14 @@ __global__ void __launch_bounds__(256) k_lds_probe(int iters, uint32_t* sink) {
     for (int k = 0; k < iters; k += 8) {
         // the slots repeat with a period of a few iterations: an opaque base
         // keeps the compiler from proving the reads loop-invariant and
-        // hoisting them out of the loop
-        int base = t + k * 256;
+        // hoisting them out of the loop;
+        int base = t + k * 128;
         asm volatile("" : "+v"(base));
+        // eight distinct slots per thread and iteration (offsets j * 128 of
+        // 1024), so that no two reads of an iteration can be merged
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
-            const uint4 v = buf[(base + j * 256) & (kLdsProbeSlots - 1)];
+            const uint4 v = buf[(base + j * 128) & (kLdsProbeSlots - 1)];
             a0 ^= v.x; a1 += v.y; a2 ^= v.z; a3 += v.w;
         }
     }

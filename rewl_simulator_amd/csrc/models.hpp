@@ -163,11 +163,29 @@ struct Model<1> {
         return E + 0.5 * local_energy(J, K, has_field, h, own, nb);
     }
     // E' of flipping sigma (fl = 0) or tau (fl = 1) of a site (change_state,
-    // ashkin_teller2d_hamiltonian.hpp:223-250): E + local(new) - local(old)
+    // ashkin_teller2d_hamiltonian.hpp:223-250): E + local(new) - local(old),
+    // each sum in the reference's order.  One of sigma, tau flips, so the
+    // proposal's products K s t and h (s t) are the current ones negated,
+    // exactly (every factor is +-1 or the coupling, a negation is exact).
     __device__ static double proposal_energy(double J, double K, bool has_field, double h, double E,
                                              const double own[2], const double nb[2][4], int fl) {
-        const double prop[2] = {fl == 0 ? -own[0] : own[0], fl == 1 ? -own[1] : own[1]};
-        return E + local_energy(J, K, has_field, h, prop, nb) - local_energy(J, K, has_field, h, own, nb);
+        const double sg = own[0], ta = own[1];
+        const double Js = J * sg, Jt = J * ta, Kst = K * sg * ta;
+        const double hst = has_field ? h * (sg * ta) : 0.;
+        const double Jsn = fl == 0 ? -Js : Js, Jtn = fl == 1 ? -Jt : Jt;
+        double en_new = 0., en_old = 0.;
+        if (has_field) { en_new += -hst; en_old += hst; }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const double nn = nb[0][k] * nb[1][k];
+            en_new = add_term(en_new, Jsn, nb[0][k]);
+            en_new = add_term(en_new, Jtn, nb[1][k]);
+            en_new = add_term(en_new, -Kst, nn);
+            en_old = add_term(en_old, Js, nb[0][k]);
+            en_old = add_term(en_old, Jt, nb[1][k]);
+            en_old = add_term(en_old, Kst, nn);
+        }
+        return E + -en_new - -en_old;
     }
 
     // Packed integer path: the flipped plane P (and the other plane Q) give

@@ -223,9 +223,10 @@ __device__ void sweep_walker_field(const KArgs& a, const DevGroup& G, const DevS
 // row-unrolled walker) this is a bit extract and one shift-or.
 __device__ __forceinline__ double spin_d(uint64_t w, int x) {
     const uint32_t half = x < 32 ? (uint32_t)w : (uint32_t)(w >> 32);
-    // bit x moved to bit 31, merged into the high word of 1.0 (v_bfi_b32)
-    const uint32_t hi = __builtin_amdgcn_ubfe(half, x & 31, 1) << 31;
-    return __hiloint2double((int)(hi | 0x3FF00000u), 0);
+    // bit x moved to bit 31 and merged into the high word of 1.0: a shift
+    // and one and-or
+    const uint32_t hi = ((half << (31 - (x & 31))) & 0x80000000u) | 0x3FF00000u;
+    return __hiloint2double((int)hi, 0);
 }
 
 // The row-unrolled field walker: the same chain, draws and FP64 arithmetic as
@@ -270,7 +271,7 @@ __device__ void sweep_walker_field_rows(const KArgs& a, const DevGroup& G, const
     int fl = active ? a.w_flavor[g] : 0;
     double Ed = active ? a.w_E[g] : 0.0;
     int cb = min(max((int)((Ed - Wn.emin) / Wn.bs), 0), Wn.nbins - 1);
-    double lc = active ? s_lng[cb * stride + lane] : 0.0;
+    double lc = active ? s_lng[__mul24(cb, stride) + lane] : 0.0;
     const int plo = Wn.phi_lo, phi = Wn.phi_hi, nbmax = Wn.nbins - 1;
     const double emin = Wn.emin, bs = Wn.bs;
     const double inv_bs = 1.0 / bs;
@@ -327,17 +328,17 @@ __device__ void sweep_walker_field_rows(const KArgs& a, const DevGroup& G, const
                         const double xe = En - emin;
                         const double yq = xe * inv_bs;
                         int nb = in ? min(max((int)yq, 0), nbmax) : cb;
-                        double ln = s_lng[nb * stride + lane];
+                        double ln = s_lng[__mul24(nb, stride) + lane];
                         const double m = rint(yq);
                         if (in && yq > 0.5 && fabs(yq - m) <= yq * 0x1p-48) {
                             nb = min(max((int)(xe / bs), 0), nbmax);
-                            ln = s_lng[nb * stride + lane];
+                            ln = s_lng[__mul24(nb, stride) + lane];
                         }
                         const bool acc = in && (ln - lc < B);
                         if (acc) { Ed = En; cb = nb; lc = ln; }
                         lc += lnf;
-                        s_lng[cb * stride + lane] = lc;
-                        atomicAdd(&s_cnt[cb * stride + lane], 1u);
+                        s_lng[__mul24(cb, stride) + lane] = lc;
+                        atomicAdd(&s_cnt[__mul24(cb, stride) + lane], 1u);
                         if (acc) {
                             plane_set(cu, P, plane_pick(cu, P) ^ (1ull << x));
                             plane_set(sc, P, -plane_pick(sc, P));

@@ -90,11 +90,23 @@ def anneal(systems, params, spins=True):
 class Engine:
     """One rewl_ctx: every walker of the given systems on the current GPU."""
 
-    def __init__(self, systems, params):
+    def __init__(self, systems, params, autotune=False):
+        """autotune: before creating the context, time the candidate launch
+        shapes of this workload on scratch contexts (autotune.autotune: kernel
+        variant, producer wavefronts, walkers per wavefront) and create it
+        with the fastest; True, or a JSON path that caches the choice across
+        processes.  Every shape runs the same chains."""
         self.lib = abi.load_engine()
         self._keep = []
         self.systems = [dict(s) for s in systems]
         self.params = dict(params)
+        if autotune:
+            from . import autotune as tuner
+            best, _ = tuner.autotune(self.systems, self.params,
+                                     cache_path=autotune if isinstance(autotune, str) else None)
+            if best:
+                self.params.update(best)
+                self.tuned_shape = dict(best)
         descs = _desc_array(self.systems, self._keep)
         pp = _params(self.params)
         ctx = C.c_void_p()
@@ -152,11 +164,25 @@ class Engine:
     def finalize(self):
         abi.check(self.lib.rewl_finalize(self.ctx))
 
-    def refine(self, sweeps, sweeps_per_exchange=None):
+    def refine(self, sweeps, sweeps_per_exchange=None, chunk=None, progress=None):
         """Multicanonical refinement of the finished run (rewl_refine): fixed
-        weights 1/g for `sweeps` sweeps, then ln g += ln H per window."""
+        weights 1/g for `sweeps` sweeps, then ln g += ln H per window.  With
+        `chunk`, the sweeps run in parts of that many (rewl_refine_begin /
+        _sweeps / _end, the same result) and progress(done) is called after
+        each."""
         spe = self.params["sweeps_per_exchange"] if sweeps_per_exchange is None else sweeps_per_exchange
-        abi.check(self.lib.rewl_refine(self.ctx, int(sweeps), int(spe)))
+        if not chunk:
+            abi.check(self.lib.rewl_refine(self.ctx, int(sweeps), int(spe)))
+            return
+        abi.check(self.lib.rewl_refine_begin(self.ctx))
+        done = 0
+        while done < sweeps:
+            k = min(int(chunk), int(sweeps) - done)
+            abi.check(self.lib.rewl_refine_sweeps(self.ctx, k, int(spe)))
+            done += k
+            if progress:
+                progress(done)
+        abi.check(self.lib.rewl_refine_end(self.ctx))
 
     def synchronize(self):
         abi.check(self.lib.rewl_synchronize(self.ctx))

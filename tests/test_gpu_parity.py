@@ -5,6 +5,8 @@ mt19937_64 in replay mode, or the engine's Philox positions) from the same
 initial state, so spins, energies, ln g, counts, visits and stream cursors
 must agree bit for bit after every step.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -419,6 +421,27 @@ def test_autotuner_picks_a_shape_that_runs(gpu):
     eng.adjust_to_range(); ora.adjust_to_range()
     eng.sweeps(10); ora.sweeps(10)
     assert_same(eng, ora, f"autotuned shape {best}", cursors=False)
+
+
+def test_engine_autotune_creates_the_tuned_shape(gpu):
+    """Engine(..., autotune=...) creates its context with the autotuner's
+    choice (cached in a JSON file for the next process) and runs the oracle's
+    chain with it."""
+    import tempfile
+    print("case: engine autotune", flush=True)
+    systems = [configs.ising_full_range(16, 8.0, 2, 4)]
+    p = configs.params(sweeps_per_check=20, sweeps_per_exchange=20)
+    with tempfile.TemporaryDirectory() as d:
+        cache = os.path.join(d, "shapes.json")
+        eng = Engine(systems, p, autotune=cache)
+        assert os.path.exists(cache)
+        assert set(eng.tuned_shape) == {"walkers_per_group", "producers", "sweep_kernel"}
+        assert all(eng.params[k] == v for k, v in eng.tuned_shape.items())
+    ora = Oracle(systems, eng.params)
+    eng.adjust_to_range(); ora.adjust_to_range()
+    eng.sweeps(10); ora.sweeps(10)
+    assert_same(eng, ora, f"engine autotuned {eng.tuned_shape}", cursors=False)
+    eng.close(); ora.close()
 
 
 def test_retired_systems_equal_their_own_runs(gpu):

@@ -37,6 +37,30 @@ def test_refine_matches_oracle(gpu, rng):
     eng.close(); ora.close()
 
 
+def test_refine_in_parts_equals_one_call(gpu):
+    """rewl_refine_begin + rewl_refine_sweeps in parts that cut across the
+    exchange cadence + rewl_refine_end: the same chains and ln g as one
+    rewl_refine call, bit for bit."""
+    print("case: refine in parts", flush=True)
+    s = configs.ising_full_range(8, 4.0, 3, 3)
+    p = configs.params(flatness_mode=abi.FLAT_MIN_OVER_MEAN, flatness=0.8, schedule=abi.SCHED_HALVE,
+                       lnf_final=1e-2, sweeps_per_check=100, sweeps_per_exchange=50)
+    one, parts = Engine([s], p), Engine([s], p)
+    for e in (one, parts):
+        e.adjust_to_range()
+        e.run(200)
+    one.refine(3000, 50)
+    seen = []
+    parts.refine(3000, 50, chunk=700, progress=seen.append)
+    assert seen == [700, 1400, 2100, 2800, 3000]
+    for g in range(one.num_walkers):
+        assert one.energy(g) == parts.energy(g)
+        np.testing.assert_array_equal(one.spins(g), parts.spins(g))
+        np.testing.assert_array_equal(one.visits(g), parts.visits(g))
+        np.testing.assert_array_equal(one.logdos(g).view(np.uint64), parts.logdos(g).view(np.uint64))
+    one.close(); parts.close()
+
+
 def test_refined_lng_within_0_01_of_exact_L16(gpu):
     """2D Ising L = 16 (the first workload's lattice: full spectrum, bins of
     4, 4 windows at 75% overlap), here 128 walkers per window: Wang-Landau to
