@@ -235,11 +235,51 @@ def evaluate(mlp, specs, lnf_final=1e-6, lnf_warm=1e-3, ref_lnf=1e-8, max_checks
           for r, (L, sigma, seed) in zip(yardstick, specs)]
     res["prediction_error_max"] = float(np.max(pe))
     res["prediction_error_median"] = float(np.median(pe))
+    res["_reference_curves"] = ref
     res["speedup_wall"] = res["plain"]["seconds"] / (res["warm"]["seconds"] + res["warm"]["init_seconds"])
     res["speedup_sweeps"] = res["plain"]["sweeps"] / max(res["warm"]["sweeps"], 1)
     log(f"evaluate: speedup {res['speedup_wall']:.2f}x wall, {res['speedup_sweeps']:.2f}x sweeps; "
         f"warm vs plain max |d ln g| {res['warm_vs_plain_max']:.3f}; network error {res['prediction_error_max']:.3f}")
     return res
+
+
+def evaluate_refine(mlp, specs, ref, plain, first=2000, sweeps_per_exchange=250, log=print):
+    """The network's ln g as fixed weights from the start: rounds of
+    multicanonical refinement (rewl_refine; each samples with ln f = 0 and
+    then corrects ln g by ln H), the round length doubling from `first`,
+    until the ln g error against the reference (max and median over the
+    realisations, anchored, over commonly visited bins) is no worse than
+    plain Wang-Landau's at its final ln f, or the sweeps reach plain's.  The
+    network's prediction replaces the whole Wang-Landau run: without it, the
+    flat start's fixed weights would leave the walkers at the most probable
+    energies."""
+    systems = rfat_systems(specs)
+    rounds = []
+    with Engine(systems, rfat_params(1e-6)) as eng:
+        eng.adjust_to_range()
+        t0 = time.perf_counter()
+        warm_start(eng, mlp, lambda i: specs[i][1], 1e-3)
+        t_init = time.perf_counter() - t0
+        total, r, secs = 0, int(first), 0.0
+        while total < plain["sweeps"]:
+            t1 = time.perf_counter()
+            eng.refine(r, sweeps_per_exchange)
+            secs += time.perf_counter() - t1
+            total += r
+            c = curves(eng, specs)
+            errs = [anchored_max_error(g["lng"], rr["lng"], rr["visited"] & g["visited"]) for g, rr in zip(c, ref)]
+            rounds.append(dict(sweeps=total, seconds=secs, lng_error_max=float(np.max(errs)),
+                               lng_error_median=float(np.median(errs))))
+            log(f"refine from the network: {total} sweeps, {secs:.1f} s, ln g error max {np.max(errs):.3f}, "
+                f"median {np.median(errs):.3f}")
+            if np.max(errs) <= plain["lng_error_max"] and np.median(errs) <= plain["lng_error_median"]:
+                break
+            r *= 2
+    last = rounds[-1]
+    reached = last["lng_error_max"] <= plain["lng_error_max"] and last["lng_error_median"] <= plain["lng_error_median"]
+    return dict(rounds=rounds, init_seconds=t_init, reached_plain_error=bool(reached),
+                speedup_sweeps=plain["sweeps"] / last["sweeps"],
+                speedup_wall=plain["seconds"] / (last["seconds"] + t_init))
 
 
 def budget_run(mlp, specs, checks, lnf_warm=1e-3, log=print):
@@ -287,6 +327,9 @@ def main():
     ap.add_argument("--steps", type=int, default=4000)
     ap.add_argument("--budget-L", type=int, default=0, help="also a fixed-budget comparison at this L")
     ap.add_argument("--budget-checks", type=int, default=200)
+    ap.add_argument("--refine-first", type=int, default=0, metavar="SWEEPS",
+                    help="also run the network's ln g through rounds of multicanonical refinement (first round "
+                         "SWEEPS, doubling) until it matches plain Wang-Landau's error (needs --ref-lnf)")
     a = ap.parse_args()
     os.makedirs(a.out, exist_ok=True)
     logf = open(os.path.join(a.out, "log.txt"), "a")
@@ -316,6 +359,9 @@ def main():
             log(json.dumps(info))
             return
     res = evaluate(mlp, held, lnf_final=a.lnf_final, lnf_warm=a.lnf_warm, ref_lnf=a.ref_lnf, log=log)
+    ref = res.pop("_reference_curves")
+    if a.refine_first and ref is not None:
+        res["refine_from_network"] = evaluate_refine(mlp, held, ref, res["plain"], first=a.refine_first, log=log)
     res.update(info)
     res["heldout"] = dict(count=len(held), L=held[0][0], sigma=held[0][1])
     if a.budget_L:
