@@ -644,6 +644,28 @@ def time_to_lnf_leg(systems, params, a):
                               "lng_max_abs_error_normalised", "lng_max_abs_error_overlap_join")}
 
 
+# The walker's instruction stream of the default launch: 24 instructions
+# between successive attempts of the unrolled L = 64 row (the median over the
+# row in the ISA of k_sweep_tuned<0, 64, true, 1, 0>), about 16 of them VALU.
+# A wavefront issues at most one VALU instruction every 4 cycles, so the
+# attempt cannot take fewer than about 4 cycles per instruction.
+WALKER_INSTRUCTIONS_PER_ATTEMPT = 24
+
+
+def latency_model(cycles, clock_ghz, walkers, walkers_per_wave, default_launch):
+    """Cycles per attempt per walker (the walker's chain is the bound of this
+    workload), and for the default launch the issue floor of its instruction
+    stream and the fraction of it reached."""
+    out = {"cycles_per_attempt_per_walker": cycles, "clock_ghz_nominal": clock_ghz, "cus": 256}
+    if default_launch:
+        floor = 4.0 * WALKER_INSTRUCTIONS_PER_ATTEMPT
+        out.update({"walker_wavefronts": -(-walkers // 33), "walkers_per_wavefront": walkers_per_wave,
+                    "issue_floor": {"instructions_per_attempt": WALKER_INSTRUCTIONS_PER_ATTEMPT,
+                                    "cycles_per_instruction": 4, "floor_cycles": floor,
+                                    "frac": floor / cycles}})
+    return out
+
+
 def params_bin_size(eng):
     return eng.systems[0]["bin_size"]
 
@@ -762,9 +784,7 @@ def main():
                     "frac": traffic / avg_launch_s / 1e9 / HBM_PEAK_GBS if traffic else None},
             # one walker is one sequential chain: what bounds this workload is
             # the walker wavefront's time per attempt, not a bandwidth
-            "latency": {"cycles_per_attempt_per_walker": cycles, "clock_ghz_nominal": clock_ghz,
-                        "walker_wavefronts": -(-W // 33), "walkers_per_wavefront": walkers_per_wave,
-                        "cus": 256},
+            "latency": latency_model(cycles, clock_ghz, W, walkers_per_wave, default_launch),
         },
         "state": {"min_lnf_after": lnf, "flat_events": st1["flat_events"],
                   "exchanges_accepted": st1["exchanges_accepted"]},

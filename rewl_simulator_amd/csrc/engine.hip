@@ -342,6 +342,13 @@ int launch_sweep_field(rewl_ctx* c, int cls, int nsweeps) {
     return REWL_OK;
 }
 
+// LDS bytes per (bin, walker) of the row-unrolled kernel's observable sums:
+// with observables = 1 and three moments (Ising) they stay in LDS for the
+// launch (sweep_tuned.hpp); otherwise none.
+size_t tuned_obs_bytes(int model, int observables) {
+    return observables == 1 && model_info(model).values == 3 ? 24 : 0;
+}
+
 // Philox mode, integer path, L in kTunedL: the row-unrolled kernel (sweep_tuned.hpp).
 template <int MODEL, int L>
 int launch_sweep_tuned(rewl_ctx* c, int li, int nsweeps) {
@@ -351,7 +358,8 @@ int launch_sweep_tuned(rewl_ctx* c, int li, int nsweeps) {
     const size_t lanes = (size_t)c->tuned_maxlanes[MODEL][li];
     const int guard = c->tuned_guard[MODEL][li];
     const size_t lds = ((size_t)c->tuned_maxbins[MODEL][li] + 2 * guard) * lanes * kTunedSlot +
-                       (size_t)tuned_ring_stride(L) * lanes + (c->p.observables == 2 ? 16 * (size_t)L : 0);
+                       (size_t)tuned_ring_stride(L) * lanes + (c->p.observables == 2 ? 16 * (size_t)L : 0) +
+                       (size_t)c->tuned_maxbins[MODEL][li] * lanes * tuned_obs_bytes(MODEL, c->p.observables);
     const bool simple = c->tuned_simple[MODEL][li];
     const int np = c->tuned_nprod;
     // sweep_kernel 2: no speculation, 3 (and auto): next slot for both outcomes,
@@ -597,7 +605,8 @@ int rewl_create(const rewl_system_desc* systems, int32_t num_systems, const rewl
             // observables = 2: the cos/sin table of the correlator after the rings
             const size_t room = (size_t)c->lds_max - (P.observables == 2 ? 16 * (size_t)S.d.L : 0);
             wpg = std::min(kLanes, (int)(room / ((size_t)(maxbins + 2 * guard) * kTunedSlot +
-                                                 (size_t)tuned_ring_stride(S.d.L))));
+                                                 (size_t)tuned_ring_stride(S.d.L) +
+                                                 (size_t)maxbins * tuned_obs_bytes(S.d.model, P.observables))));
         }
         else if (fast)
             wpg = std::min(kLanes, (int)((size_t)c->lds_max /
@@ -765,7 +774,8 @@ int rewl_create(const rewl_system_desc* systems, int32_t num_systems, const rewl
             if (c->tuned_groups[m][i].empty()) continue;
             const size_t need = ((size_t)c->tuned_maxbins[m][i] + 2 * c->tuned_guard[m][i]) * c->tuned_maxlanes[m][i] *
                                     kTunedSlot + (size_t)tuned_ring_stride(kTunedL[i]) * c->tuned_maxlanes[m][i] +
-                                (P.observables == 2 ? 16 * (size_t)kTunedL[i] : 0);
+                                (P.observables == 2 ? 16 * (size_t)kTunedL[i] : 0) +
+                                (size_t)c->tuned_maxbins[m][i] * c->tuned_maxlanes[m][i] * tuned_obs_bytes(m, P.observables);
             if (need > (size_t)c->lds_max) return bail(fail(REWL_ERR_INVALID, "group too wide for LDS"));
             if ((rc = upload(c->d_tuned_groups[m][i], c->tuned_groups[m][i], c->stream))) return bail(rc);
         }

@@ -188,6 +188,12 @@ __device__ __forceinline__ void sweep_walker_tuned(const KArgs& a, const DevGrou
     // slot of bin b: (b + guard) * stride16 + lane * 16, b in [-guard, nbins + guard)
     const uint32_t zero_off = lds_base() + (uint32_t)(guard * stride16 + lane * kTunedSlot);
     const uint32_t sent = zero_off + (uint32_t)(nbins * stride16);   // first upper guard bin
+    // observables = 1 with three moments (Ising): the per-bin sums live in LDS
+    // after the ring, [bin][walker][3] FP64, added at every attempt and
+    // added to HBM after the launch; the bin's sums of the current slot are
+    // at obs_lane + 1.5 (cbo - zero_off)
+    constexpr bool LOBS = OBS == 1 && Mdl::kValues == 3;
+    const uint32_t obs_lane = cs_tab + (uint32_t)(lane * 24);
 
     if (active) {
         for (int b = -guard; b < gbins + guard; ++b) {
@@ -199,6 +205,10 @@ __device__ __forceinline__ void sweep_walker_tuned(const KArgs& a, const DevGrou
             // all-time updates of the bin = this word + its current count
             if (OBS == 2) lds_store_u32(off + 12, inside ? a.obs_tot[hb + (int64_t)b * kLanes] - c : 0u);
         }
+        if (LOBS)
+            for (int b = 0; b < gbins; ++b)
+#pragma unroll
+                for (int i = 0; i < 3; ++i) lds_store_f64(obs_lane + (uint32_t)(b * stride * 24 + 8 * i), 0.);
     }
     __syncthreads();   // histograms loaded, first row of thresholds ready
 
@@ -247,6 +257,15 @@ __device__ __forceinline__ void sweep_walker_tuned(const KArgs& a, const DevGrou
                           csf, fb);
     }
     auto observe = [&]() {
+        if constexpr (LOBS) {
+            double v[3];
+            Mdl::values(L * L, mag, v);
+            const uint32_t d = cbo - zero_off;
+            const uint32_t oa = obs_lane + d + (d >> 1);
+#pragma unroll
+            for (int i = 0; i < 3; ++i) lds_add_f64(oa + 8u * (uint32_t)i, v[i]);
+            return;
+        }
         if (cbo != acck) {
             flush_obs();
             acck = cbo;
@@ -499,7 +518,13 @@ __device__ __forceinline__ void sweep_walker_tuned(const KArgs& a, const DevGrou
     }
 
     if (active) {
-        if (OBS) flush_obs();
+        if (OBS && !LOBS) flush_obs();
+        if (LOBS)
+            for (int b = 0; b < nbins; ++b) {
+                double* p = a.obs + (hb + (int64_t)b * kLanes) * a.nobs;
+#pragma unroll
+                for (int i = 0; i < 3; ++i) p[i] += lds_f64(obs_lane + (uint32_t)(b * stride * 24 + 8 * i));
+            }
         for (int b = 0; b < nbins; ++b) {
             const uint32_t off = zero_off + (uint32_t)(b * stride16);
             a.lng[hb + (int64_t)b * kLanes] = lds_f64(off);
@@ -579,8 +604,9 @@ __global__ void __launch_bounds__(512) k_sweep_tuned(KArgs a, const int32_t* gro
         const uint32_t ring = lds_base() + (uint32_t)((G.nbins + 2 * guard) * G.nwalkers * kTunedSlot);
         const int lane = tid & (kLanes - 1);
         const uint32_t ring_lane = ring + (uint32_t)(lane * tuned_ring_stride(L));
-        // observables = 2: cos and sin of qmin k after the ring (Transform_Kernel,
-        // fourier_correlator.hpp:14-42), visible to the walker after its first barrier
+        // after the ring: with observables = 2 the cos and sin of qmin k
+        // (Transform_Kernel, fourier_correlator.hpp:14-42), visible to the walker
+        // after its first barrier; with observables = 1 (Ising) the per-bin sums
         const uint32_t cs_tab = ring + (uint32_t)(G.nwalkers * tuned_ring_stride(L));
         if (OBS == 2) {
             const double qmin = 2. * acos(-1.) / (double)L;
