@@ -343,10 +343,10 @@ int launch_sweep_field(rewl_ctx* c, int cls, int nsweeps) {
 }
 
 // LDS bytes per (bin, walker) of the row-unrolled kernel's observable sums:
-// with observables = 1 and three moments (Ising) they stay in LDS for the
-// launch (sweep_tuned.hpp); otherwise none.
+// with observables on, the moment sums stay in LDS for the launch
+// (sweep_tuned.hpp); otherwise none.
 size_t tuned_obs_bytes(int model, int observables) {
-    return observables == 1 && model_info(model).values == 3 ? 24 : 0;
+    return observables >= 1 ? 8 * (size_t)model_info(model).values : 0;
 }
 
 // Philox mode, integer path, L in kTunedL: the row-unrolled kernel (sweep_tuned.hpp).

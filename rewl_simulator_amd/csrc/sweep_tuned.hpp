@@ -188,12 +188,14 @@ __device__ __forceinline__ void sweep_walker_tuned(const KArgs& a, const DevGrou
     // slot of bin b: (b + guard) * stride16 + lane * 16, b in [-guard, nbins + guard)
     const uint32_t zero_off = lds_base() + (uint32_t)(guard * stride16 + lane * kTunedSlot);
     const uint32_t sent = zero_off + (uint32_t)(nbins * stride16);   // first upper guard bin
-    // observables = 1 with three moments (Ising): the per-bin sums live in LDS
-    // after the ring, [bin][walker][3] FP64, added at every attempt and
-    // added to HBM after the launch; the bin's sums of the current slot are
-    // at obs_lane + 1.5 (cbo - zero_off)
-    constexpr bool LOBS = OBS == 1 && Mdl::kValues == 3;
-    const uint32_t obs_lane = cs_tab + (uint32_t)(lane * 24);
+    // observables: the per-bin moment sums live in LDS after the ring (and
+    // after the correlator's cos/sin table with observables = 2),
+    // [bin][walker][kValues] FP64, added at every attempt and added to HBM
+    // after the launch; the current slot's sums are at
+    // obs_lane + (cbo - zero_off) / 2 * kValues
+    constexpr bool LOBS = OBS >= 1;
+    constexpr int NVL = Mdl::kValues;
+    const uint32_t obs_lane = cs_tab + (uint32_t)(OBS == 2 ? 16 * L : 0) + (uint32_t)(lane * 8 * NVL);
 
     if (active) {
         for (int b = -guard; b < gbins + guard; ++b) {
@@ -208,7 +210,7 @@ __device__ __forceinline__ void sweep_walker_tuned(const KArgs& a, const DevGrou
         if (LOBS)
             for (int b = 0; b < gbins; ++b)
 #pragma unroll
-                for (int i = 0; i < 3; ++i) lds_store_f64(obs_lane + (uint32_t)(b * stride * 24 + 8 * i), 0.);
+                for (int i = 0; i < NVL; ++i) lds_store_f64(obs_lane + (uint32_t)((b * stride * NVL + i) * 8), 0.);
     }
     __syncthreads();   // histograms loaded, first row of thresholds ready
 
@@ -258,12 +260,11 @@ __device__ __forceinline__ void sweep_walker_tuned(const KArgs& a, const DevGrou
     }
     auto observe = [&]() {
         if constexpr (LOBS) {
-            double v[3];
+            double v[NVL];
             Mdl::values(L * L, mag, v);
-            const uint32_t d = cbo - zero_off;
-            const uint32_t oa = obs_lane + d + (d >> 1);
+            const uint32_t oa = obs_lane + (uint32_t)__mul24((int)((cbo - zero_off) >> 1), NVL);
 #pragma unroll
-            for (int i = 0; i < 3; ++i) lds_add_f64(oa + 8u * (uint32_t)i, v[i]);
+            for (int i = 0; i < NVL; ++i) lds_add_f64(oa + 8u * (uint32_t)i, v[i]);
             return;
         }
         if (cbo != acck) {
@@ -523,7 +524,8 @@ __device__ __forceinline__ void sweep_walker_tuned(const KArgs& a, const DevGrou
             for (int b = 0; b < nbins; ++b) {
                 double* p = a.obs + (hb + (int64_t)b * kLanes) * a.nobs;
 #pragma unroll
-                for (int i = 0; i < 3; ++i) p[i] += lds_f64(obs_lane + (uint32_t)(b * stride * 24 + 8 * i));
+                for (int i = 0; i < NVL; ++i)
+                    p[(i / 3) * GK + i % 3] += lds_f64(obs_lane + (uint32_t)((b * stride * NVL + i) * 8));
             }
         for (int b = 0; b < nbins; ++b) {
             const uint32_t off = zero_off + (uint32_t)(b * stride16);
