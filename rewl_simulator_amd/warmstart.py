@@ -120,14 +120,15 @@ def curves(eng, specs):
     return out
 
 
-def run_until_done(eng, max_checks, log=None, every=50, name=""):
+def run_until_done(eng, max_checks, log=None, every=50, name="", max_seconds=None):
     """Run check cycles until every window is at its final ln f; returns
     (converged, seconds).  Runs in chunks of `every` checks so that progress
-    (the slowest window's ln f) can be logged."""
+    (the slowest window's ln f) can be logged; with max_seconds, no new chunk
+    starts after that much wall time."""
     eng.synchronize()
     t0 = time.perf_counter()
     done, checks = False, 0
-    while checks < max_checks and not done:
+    while checks < max_checks and not done and (max_seconds is None or time.perf_counter() - t0 < max_seconds):
         k = min(every, max_checks - checks)
         done = eng.run(k)
         checks += k
@@ -138,15 +139,17 @@ def run_until_done(eng, max_checks, log=None, every=50, name=""):
     return done, time.perf_counter() - t0
 
 
-def generate(specs, lnf_final=1e-6, batch=256, max_checks=20000, log=print):
-    """Converged curves for every spec, `batch` realisations per context."""
+def generate(specs, lnf_final=1e-6, batch=256, max_checks=20000, log=print, max_seconds=None):
+    """Converged curves for every spec, `batch` realisations per context
+    (with max_seconds, each context stops after that much wall time and its
+    windows keep the ln g they have)."""
     out = []
     for b in range(0, len(specs), batch):
         part = specs[b:b + batch]
         t0 = time.perf_counter()
         with Engine(rfat_systems(part), rfat_params(lnf_final)) as eng:
             eng.adjust_to_range()
-            done, secs = run_until_done(eng, max_checks, log=log, name="generate")
+            done, secs = run_until_done(eng, max_checks, log=log, name="generate", max_seconds=max_seconds)
             eng.finalize()
             out.extend(curves(eng, part))
             st = eng.stats()
@@ -301,6 +304,53 @@ def budget_run(mlp, specs, checks, lnf_warm=1e-3, log=print):
     return res
 
 
+def stage_trace(specs, seconds, mlp=None, lnf_warm=1e-3, every=20, log=print):
+    """Plain (flat ln g, ln f = 1) or, with `mlp`, warm-started (network ln g,
+    ln f = lnf_warm) Wang-Landau on the systems for a wall-clock budget,
+    recording per window the sweeps after which its ln f was first at or
+    below 2^-j, j = 0..20 (2^-20 < 1e-6, the final ln f): the time each
+    window spends in each stage.  reach[k, j] = -1 where window k has not got
+    there; j at or above the starting stage is the first check's sweeps."""
+    systems = rfat_systems(specs)
+    params = rfat_params(1e-6)
+    spc = params["sweeps_per_check"]
+    with Engine(systems, params) as eng:
+        eng.adjust_to_range()
+        if mlp is not None:
+            warm_start(eng, mlp, lambda i: specs[i][1], lnf_warm)
+        nw = eng.num_windows
+        index = np.array([w["window"] for w in eng.windows])
+        system = np.array([w["system"] for w in eng.windows])
+        reach = np.full((nw, 21), -1, dtype=np.int64)
+        eng.synchronize()
+        t0 = time.perf_counter()
+        checks, done = 0, False
+        while not done and time.perf_counter() - t0 < seconds:
+            done = eng.run(every)
+            checks += every
+            lnf = np.array([eng.window_info(k)["lnf"] for k in range(nw)])
+            stage = np.minimum(np.floor(-np.log2(lnf) + 1e-9).astype(np.int64), 20)
+            for k in range(nw):
+                row = reach[k, :stage[k] + 1]
+                row[row < 0] = checks * spc
+            slow = ", ".join(f"{lnf[index == w].max():.1e}" for w in range(index.max() + 1))
+            log(f"stage trace: {checks * spc} sweeps, {time.perf_counter() - t0:.0f} s; slowest ln f per window "
+                f"index [{slow}]; windows done {(lnf < 1e-6).mean():.3f}")
+        eng.synchronize()
+        secs = time.perf_counter() - t0
+    summary = []
+    for w in range(index.max() + 1):
+        sel = index == w
+        row = {}
+        for j in (1, 10, 20):
+            got = reach[sel, j][reach[sel, j] >= 0]
+            row[f"lnf_2^-{j}"] = dict(windows=int(got.size), of=int(sel.sum()),
+                                      median_sweeps=float(np.median(got)) if got.size else None)
+        summary.append(row)
+    return dict(sweeps=checks * spc, seconds=secs, converged=bool(done), start="network" if mlp else "flat",
+                per_window_index=summary, reach=reach.tolist(), window=index.tolist(), system=system.tolist())
+
+
 def training_specs(n, sizes=(16, 24, 32, 48), sigmas=(0.0, 0.25, 0.5, 0.75, 1.0), seed0=50000):
     """n (L, sigma, seed) triples cycling sizes and disorder strengths; the
     disorder seeds are disjoint from the held-out ones."""
@@ -327,6 +377,13 @@ def main():
     ap.add_argument("--steps", type=int, default=4000)
     ap.add_argument("--budget-L", type=int, default=0, help="also a fixed-budget comparison at this L")
     ap.add_argument("--budget-checks", type=int, default=200)
+    ap.add_argument("--heldout-seed0", type=int, default=900000, help="disorder seed of the first held-out realisation")
+    ap.add_argument("--generate-seconds", type=float, default=0.0,
+                    help="stop generating after this much wall time (windows keep the ln g they have)")
+    ap.add_argument("--stage-trace", type=float, default=0.0, metavar="SECONDS",
+                    help="instead of the comparison: per-window stage times of the held-out realisations for "
+                         "SECONDS, from a flat ln g at ln f = 1 or from the network's at --lnf-warm (--stage-start)")
+    ap.add_argument("--stage-start", choices=("flat", "network"), default="network")
     ap.add_argument("--refine-first", type=int, default=0, metavar="SWEEPS",
                     help="also run the network's ln g through rounds of multicanonical refinement (first round "
                          "SWEEPS, doubling) until it matches plain Wang-Landau's error (needs --ref-lnf)")
@@ -342,14 +399,19 @@ def main():
     sizes = tuple(int(v) for v in a.sizes.split(","))
     sigmas = tuple(float(v) for v in a.sigmas.split(","))
     train_specs = training_specs(a.curves, sizes=sizes, sigmas=sigmas)
-    held = heldout_specs(a.heldout, L=a.heldout_L)
+    held = heldout_specs(a.heldout, L=a.heldout_L, seed0=a.heldout_seed0)
+    if a.stage_trace and a.stage_start == "flat":
+        res = stage_trace(held, a.stage_trace, log=log)
+        log(json.dumps(res["per_window_index"]))
+        json.dump(res, open(os.path.join(a.out, "stage_trace_flat.json"), "w"))
+        return
     if a.mlp_in:
         mlp = MLP()
         mlp.load(a.mlp_in)
         info = dict(mlp_in=a.mlp_in)
     else:
         t0 = time.perf_counter()
-        cl = generate(train_specs, lnf_final=a.lnf_final, log=log)
+        cl = generate(train_specs, lnf_final=a.lnf_final, log=log, max_seconds=a.generate_seconds or None)
         gen_s = time.perf_counter() - t0
         mlp, hist = train(cl, steps=a.steps, log=log)
         info = dict(generate_seconds=gen_s, train_ms=mlp.last_ms(), curves=len(cl),
@@ -358,6 +420,12 @@ def main():
         if a.no_eval:
             log(json.dumps(info))
             return
+    if a.stage_trace:
+        res = stage_trace(held, a.stage_trace, mlp=mlp, lnf_warm=a.lnf_warm, log=log)
+        res.update(info)
+        log(json.dumps(res["per_window_index"]))
+        json.dump(res, open(os.path.join(a.out, "stage_trace_network.json"), "w"))
+        return
     res = evaluate(mlp, held, lnf_final=a.lnf_final, lnf_warm=a.lnf_warm, ref_lnf=a.ref_lnf, log=log)
     ref = res.pop("_reference_curves")
     if a.refine_first and ref is not None:
