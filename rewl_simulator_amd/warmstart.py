@@ -304,14 +304,24 @@ def budget_run(mlp, specs, checks, lnf_warm=1e-3, log=print):
     return res
 
 
-def stage_trace(specs, seconds, mlp=None, lnf_warm=1e-3, every=20, log=print):
+def record_stages(reach, lnf, sweeps):
+    """reach[k, j] = sweeps for every window k whose ln f is now at or below
+    2^-j and that has no entry for j yet (j up to reach.shape[1] - 1)."""
+    stage = np.minimum(np.floor(-np.log2(np.asarray(lnf, dtype=np.float64)) + 1e-9).astype(np.int64),
+                       reach.shape[1] - 1)
+    for k in range(reach.shape[0]):
+        row = reach[k, :stage[k] + 1]
+        row[row < 0] = sweeps
+
+
+def stage_trace(specs, seconds, mlp=None, lnf_warm=1e-3, every=20, log=print, windows=8):
     """Plain (flat ln g, ln f = 1) or, with `mlp`, warm-started (network ln g,
     ln f = lnf_warm) Wang-Landau on the systems for a wall-clock budget,
     recording per window the sweeps after which its ln f was first at or
     below 2^-j, j = 0..20 (2^-20 < 1e-6, the final ln f): the time each
     window spends in each stage.  reach[k, j] = -1 where window k has not got
     there; j at or above the starting stage is the first check's sweeps."""
-    systems = rfat_systems(specs)
+    systems = rfat_systems(specs, windows=windows)
     params = rfat_params(1e-6)
     spc = params["sweeps_per_check"]
     with Engine(systems, params) as eng:
@@ -329,10 +339,7 @@ def stage_trace(specs, seconds, mlp=None, lnf_warm=1e-3, every=20, log=print):
             done = eng.run(every)
             checks += every
             lnf = np.array([eng.window_info(k)["lnf"] for k in range(nw)])
-            stage = np.minimum(np.floor(-np.log2(lnf) + 1e-9).astype(np.int64), 20)
-            for k in range(nw):
-                row = reach[k, :stage[k] + 1]
-                row[row < 0] = checks * spc
+            record_stages(reach, lnf, checks * spc)
             slow = ", ".join(f"{lnf[index == w].max():.1e}" for w in range(index.max() + 1))
             log(f"stage trace: {checks * spc} sweeps, {time.perf_counter() - t0:.0f} s; slowest ln f per window "
                 f"index [{slow}]; windows done {(lnf < 1e-6).mean():.3f}")
@@ -384,6 +391,7 @@ def main():
                     help="instead of the comparison: per-window stage times of the held-out realisations for "
                          "SECONDS, from a flat ln g at ln f = 1 or from the network's at --lnf-warm (--stage-start)")
     ap.add_argument("--stage-start", choices=("flat", "network"), default="network")
+    ap.add_argument("--stage-windows", type=int, default=8, help="windows per held-out realisation in --stage-trace")
     ap.add_argument("--refine-first", type=int, default=0, metavar="SWEEPS",
                     help="also run the network's ln g through rounds of multicanonical refinement (first round "
                          "SWEEPS, doubling) until it matches plain Wang-Landau's error (needs --ref-lnf)")
@@ -401,7 +409,7 @@ def main():
     train_specs = training_specs(a.curves, sizes=sizes, sigmas=sigmas)
     held = heldout_specs(a.heldout, L=a.heldout_L, seed0=a.heldout_seed0)
     if a.stage_trace and a.stage_start == "flat":
-        res = stage_trace(held, a.stage_trace, log=log)
+        res = stage_trace(held, a.stage_trace, log=log, windows=a.stage_windows)
         log(json.dumps(res["per_window_index"]))
         json.dump(res, open(os.path.join(a.out, "stage_trace_flat.json"), "w"))
         return
@@ -421,7 +429,7 @@ def main():
             log(json.dumps(info))
             return
     if a.stage_trace:
-        res = stage_trace(held, a.stage_trace, mlp=mlp, lnf_warm=a.lnf_warm, log=log)
+        res = stage_trace(held, a.stage_trace, mlp=mlp, lnf_warm=a.lnf_warm, log=log, windows=a.stage_windows)
         res.update(info)
         log(json.dumps(res["per_window_index"]))
         json.dump(res, open(os.path.join(a.out, "stage_trace_network.json"), "w"))

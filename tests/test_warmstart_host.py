@@ -63,3 +63,18 @@ def test_rfat_systems_half_range():
     assert s["e_max"] == 0.0
     assert s["bin_size"] == pytest.approx(-s["e_min"] / 512)
     assert s["num_windows"] == 8 and s["replicas_per_window"] == 4
+
+
+def test_record_stages_first_time_at_or_below_each_power_of_two():
+    reach = np.full((4, 21), -1, dtype=np.int64)
+    # ln f = 1, 2^-1 exactly, 1e-3 (between 2^-10 and 2^-9), below 1e-6
+    W.record_stages(reach, [1.0, 0.5, 1e-3, 5e-7], 250)
+    assert reach[0].tolist() == [250] + [-1] * 20
+    assert reach[1].tolist() == [250, 250] + [-1] * 19
+    assert reach[2].tolist() == [250] * 10 + [-1] * 11      # 2^-9 = 1.95e-3 >= 1e-3 > 2^-10
+    assert reach[3].tolist() == [250] * 21                  # clipped at 2^-20 < 1e-6
+    # later checks fill only the stages newly reached; earlier entries stay
+    W.record_stages(reach, [0.25, 0.5, 2.0 ** -10, 5e-7], 500)
+    assert reach[0, :3].tolist() == [250, 500, 500] and reach[0, 3] == -1
+    assert reach[1].tolist() == [250, 250] + [-1] * 19
+    assert reach[2, 9] == 250 and reach[2, 10] == 500 and reach[2, 11] == -1
