@@ -440,6 +440,17 @@ def converge_batch(a):
     eng.synchronize()
     t1 = time.perf_counter()
     st1 = eng.stats()
+    # per lattice size and window index: windows at the final ln f, and
+    # windows past their first stage (ln f halved at least once)
+    per_window = {}
+    infos = [eng.window_info(k) for k in range(eng.num_windows)]
+    for L in sorted(set(sizes)):
+        rows = []
+        for w in range(max(i["window"] for i in infos) + 1):
+            sel = [i for i in infos if sizes[i["system"]] == L and i["window"] == w]
+            rows.append({"windows": len(sel), "done": sum(int(i["done"]) for i in sel),
+                         "past_first_stage": sum(int(i["lnf"] < 1.0) for i in sel)})
+        per_window[str(L)] = rows
     eng.close()
     by_L = {}
     for L in sorted(set(sizes)):
@@ -455,7 +466,7 @@ def converge_batch(a):
         "converged": len(finished) == len(systems), "checks": checks, "sweeps_per_check": n,
         "retire": not a.no_retire, "sweeps": st1["sweeps"] - st0["sweeps"],
         "attempts": st1["attempts"] - st0["attempts"], "wall_s": t1 - t0,
-        "by_L": by_L,
+        "by_L": by_L, "per_window_index": per_window,
         "sweep_launch_ms_per_check": {"first_10_mean": sum(head) / max(len(head), 1),
                                       "last_10_mean": sum(tail) / max(len(tail), 1)},
         "config": {"workload": AT_WORKLOADS["config4"]["workload"]},

@@ -36,3 +36,26 @@ def test_warm_start_cuts_sweeps_on_heldout_disorder(gpu):
     factor = plain["sweeps"] / warm["sweeps"]
     print(f"warm start: {factor:.2f}x fewer sweeps to ln f < 1e-4", flush=True)
     assert factor >= 1.5, (warm["sweeps"], plain["sweeps"])
+
+
+def test_stage_trace_records_every_window_in_order(gpu):
+    """The per-window stage record of warmstart.stage_trace on small
+    realisations: a flat start is at 2^0 from the first check, every window
+    gets through its first stages, a stage is never reached before the one
+    above it, and the record is the same in a second, identical run (the
+    chains are fixed by their Philox streams)."""
+    import numpy as np
+    specs = [(8, 0.5, 777000 + i) for i in range(4)]
+    runs = [W.stage_trace(specs, 6.0, every=4, log=lambda m: None) for _ in range(2)]
+    for res in runs:
+        reach = np.array(res["reach"])
+        assert reach.shape == (4 * 8, 21) and res["start"] == "flat"
+        assert (reach[:, 0] == 4 * 250).all()
+        assert (reach[:, 3] > 0).all(), reach[:, :6]
+        for row in reach:
+            got = row[row >= 0]
+            assert (row[:got.size] == got).all()          # reached stages form a prefix
+            assert (np.diff(got) >= 0).all()
+    n = min(r["sweeps"] for r in runs)
+    a, b = (np.where(np.array(r["reach"]) <= n, np.array(r["reach"]), -1) for r in runs)
+    assert (a == b).all()
