@@ -82,6 +82,10 @@ def parse():
     ap.add_argument("--time-to-lnf", type=float, default=1e-6, metavar="LNF",
                     help="also time the workload from ln f = 1 to LNF and report it in the line (the metric's "
                          "second half); 0 to skip")
+    ap.add_argument("--warm-start-lnf", type=float, default=1e-4, metavar="LNF",
+                    help="also run the learned warm start end to end at a small size (training curves, the "
+                         "network, held-out realisations plain vs warm-started to LNF) and report the factor in "
+                         "the line; 0 to skip")
     ap.add_argument("--refine-copies", type=int, default=1, metavar="R",
                     help="with --refine: refine in a context with R times the replicas per window, every walker "
                          "seeded with the converged ln g and configurations")
@@ -655,6 +659,41 @@ def time_to_lnf_leg(systems, params, a):
                               "lng_max_abs_error_normalised", "lng_max_abs_error_overlap_join")}
 
 
+def warm_start_leg(a):
+    """The metric's warm-start comparison, sized to run in about a minute:
+    the learned warm start end to end.  Training curves: 24 random-strain AT
+    realisations at L = 8 and 12 (sigma 0, 0.5, 1) run to ln f <
+    a.warm_start_lnf; the network trained on them by the library's kernels;
+    8 held-out L = 12 realisations (sigma 0.5) run from a flat ln g at
+    ln f = 1 (plain) and from the network's ln g at 1e-3 (warm) to the same
+    final ln f.  Sweep counts are deterministic (Philox streams, fixed
+    seeds); wall times are the engine's."""
+    from rewl_simulator_amd import warmstart as W
+
+    def log(msg):
+        print("# " + msg.strip(), file=sys.stderr, flush=True)
+
+    t0 = time.perf_counter()
+    curves = W.generate(W.training_specs(24, sizes=(8, 12), sigmas=(0.0, 0.5, 1.0)), lnf_final=a.warm_start_lnf,
+                        batch=24, max_checks=4000, log=log)
+    mlp, _ = W.train(curves, steps=6000, batch=8192, log=log)
+    res = W.evaluate(mlp, W.heldout_specs(8, L=12, sigma=0.5), lnf_final=a.warm_start_lnf, ref_lnf=0.0,
+                     max_checks=4000, log=log)
+    plain, warm = res["plain"], res["warm"]
+    return {
+        "metric": "sweeps and wall time to ln f < %g, learned warm start vs plain" % a.warm_start_lnf,
+        "workload": "8 held-out random-strain AT realisations, L=12, sigma=0.5, 8 windows x 4 walkers, 512 bins "
+                    "over [E(all-up), 0); network trained on 24 engine-produced curves at L=8, 12",
+        "plain": {"sweeps": plain["sweeps"], "seconds": plain["seconds"], "converged": plain["converged"]},
+        "warm": {"sweeps": warm["sweeps"], "seconds": warm["seconds"], "init_seconds": warm["init_seconds"],
+                 "converged": warm["converged"], "lnf_start": warm["lnf_start"]},
+        "factor_sweeps": res["speedup_sweeps"], "factor_wall": res["speedup_wall"],
+        "warm_vs_plain_max_abs_lng": res["warm_vs_plain_max"],
+        "network_error_max": res["prediction_error_max"],
+        "leg_seconds": time.perf_counter() - t0,
+    }
+
+
 # The walker's instruction stream of the default launch: 24 instructions
 # between successive attempts of the unrolled L = 64 row (the median over the
 # row in the ISA of k_sweep_tuned<0, 64, true, 1, 0>), about 16 of them VALU.
@@ -807,6 +846,8 @@ def main():
         line["reference_cadence"] = reference_cadence(systems, params, a.cadence_sweeps)
     if a.time_to_lnf > 0:
         line["time_to_lnf"] = time_to_lnf_leg(systems, params, a)
+    if a.warm_start_lnf > 0:
+        line["warm_start"] = warm_start_leg(a)
     if not a.no_cpu:
         line["cpu_baseline"] = cpu_baselines(a, systems, params, rate)
     else:
