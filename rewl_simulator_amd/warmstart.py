@@ -320,7 +320,7 @@ def stage_trace(specs, seconds, mlp=None, lnf_warm=1e-3, every=20, log=print, wi
     recording per window the sweeps after which its ln f was first at or
     below 2^-j, j = 0..20 (2^-20 < 1e-6, the final ln f): the time each
     window spends in each stage.  reach[k, j] = -1 where window k has not got
-    there; j at or above the starting stage is the first check's sweeps."""
+    there; j at or below the starting stage gets the first check's sweeps."""
     systems = rfat_systems(specs, windows=windows)
     params = rfat_params(1e-6)
     spc = params["sweeps_per_check"]
@@ -354,7 +354,7 @@ def stage_trace(specs, seconds, mlp=None, lnf_warm=1e-3, every=20, log=print, wi
             row[f"lnf_2^-{j}"] = dict(windows=int(got.size), of=int(sel.sum()),
                                       median_sweeps=float(np.median(got)) if got.size else None)
         summary.append(row)
-    return dict(sweeps=checks * spc, seconds=secs, converged=bool(done), start="network" if mlp else "flat",
+    return dict(sweeps=checks * spc, seconds=secs, converged=bool(done), start="flat" if mlp is None else "network",
                 per_window_index=summary, reach=reach.tolist(), window=index.tolist(), system=system.tolist())
 
 
