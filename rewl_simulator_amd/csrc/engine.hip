@@ -165,6 +165,7 @@ struct rewl_ctx {
     DevBuf<unsigned long long> d_exch;
     DevBuf<BoundEntry> d_bound_tab;
     DevBuf<double> d_obs;       // observable sums (observables on)
+    DevBuf<double> d_stage;     // rewl_set_logdos: one walker's ln g, copied linearly, then scattered
     DevBuf<uint32_t> d_obs_tot; // updates so far per (bin, walker) (observables with correlators)
     DevBuf<uint32_t> d_dens;    // AT sigma-tau density counts [(hist index) * 1024 + cell] (densities on)
     int nobs = 0;
@@ -205,7 +206,7 @@ void destroy_ctx(rewl_ctx* c) {
     c->d_w_hist.release(); c->d_w_lat.release(); c->d_win_iter.release(); c->d_win_sweeps.release();
     c->d_w_E.release(); c->d_win_lnf.release(); c->d_fields.release();
     c->d_w_cursor.release(); c->d_lat.release(); c->d_visits.release(); c->d_ring.release(); c->d_ring_filled.release();
-    c->d_exclude.release(); c->d_lng.release(); c->d_cnt.release(); c->d_exch.release(); c->d_bound_tab.release(); c->d_obs.release(); c->d_obs_tot.release(); c->d_dens.release();
+    c->d_exclude.release(); c->d_lng.release(); c->d_cnt.release(); c->d_exch.release(); c->d_bound_tab.release(); c->d_obs.release(); c->d_obs_tot.release(); c->d_dens.release(); c->d_stage.release();
     c->d_sys.release(); c->d_win.release(); c->d_grp.release();
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -1309,12 +1310,30 @@ int rewl_get_rng_cursor(rewl_ctx* c, int32_t g, uint64_t* cur) {
     return REWL_OK;
 }
 
+// One walker's ln g column ([bin][lane] layout) from a linear staging copy.
+__global__ void k_scatter_column(double* dst, const double* src, int n) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b < n) dst[(int64_t)b * kLanes] = src[b];
+}
+
 int rewl_set_logdos(rewl_ctx* c, int32_t g, const double* lng, int32_t n) {
     if (!walker_ok(c, g) || !lng) return fail(REWL_ERR_INVALID, "bad walker");
     const int nb = c->win[c->w_window[g]].nbins;
     if (n != nb) return fail(REWL_ERR_INVALID, "ln g length must equal the window's bin count");
-    HIP_TRY(hipMemcpy2DAsync(c->d_lng.p + c->w_hist[g], kLanes * sizeof(double), lng, sizeof(double), sizeof(double),
-                             (size_t)nb, hipMemcpyHostToDevice, c->stream));
+    // a strided copy of single doubles takes milliseconds per walker; a
+    // linear copy into a staging buffer and one scatter launch, microseconds
+    if (c->d_stage.n < (size_t)nb) {
+        c->d_stage.release();
+        const hipError_t e = c->d_stage.alloc((size_t)nb);
+        if (e != hipSuccess) {
+            c->d_stage.release();
+            return fail(REWL_ERR_DEVICE, std::string("staging buffer: ") + hipGetErrorString(e));
+        }
+    }
+    HIP_TRY(hipMemcpyAsync(c->d_stage.p, lng, (size_t)nb * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(k_scatter_column, dim3((nb + 255) / 256), dim3(256), 0, c->stream, c->d_lng.p + c->w_hist[g],
+                       (const double*)c->d_stage.p, nb);
+    HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(c->stream));
     return REWL_OK;
 }

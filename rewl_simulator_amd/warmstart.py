@@ -164,7 +164,7 @@ def dataset(curve_list):
         L = c["L"]
         N = L * L
         m = c["visited"]
-        xs.append(features(c["E"][m], N, L, K_AT, c["sigma"]))
+        xs.append(features(c["E"][m], N, L, c.get("K", K_AT), c["sigma"]))
         ts.append(c["lng"][m] / N)
     return np.concatenate(xs), np.concatenate(ts)
 
@@ -356,6 +356,36 @@ def stage_trace(specs, seconds, mlp=None, lnf_warm=1e-3, every=20, log=print, wi
         summary.append(row)
     return dict(sweeps=checks * spc, seconds=secs, converged=bool(done), start="flat" if mlp is None else "network",
                 per_window_index=summary, reach=reach.tolist(), window=index.tolist(), system=system.tolist())
+
+
+def ising_systems(sizes=(16, 24, 32, 48), bin_size=8.0, replicas=16):
+    """Clean 2D Ising at each L in the layout of the L = 64 workload: bins of
+    width 8 over the full spectrum (top level included, unreachable levels
+    excluded), windows of 75% overlap, L / 2 of them."""
+    return [configs.ising_full_range(L, bin_size, max(4, L // 2), replicas) for L in sizes]
+
+
+def ising_params(lnf_final=1e-6):
+    return configs.params(flatness_mode=abi.FLAT_MIN_OVER_MEAN, flatness=0.8, schedule=abi.SCHED_HALVE,
+                          lnf_final=lnf_final, sweeps_per_check=1000, sweeps_per_exchange=1000, average_mode=1)
+
+
+def ising_curves(sizes=(16, 24, 32, 48), lnf_final=1e-6, max_checks=5000, log=print):
+    """Engine-produced ln g curves of the clean 2D Ising model at each L (one
+    context, every L run to ln f < lnf_final), stitched by the reference's
+    join and normalised to 2^N, as training curves (K = 0, sigma = 0)."""
+    systems = ising_systems(sizes)
+    t0 = time.perf_counter()
+    with Engine(systems, ising_params(lnf_final)) as eng:
+        eng.adjust_to_range()
+        done, secs = run_until_done(eng, max_checks, log=log, every=20, name="ising curves")
+        eng.finalize()
+        out = []
+        for i, L in enumerate(sizes):
+            E, lng = eng.stitched(i, states_per_site=2.0)
+            out.append(dict(L=L, K=0.0, sigma=0.0, seed=0, E=E, lng=lng, visited=np.isfinite(lng)))
+    log(f"ising curves: L = {list(sizes)}, converged={done}, {secs:.1f} s run, {time.perf_counter() - t0:.1f} s total")
+    return out, done, secs
 
 
 def training_specs(n, sizes=(16, 24, 32, 48), sigmas=(0.0, 0.25, 0.5, 0.75, 1.0), seed0=50000):

@@ -73,3 +73,28 @@ def test_failed_adjust_then_reachable_state_adjusts(gpu):
         w = eng.windows[eng.walker_window[g]]
         assert w["e_min"] - 0.5 <= eng.energy(g) < w["e_max"] + 0.5, g
     eng.close()
+
+
+def test_set_logdos_round_trip_leaves_other_walkers(gpu):
+    """rewl_set_logdos writes exactly one walker's column of the [bin][lane]
+    histogram layout (through a staging copy and a scatter launch): it reads
+    back bit for bit, every other walker keeps its ln g, and a second call
+    with a longer window reuses or grows the staging buffer."""
+    print("case: set_logdos round trip", flush=True)
+    systems, params = configs.config1()
+    with Engine(systems + [configs.ising_full_range(24, 8.0, 6, 3, seed=9)], params) as eng:
+        eng.adjust_to_range()
+        eng.sweeps(50)
+        before = [eng.logdos(g) for g in range(eng.num_walkers)]
+        rng = np.random.default_rng(5)
+        # a walker of the second system (129 bins per window), then the first
+        # and last walker of the first (147 bins: the staging buffer grows)
+        picks = [eng.num_walkers - 2, 0, systems[0]["num_windows"] * systems[0]["replicas_per_window"] - 1]
+        new = {}
+        for g in picks:
+            new[g] = rng.normal(0.0, 50.0, size=before[g].size)
+            eng.set_logdos(g, new[g])
+        for g in range(eng.num_walkers):
+            got = eng.logdos(g)
+            np.testing.assert_array_equal(got, new[g] if g in new else before[g])
+        eng.sweeps(5)   # the engine still runs on the imported ln g

@@ -78,3 +78,20 @@ def test_record_stages_first_time_at_or_below_each_power_of_two():
     assert reach[0, :3].tolist() == [250, 500, 500] and reach[0, 3] == -1
     assert reach[1].tolist() == [250, 250] + [-1] * 19
     assert reach[2, 9] == 250 and reach[2, 10] == 500 and reach[2, 11] == -1
+
+
+def test_ising_training_layout_and_features():
+    systems = W.ising_systems((16, 48))
+    assert [s["L"] for s in systems] == [16, 48]
+    assert [s["num_windows"] for s in systems] == [8, 24]
+    for s in systems:
+        N = s["L"] ** 2
+        assert s["bin_size"] == 8.0 and s["e_min"] == -2.0 * N and s["e_max"] == 2.0 * N + 8.0
+        assert s["K"] == 0.0
+    # clean-Ising curves carry K = 0 into the network's inputs; the AT curves
+    # (no "K" key) keep the Ashkin-Teller coupling
+    E = np.array([-512.0, -504.0])
+    x, t = W.dataset([dict(L=16, K=0.0, sigma=0.0, E=E, lng=np.array([0.7, 6.2]), visited=np.array([True, True])),
+                      dict(L=16, sigma=0.5, E=E, lng=np.array([1.0, 2.0]), visited=np.array([True, False]))])
+    assert x.shape == (3, 4) and (x[:2, 2] == 0.0).all() and x[2, 2] == W.K_AT
+    assert np.allclose(t, [0.7 / 256, 6.2 / 256, 1.0 / 256])
