@@ -82,16 +82,16 @@ def parse():
     ap.add_argument("--time-to-lnf", type=float, default=1e-6, metavar="LNF",
                     help="also time the workload from ln f = 1 to LNF and report it in the line (the metric's "
                          "second half); 0 to skip")
-    ap.add_argument("--warm-start-lnf", type=float, default=1e-4, metavar="LNF",
+    ap.add_argument("--warm-start-lnf", type=float, default=0.0, metavar="LNF",
                     help="also run the learned warm start end to end at a small size (training curves, the "
-                         "network, held-out realisations plain vs warm-started to LNF) and report the factor in "
-                         "the line; 0 to skip")
-    ap.add_argument("--warm-start-config2", action="store_true",
-                    help="also time this workload to the --time-to-lnf target from a learned warm start: the network "
-                         "trained on engine-produced clean-Ising curves at L = 16-48, this workload started from its "
-                         "ln g at ln f = 1e-3 (reported beside the plain time_to_lnf leg)")
+                         "network, held-out random-strain realisations at L = 12, plain vs warm-started to LNF, "
+                         "about a minute) and report the factor in the line; 0 (the default) to skip")
+    ap.add_argument("--no-warm-start-config2", dest="warm_start_config2", action="store_false",
+                    help="skip the learned warm start of this workload (by default, with --time-to-lnf, the line also "
+                         "times this workload to that target from the network's ln g at ln f = 1e-3, the network "
+                         "trained on engine-produced clean-Ising curves at L = 16-48: about three minutes)")
     ap.add_argument("--warm-start-source", choices=("network", "exact"), default="network",
-                    help="with --warm-start-config2: seed from the network (the learned warm start) or from the "
+                    help="the warm start of this workload seeds from the network (the learned warm start) or from the "
                          "exact DoS (the bound any start at ln f = 1e-3 can reach)")
     ap.add_argument("--refine-copies", type=int, default=1, metavar="R",
                     help="with --refine: refine in a context with R times the replicas per window, every walker "
