@@ -131,7 +131,7 @@ def calibrate_copy(nbytes):
 
 def host_cpus():
     """The host CPUs this process may use: the cgroup CPU quota when one is set
-    (the GPU box's share of its host), else the affinity mask; with nproc and
+    (this machine's share of its host), else the affinity mask; with nproc and
     the CPU model for the record."""
     info = {"os_cpu_count": os.cpu_count(), "affinity": len(os.sched_getaffinity(0))}
     quota = None

@@ -67,7 +67,7 @@ def build_oracle(verbose=False):
 
 
 REFERENCE_SRC = "/root/reference/src"
-REF_BASELINE = os.path.join(ORACLE_DIR, "_ref", "rewl_ref_ising_L64")
+REF_PROGRAM = os.path.join(ORACLE_DIR, "_ref", "rewl_ref_ising_L64")
 
 
 # The reference program builds timed as CPU baselines: its defaults, and the
@@ -94,7 +94,7 @@ def build_reference(verbose=False):
             print(" ".join(f"{k}={v}" for k, v in dict(REF_NAME=name, **opts).items()) + " " + " ".join(cmd),
                   flush=True)
         subprocess.run(cmd, env=env, check=True, stdout=None if verbose else subprocess.DEVNULL)
-    return REF_BASELINE
+    return REF_PROGRAM
 
 
 def main(argv):

@@ -1,4 +1,4 @@
-"""Python driver of the HIP REWL engine over the C ABI (include/rewl_c.h).
+"""Python front end of the HIP REWL engine over the C ABI (include/rewl_c.h).
 
 Mirrors the reference's REWL_simulation surface (rewl_simulation.hpp:21-66):
 construct from model and REWL parameters, adjust walkers into their windows,
