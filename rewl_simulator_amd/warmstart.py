@@ -314,14 +314,14 @@ def record_stages(reach, lnf, sweeps):
         row[row < 0] = sweeps
 
 
-def stage_trace(specs, seconds, mlp=None, lnf_warm=1e-3, every=20, log=print, windows=8):
+def stage_trace(specs, seconds, mlp=None, lnf_warm=1e-3, every=20, log=print, windows=8, bins=512):
     """Plain (flat ln g, ln f = 1) or, with `mlp`, warm-started (network ln g,
     ln f = lnf_warm) Wang-Landau on the systems for a wall-clock budget,
     recording per window the sweeps after which its ln f was first at or
     below 2^-j, j = 0..20 (2^-20 < 1e-6, the final ln f): the time each
     window spends in each stage.  reach[k, j] = -1 where window k has not got
     there; j at or below the starting stage gets the first check's sweeps."""
-    systems = rfat_systems(specs, windows=windows)
+    systems = rfat_systems(specs, bins=bins, windows=windows)
     params = rfat_params(1e-6)
     spc = params["sweeps_per_check"]
     with Engine(systems, params) as eng:
@@ -422,6 +422,7 @@ def main():
                          "SECONDS, from a flat ln g at ln f = 1 or from the network's at --lnf-warm (--stage-start)")
     ap.add_argument("--stage-start", choices=("flat", "network"), default="network")
     ap.add_argument("--stage-windows", type=int, default=8, help="windows per held-out realisation in --stage-trace")
+    ap.add_argument("--stage-bins", type=int, default=512, help="bins per held-out realisation in --stage-trace")
     ap.add_argument("--refine-first", type=int, default=0, metavar="SWEEPS",
                     help="also run the network's ln g through rounds of multicanonical refinement (first round "
                          "SWEEPS, doubling) until it matches plain Wang-Landau's error (needs --ref-lnf)")
@@ -439,7 +440,7 @@ def main():
     train_specs = training_specs(a.curves, sizes=sizes, sigmas=sigmas)
     held = heldout_specs(a.heldout, L=a.heldout_L, seed0=a.heldout_seed0)
     if a.stage_trace and a.stage_start == "flat":
-        res = stage_trace(held, a.stage_trace, log=log, windows=a.stage_windows)
+        res = stage_trace(held, a.stage_trace, log=log, windows=a.stage_windows, bins=a.stage_bins)
         log(json.dumps(res["per_window_index"]))
         json.dump(res, open(os.path.join(a.out, "stage_trace_flat.json"), "w"))
         return
@@ -459,7 +460,8 @@ def main():
             log(json.dumps(info))
             return
     if a.stage_trace:
-        res = stage_trace(held, a.stage_trace, mlp=mlp, lnf_warm=a.lnf_warm, log=log, windows=a.stage_windows)
+        res = stage_trace(held, a.stage_trace, mlp=mlp, lnf_warm=a.lnf_warm, log=log, windows=a.stage_windows,
+                          bins=a.stage_bins)
         res.update(info)
         log(json.dumps(res["per_window_index"]))
         json.dump(res, open(os.path.join(a.out, "stage_trace_network.json"), "w"))
