@@ -139,7 +139,7 @@ def run_until_done(eng, max_checks, log=None, every=50, name="", max_seconds=Non
     return done, time.perf_counter() - t0
 
 
-def generate(specs, lnf_final=1e-6, batch=256, max_checks=20000, log=print, max_seconds=None):
+def generate(specs, lnf_final=1e-6, batch=256, max_checks=20000, log=print, max_seconds=None, bins=512):
     """Converged curves for every spec, `batch` realisations per context
     (with max_seconds, each context stops after that much wall time and its
     windows keep the ln g they have)."""
@@ -147,7 +147,7 @@ def generate(specs, lnf_final=1e-6, batch=256, max_checks=20000, log=print, max_
     for b in range(0, len(specs), batch):
         part = specs[b:b + batch]
         t0 = time.perf_counter()
-        with Engine(rfat_systems(part), rfat_params(lnf_final)) as eng:
+        with Engine(rfat_systems(part, bins=bins), rfat_params(lnf_final)) as eng:
             eng.adjust_to_range()
             done, secs = run_until_done(eng, max_checks, log=log, name="generate", max_seconds=max_seconds)
             eng.finalize()
@@ -190,13 +190,13 @@ def anchored_max_error(a, b, mask):
     return float(np.max(np.abs(d)))
 
 
-def evaluate(mlp, specs, lnf_final=1e-6, lnf_warm=1e-3, ref_lnf=1e-8, max_checks=40000, log=print):
+def evaluate(mlp, specs, lnf_final=1e-6, lnf_warm=1e-3, ref_lnf=1e-8, max_checks=40000, log=print, bins=512):
     """Plain (flat ln g, ln f = 1) and warm-started (network ln g, ln f =
     lnf_warm) runs of the same held-out systems to ln f < lnf_final.  The
     plain result is the yardstick of the warm one (max |difference| over commonly visited
     bins, anchored); with ref_lnf > 0 both are also compared to a plain run
     taken to ref_lnf."""
-    systems = rfat_systems(specs)
+    systems = rfat_systems(specs, bins=bins)
     res = {}
     ref = None
     if ref_lnf > 0:
@@ -246,7 +246,7 @@ def evaluate(mlp, specs, lnf_final=1e-6, lnf_warm=1e-3, ref_lnf=1e-8, max_checks
     return res
 
 
-def evaluate_refine(mlp, specs, ref, plain, first=2000, sweeps_per_exchange=250, log=print):
+def evaluate_refine(mlp, specs, ref, plain, first=2000, sweeps_per_exchange=250, log=print, bins=512):
     """The network's ln g as fixed weights from the start: rounds of
     multicanonical refinement (rewl_refine; each samples with ln f = 0 and
     then corrects ln g by ln H), the round length doubling from `first`,
@@ -256,7 +256,7 @@ def evaluate_refine(mlp, specs, ref, plain, first=2000, sweeps_per_exchange=250,
     network's prediction replaces the whole Wang-Landau run: without it, the
     flat start's fixed weights would leave the walkers at the most probable
     energies."""
-    systems = rfat_systems(specs)
+    systems = rfat_systems(specs, bins=bins)
     rounds = []
     with Engine(systems, rfat_params(1e-6)) as eng:
         eng.adjust_to_range()
@@ -285,10 +285,10 @@ def evaluate_refine(mlp, specs, ref, plain, first=2000, sweeps_per_exchange=250,
                 speedup_wall=plain["seconds"] / (last["seconds"] + t_init))
 
 
-def budget_run(mlp, specs, checks, lnf_warm=1e-3, log=print):
+def budget_run(mlp, specs, checks, lnf_warm=1e-3, log=print, bins=512):
     """Plain and warm-started runs of the same systems for a fixed number of
     check cycles: the slowest and median window ln f each reaches."""
-    systems = rfat_systems(specs)
+    systems = rfat_systems(specs, bins=bins)
     res = {}
     for name in ("plain", "warm"):
         with Engine(systems, rfat_params(1e-6)) as eng:
@@ -394,6 +394,17 @@ def training_specs(n, sizes=(16, 24, 32, 48), sigmas=(0.0, 0.25, 0.5, 0.75, 1.0)
     return [(sizes[i % len(sizes)], sigmas[(i // len(sizes)) % len(sigmas)], seed0 + i) for i in range(n)]
 
 
+def config5(n=64, bins=512, seed0=900000):
+    """The learned-warm-start workload: n held-out random-strain AT
+    realisations at L = 48 (sigma 0.5), 8 windows x 4 walkers, `bins` bins
+    over [E(all-up), 0), visited-support flatness 0.8, halving to ln f 1e-6;
+    warm-started windows begin at ln f = 1e-3.  At L = 48 bins of 22.5 (the
+    stated 512) are wider than one excitation at the low-energy end and stall
+    the lowest windows; 1024 bins do not.  Returns (specs, systems, params)."""
+    specs = heldout_specs(n, L=48, sigma=0.5, seed0=seed0)
+    return specs, rfat_systems(specs, bins=bins), rfat_params(1e-6)
+
+
 def heldout_specs(n, L=48, sigma=0.5, seed0=900000):
     return [(L, sigma, seed0 + i) for i in range(n)]
 
@@ -422,7 +433,9 @@ def main():
                          "SECONDS, from a flat ln g at ln f = 1 or from the network's at --lnf-warm (--stage-start)")
     ap.add_argument("--stage-start", choices=("flat", "network"), default="network")
     ap.add_argument("--stage-windows", type=int, default=8, help="windows per held-out realisation in --stage-trace")
-    ap.add_argument("--stage-bins", type=int, default=512, help="bins per held-out realisation in --stage-trace")
+    ap.add_argument("--bins", type=int, default=512, help="bins per realisation over [E(all-up), 0)")
+    ap.add_argument("--stage-bins", type=int, default=0, help="bins per held-out realisation in --stage-trace "
+                                                             "(default: --bins)")
     ap.add_argument("--refine-first", type=int, default=0, metavar="SWEEPS",
                     help="also run the network's ln g through rounds of multicanonical refinement (first round "
                          "SWEEPS, doubling) until it matches plain Wang-Landau's error (needs --ref-lnf)")
@@ -440,7 +453,7 @@ def main():
     train_specs = training_specs(a.curves, sizes=sizes, sigmas=sigmas)
     held = heldout_specs(a.heldout, L=a.heldout_L, seed0=a.heldout_seed0)
     if a.stage_trace and a.stage_start == "flat":
-        res = stage_trace(held, a.stage_trace, log=log, windows=a.stage_windows, bins=a.stage_bins)
+        res = stage_trace(held, a.stage_trace, log=log, windows=a.stage_windows, bins=a.stage_bins or a.bins)
         log(json.dumps(res["per_window_index"]))
         json.dump(res, open(os.path.join(a.out, "stage_trace_flat.json"), "w"))
         return
@@ -450,7 +463,7 @@ def main():
         info = dict(mlp_in=a.mlp_in)
     else:
         t0 = time.perf_counter()
-        cl = generate(train_specs, lnf_final=a.lnf_final, log=log, max_seconds=a.generate_seconds or None)
+        cl = generate(train_specs, lnf_final=a.lnf_final, log=log, max_seconds=a.generate_seconds or None, bins=a.bins)
         gen_s = time.perf_counter() - t0
         mlp, hist = train(cl, steps=a.steps, log=log)
         info = dict(generate_seconds=gen_s, train_ms=mlp.last_ms(), curves=len(cl),
@@ -461,19 +474,20 @@ def main():
             return
     if a.stage_trace:
         res = stage_trace(held, a.stage_trace, mlp=mlp, lnf_warm=a.lnf_warm, log=log, windows=a.stage_windows,
-                          bins=a.stage_bins)
+                          bins=a.stage_bins or a.bins)
         res.update(info)
         log(json.dumps(res["per_window_index"]))
         json.dump(res, open(os.path.join(a.out, "stage_trace_network.json"), "w"))
         return
-    res = evaluate(mlp, held, lnf_final=a.lnf_final, lnf_warm=a.lnf_warm, ref_lnf=a.ref_lnf, log=log)
+    res = evaluate(mlp, held, lnf_final=a.lnf_final, lnf_warm=a.lnf_warm, ref_lnf=a.ref_lnf, log=log, bins=a.bins)
     ref = res.pop("_reference_curves")
     if a.refine_first and ref is not None:
-        res["refine_from_network"] = evaluate_refine(mlp, held, ref, res["plain"], first=a.refine_first, log=log)
+        res["refine_from_network"] = evaluate_refine(mlp, held, ref, res["plain"], first=a.refine_first, log=log,
+                                                      bins=a.bins)
     res.update(info)
     res["heldout"] = dict(count=len(held), L=held[0][0], sigma=held[0][1])
     if a.budget_L:
-        res["budget"] = budget_run(mlp, heldout_specs(a.heldout, L=a.budget_L), a.budget_checks, log=log)
+        res["budget"] = budget_run(mlp, heldout_specs(a.heldout, L=a.budget_L), a.budget_checks, log=log, bins=a.bins)
         res["budget"]["L"] = a.budget_L
     log(json.dumps(res))
     json.dump(res, open(os.path.join(a.out, "result.json"), "w"), indent=1)

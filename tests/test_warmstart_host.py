@@ -95,3 +95,15 @@ def test_ising_training_layout_and_features():
                       dict(L=16, sigma=0.5, E=E, lng=np.array([1.0, 2.0]), visited=np.array([True, False]))])
     assert x.shape == (3, 4) and (x[:2, 2] == 0.0).all() and x[2, 2] == W.K_AT
     assert np.allclose(t, [0.7 / 256, 6.2 / 256, 1.0 / 256])
+
+
+def test_config5_layout_and_bin_width():
+    specs, systems, params = W.config5(n=3, bins=1024)
+    assert [s[0] for s in specs] == [48, 48, 48] and all(s[1] == 0.5 for s in specs)
+    for s in systems:
+        assert s["num_windows"] == 8 and s["replicas_per_window"] == 4 and s["e_max"] == 0.0
+        # 1024 bins over [E(all-up), 0) are narrower than one flipped spin (12 +- 2 h_i)
+        assert 11.0 < s["bin_size"] < 11.5
+    _, wide, _ = W.config5(n=1)
+    assert 22.0 < wide[0]["bin_size"] < 23.0     # the stated 512 bins: two excitations wide
+    assert params["lnf_final"] == 1e-6
